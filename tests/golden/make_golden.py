@@ -1,0 +1,127 @@
+"""Write the reference-generated fixtures under tests/golden/ (run in the build container only).
+
+The reference (pure Python/PyTorch) is imported from /root/reference to produce input/output
+vectors for the parts of the IMEX path that closed forms and invariants leave open:
+
+* ``step_trajectory.npz`` -- ChannelFlow.step_imex composition (BC order, AB2 buffers, forcing
+  controller, CN, FFT-Poisson, projection) on a 16x12x16 seeded channel, CPU branch, 3 steps.
+* ``c1_run.npz`` -- the 64x64x48 plain-method case (dt=0.005, CPU branch) for 200 steps:
+  per-step u_bulk and forcing, plus final mean/RMS profiles.
+* ``ops.npz`` -- compute_momentum_rhs_fused_imex (GPU-branch RHS, including the u-face nx /
+  v-face ny behaviour), solve_implicit_diffusion_w (shifted explicit half), and
+  TurbulenceStats.accumulate_statistics on one random field.
+
+Only data is written: inputs and the reference's outputs.  Usage:
+    PYTORCH_JIT=0 MPLBACKEND=Agg python tests/golden/make_golden.py
+"""
+import os
+import sys
+import tempfile
+
+os.environ.setdefault("PYTORCH_JIT", "0")      # the reference assumes eager mode (operators.py:1339)
+os.environ.setdefault("MPLBACKEND", "Agg")
+
+import numpy as np
+import torch
+import yaml
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REF)
+
+import solver as ref_solver                     # noqa: E402
+import operators as ref_ops                     # noqa: E402
+import turbstats as ref_turb                    # noqa: E402
+from utils import generate_grid                 # noqa: E402
+
+torch.set_default_dtype(torch.float64)
+
+
+def channel(tmp, nx, ny, nz, Lx, Ly, seed, n_stats=0, dt=0.005):
+    cfg = {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": Lx, "Ly": Ly, "Lz": 2.0},
+        "flow": {"Re": 2870.0, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": 2.6},
+        "time": {"dt": dt, "n_steps": 1, "CFL_target": 0.25},
+        "initialization": {"type": "parabolic", "perturbation_intensity": 0.1},
+        "output": {"results_folder": os.path.join(tmp, "res"), "n_out": 10, "n_save": 1000},
+        "compute": {"device": "cpu"},
+        "statistics": {"n_stats": n_stats},
+    }
+    path = os.path.join(tmp, "cfg.yaml")
+    with open(path, "w") as f:
+        yaml.safe_dump(cfg, f)
+    torch.manual_seed(seed)
+    return ref_solver.ChannelFlow(path)
+
+
+def np64(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def make_trajectory(tmp):
+    cf = channel(tmp, 16, 12, 16, 4 * np.pi, 4 * np.pi / 3, seed=0)
+    out = {"u0": np64(cf.u), "v0": np64(cf.v), "w0": np64(cf.w), "p0": np64(cf.p)}
+    for s in range(1, 4):
+        ub, f = cf.step_imex(0.005)
+        out[f"u{s}"], out[f"v{s}"], out[f"w{s}"], out[f"p{s}"] = (np64(cf.u), np64(cf.v),
+                                                                    np64(cf.w), np64(cf.p))
+        out[f"ubulk{s}"] = float(ub)
+        out[f"forcing{s}"] = float(f)
+        out[f"rhs_u_prev{s}"] = np64(cf.rhs_u_prev)
+    np.savez_compressed(os.path.join(HERE, "step_trajectory.npz"), **out)
+
+
+def make_c1(tmp):
+    nx, ny, nz = 64, 64, 48
+    cf = channel(tmp, nx, ny, nz, 4 * np.pi, 4 * np.pi / 3, seed=0)
+    ub, fo = [], []
+    for _ in range(200):
+        b, f = cf.step_imex(0.005)
+        ub.append(float(b))
+        fo.append(float(f))
+    u = cf.u[1:nx + 1, 1:ny + 1, 1:nz + 1]
+    v = cf.v[1:nx + 1, 1:ny + 1, 1:nz + 1]
+    w = cf.w[1:nx + 1, 1:ny + 1, 1:nz]
+    div = ref_solver.compute_divergence(cf.u, cf.v, cf.w, nx, ny, nz, cf.dx, cf.dy, cf.dz_f)
+    np.savez_compressed(
+        os.path.join(HERE, "c1_run.npz"),
+        u_bulk=np.array(ub), forcing=np.array(fo),
+        U_mean=np64(u.mean(dim=(0, 1))), u_rms=np64(u.std(dim=(0, 1), unbiased=False)),
+        v_rms=np64(v.std(dim=(0, 1), unbiased=False)), w_rms=np64(w.std(dim=(0, 1), unbiased=False)),
+        max_div=float(div.abs().max()))
+
+
+def make_ops():
+    g = torch.Generator().manual_seed(1234)
+    nx, ny, nz = 12, 10, 14
+    Lx, Ly, Lz, nu = 2.0, 1.5, 2.0, 1.0 / 2870.0
+    dx, dy = Lx / nx, Ly / ny
+    z_f, z_c, dz_f, dz_c = generate_grid(2.6, nz, Lz)
+    u = torch.rand(nx + 1, ny + 2, nz + 2, generator=g) - 0.5
+    v = torch.rand(nx + 2, ny + 1, nz + 2, generator=g) - 0.5
+    w = torch.rand(nx + 2, ny + 2, nz + 1, generator=g) - 0.5
+    ru, rv, rw = ref_ops.compute_momentum_rhs_fused_imex(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
+    w_cn = ref_ops.solve_implicit_diffusion_w(w, torch.tensor(0.01), nx, ny, nz, dz_c, dz_f, nu)
+    stats = ref_turb.TurbulenceStats(nx, ny, nz, Lx, Ly, Lz, z_c, z_f, dz_c, dz_f, dx, dy, nu, 180.0,
+                                     z_plus_target=15.0, device="cpu")
+    stats.accumulate_statistics(u, v, w, torch.tensor(0.05))
+    fin = stats.finalize_statistics()
+    np.savez_compressed(
+        os.path.join(HERE, "ops.npz"),
+        nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=Lz, nu=nu,
+        u=np64(u), v=np64(v), w=np64(w),
+        rhs_u=np64(ru), rhs_v=np64(rv), rhs_w=np64(rw), w_cn=np64(w_cn), cn_dt=0.01,
+        k_bot=stats.k_bot, k_top=stats.k_top,
+        U_mean=fin["U_mean"], uu_mean=fin["uu_mean"], vv_mean=fin["vv_mean"],
+        ww_mean=fin["ww_mean"], uw_mean=fin["uw_mean"], E_uu_2d=fin["E_uu_2d"],
+        E_vv_2d=fin["E_vv_2d"], E_ww_2d=fin["E_ww_2d"], E_uw_2d=fin["E_uw_2d"],
+        u_tau=fin["u_tau"])
+
+
+if __name__ == "__main__":
+    with tempfile.TemporaryDirectory() as tmp:
+        make_ops()
+        make_trajectory(tmp)
+        make_c1(tmp)
+    print("fixtures written to", HERE)
