@@ -1,0 +1,215 @@
+"""Benchmark: Mcell-steps/s of the IMEX step at 192x192x128 on one MI355X.
+
+Workload (the Re_tau~180 production case): 192x192x128, Lx=4pi, Ly=4pi/3, Lz=2, tanh gamma=2.6,
+Re_bulk=2870, constant-flow-rate forcing, adaptive dt at CFL 0.25 (dt update every 10 steps, on
+the device), turbulence statistics every 200 steps; seeded perturbed parabola (synthetic).
+`value` is the plain float64 path; the float32 path (float64 gauge mode, bulk/forcing and
+statistic sums) is reported beside it.  A "step" is one full IMEX step of the whole grid; steps
+run as HIP-graph chunks of 10 (the dt update sits inside the chunk), statistics run between
+chunks, all inside the timed region.
+
+    python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-fp32] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+NX, NY, NZ = 192, 192, 128
+HBM_PEAK_GBS = 8000.0
+PHASES = ["cn_tables", "explicit_ab2_cn", "div_fft_x", "fft_y_fwd", "poisson_pcr", "fft_y_inv",
+          "ifft_x_p", "project_bc_bulk", "cfl_max"]
+
+
+def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
+    return {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": 4 * math.pi, "Ly": 4 * math.pi / 3, "Lz": 2.0},
+        "flow": {"Re": 2870.0, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": 2.6},
+        "time": {"dt": 0.005, "n_steps": steps, "CFL_target": 0.25, "dt_update_interval": 10,
+                 "dt_max": 0.01, "dt_min": 1e-4},
+        "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": 0},
+        "compute": {"device": "cuda", "precision": precision},
+        "statistics": {"n_stats": 200, "t_stats": 0.0, "z_plus_target": 15.0},
+    }
+
+
+def algorithmic_words(nx, ny, nz):
+    """Compulsory HBM words per interior cell for each kernel (ghost cells excluded)."""
+    spec = 2.0 * (nx // 2 + 1) / nx          # half-spectrum complex words per cell
+    return {
+        "cn_tables": 0.0,
+        "explicit_ab2_cn": 12.0,              # read u, v, w, rhs_prev x3; write rhs x3, q* x3
+        "div_fft_x": 3.0 + spec,              # read q* x3; write x half spectra
+        "fft_y_fwd": 2 * spec,
+        "poisson_pcr": 2 * spec,
+        "fft_y_inv": 2 * spec,
+        "ifft_x_p": spec + 1.0,               # read spectra; write p
+        "project_bc_bulk": 7.0,               # read q* x3, p; write u, v, w
+        "cfl_max": 3.0,
+    }
+
+
+def run_gpu(precision, steps, warmup, phase_reps=20):
+    import torch
+    import torchannel_amd as tc
+    from torchannel_amd import _lib as L
+
+    ch = tc.ChannelFlow(config=config(precision, steps=steps))
+    interval = ch.dt_update_interval
+    # warm-up: forward-Euler start then AB2, eager, with the reference cadence
+    step = 0
+    for _ in range(warmup):
+        step += 1
+        if step % interval == 0:
+            ch.update_dt_device()
+        ch.step_imex(None)
+    ch.current_step = step
+    chunk = 10
+    n_chunks = steps // chunk
+    ch.run_graph(0, chunk, dt_update_interval=interval)      # capture only
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record()
+    done = 0
+    for _ in range(n_chunks):
+        ch.run_graph(1, chunk, dt_update_interval=interval)
+        done += chunk
+        if (warmup + done) % ch.n_stats == 0:
+            ch.turbulence_stats.accumulate_statistics(ch.u, ch.v, ch.w)
+    t1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0.elapsed_time(t1)
+    n_cells = NX * NY * NZ
+    out = {
+        "mcell_steps_per_s": n_cells * done / (ms * 1e-3) / 1e6,
+        "ms_per_step": ms / done,
+        "wall_s": wall,
+        "steps": done,
+        "dt_final": ch.device_dt(),
+        "u_bulk": float(ch._scal[L.S_UBULK]),
+        "stats_samples": ch.turbulence_stats.n_samples,
+    }
+    # per-kernel average duration, HIP events on the launch stream
+    per = {}
+    words = algorithmic_words(NX, NY, NZ)
+    wsize = 8 if precision == "fp64" else 4
+    for i, name in enumerate(PHASES):
+        L.call("tc_solver_phase", ch._solver, i, 2, ch._stream())
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        L.call("tc_solver_phase", ch._solver, i, phase_reps, ch._stream())
+        e1.record()
+        torch.cuda.synchronize()
+        avg_ms = e0.elapsed_time(e1) / phase_reps
+        byts = words[name] * wsize * n_cells
+        per[name] = {"avg_us": avg_ms * 1e3, "alg_bytes": byts,
+                     "gbs": byts / (avg_ms * 1e-3) / 1e9 if byts else 0.0}
+    out["kernels"] = per
+    step_kernels = [k for k in PHASES if k not in ("cfl_max",)]
+    dom = max(step_kernels, key=lambda k: per[k]["avg_us"])
+    out["dominant"] = dom
+    tot_bytes = sum(words[k] * wsize * n_cells for k in step_kernels)
+    out["step_alg_bytes"] = tot_bytes
+    out["step_gbs"] = tot_bytes / (out["ms_per_step"] * 1e-3) / 1e9
+    del ch
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(budget_s=20.0):
+    """The float64 CPU oracle (a port of the reference op sequence, GPU-branch RHS) on the
+    host cores, on a bounded sample of the same 192x192x128 workload at fixed dt."""
+    import torch
+    from oracle import channel_oracle as O
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    ch = O.Channel(NX, NY, NZ, 4 * math.pi, 4 * math.pi / 3, seed=0, gauge_zero_mode=True)
+    ch.step_imex(0.005)                       # warm-up (forward-Euler start)
+    n = 0
+    t0 = time.perf_counter()
+    while n < 10 and (time.perf_counter() - t0) < budget_s:
+        ch.step_imex(0.005)
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": NX * NY * NZ * n / dt / 1e6, "unit": "Mcell-steps/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": f"{n} float64 IMEX steps of the 192x192x128 channel (after 1 warm-up step), "
+                      f"fixed dt=0.005, torch CPU, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only (the IMEX step of one channel on one "
+              "MI355X); multi-GPU runs are not part of it.", flush=True)
+        sys.exit(2)
+    steps = max(10, (args.steps // 10) * 10)
+    warmup = max(10, (args.warmup // 10) * 10)
+    print(f"[bench] {args.precision} {NX}x{NY}x{NZ} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
+    main_run = run_gpu(args.precision, steps, warmup)
+    print(f"[bench] {args.precision}: {main_run['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
+    other = None
+    if not args.no_fp32 and args.precision == "fp64":
+        other = run_gpu("fp32", steps, warmup)
+        print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+        print(f"[bench] cpu oracle: {cpu['value']:.2f} Mcell-steps/s", file=sys.stderr, flush=True)
+    dom = main_run["dominant"]
+    k = main_run["kernels"][dom]
+    line = {
+        "metric": "Mcell-steps/s at 192x192x128 (fp32 vs fp64 plain); adjoint/forward step cost",
+        "value": round(main_run["mcell_steps_per_s"], 2),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(main_run["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic (seeded perturbed laminar parabola)",
+        "config": {"workload": "Re_tau~180 production step: 192x192x128, Lx=4pi, Ly=4pi/3, gamma=2.6, "
+                               "Re_bulk=2870, adaptive dt CFL 0.25 every 10 steps, constant flow rate, "
+                               "turbstats every 200 steps",
+                   "precision": args.precision, "graph_chunk_steps": 10},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_us": round(k["avg_us"], 2)},
+        "step_roofline": {"alg_bytes_per_step": main_run["step_alg_bytes"],
+                          "achieved_gbs": round(main_run["step_gbs"], 1),
+                          "frac": round(main_run["step_gbs"] / HBM_PEAK_GBS, 4)},
+        "kernels_us": {n: round(v["avg_us"], 2) for n, v in main_run["kernels"].items()},
+        "cpu_baseline": cpu,
+    }
+    if other is not None:
+        line["fp32"] = {"value": round(other["mcell_steps_per_s"], 2), "ms_per_step": round(other["ms_per_step"], 5),
+                        "speedup_vs_fp64": round(other["mcell_steps_per_s"] / main_run["mcell_steps_per_s"], 3),
+                        "kernels_us": {n: round(v["avg_us"], 2) for n, v in other["kernels"].items()},
+                        "step_gbs": round(other["step_gbs"], 1)}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,165 @@
+/* torchannel.h -- C ABI of the MI355X (gfx950) IMEX time step of torChannel.
+ *
+ * Every entry point takes plain device pointers to the reference's ghosted staggered arrays
+ * (z-contiguous C order, the layout torch.zeros(nx+1, ny+2, nz+2) gives):
+ *
+ *   u (nx+1, ny+2, nz+2)   v (nx+2, ny+1, nz+2)   w (nx+2, ny+2, nz+1)   p (nx+2, ny+2, nz+2)
+ *   div (nx, ny, nz)
+ *
+ * in float64 (TC_F64) or float32 (TC_F32), a hipStream_t passed as `void*`, and returns 0 on
+ * success, a negative hipError_t on a HIP failure, or a TC_ERR_* code.  tc_strerror() maps
+ * codes to text.  The library allocates only its own plans and workspaces (freed by the
+ * matching *_destroy); it never allocates or frees caller memory.  Calls that share a grid,
+ * Poisson plan or solver object must be issued on one stream at a time.
+ *
+ * Reference citations are to gmcavallazzi/torChannel (file:line).
+ */
+#ifndef TORCHANNEL_H
+#define TORCHANNEL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- codes and enums ------------------------------------------------------------------- */
+#define TC_OK 0
+#define TC_ERR_ARG (-1000)          /* invalid argument */
+#define TC_ERR_SHAPE (-1001)        /* unsupported grid size */
+#define TC_ERR_DTYPE (-1002)        /* unsupported dtype */
+#define TC_ERR_UNSUPPORTED (-1003)  /* size outside the compiled kernel range */
+#define TC_ERR_NOMEM (-1004)        /* library workspace allocation failed */
+
+#define TC_TOP_DIRICHLET 0          /* no-slip top wall  (solver.py:53-54) */
+#define TC_TOP_NEUMANN 1            /* free-slip top wall (solver.py:51-52) */
+
+#define TC_BRANCH_GPU 0             /* fused explicit RHS (solver.py:593-600) */
+#define TC_BRANCH_CPU 1             /* separate RHS kernels (solver.py:601-622) */
+
+#define TC_F64 0
+#define TC_F32 1
+
+#define TC_NSCAL 8                  /* length of a solver's float64 scalar block */
+
+typedef struct tc_grid tc_grid;
+typedef struct tc_poisson tc_poisson;
+typedef struct tc_solver tc_solver;
+
+const char* tc_strerror(int code);
+int tc_version(void);
+
+/* ---- grid -------------------------------------------------------------------------------
+ * Extents, spacings, viscosity and the wall-normal metric (host float64 arrays dz_f[nz],
+ * dz_c[nz+1] as generate_grid returns them, utils.py:5-45).  Replaces the (nx, ny, nz, dx, dy,
+ * dz_c, dz_f, nu, top_wall_bc_type) argument tuple threaded through operators.py:1038-1044 and
+ * the ChannelFlow attributes of solver.py:111-234.  nz in [4, 256]; nx, ny even, of the form
+ * 2^a 3^b 5^c, in [4, 1536] for the Poisson plan and the solver. */
+int tc_grid_create(int nx, int ny, int nz, double dx, double dy, double nu, int top_bc, int dtype,
+                   const double* dz_f_host, const double* dz_c_host, tc_grid** out);
+int tc_grid_destroy(tc_grid* g);
+
+/* apply_bc_all (solver.py:23-79): periodic x/y images, then wall reflections, in place. */
+int tc_apply_bc(const tc_grid* g, void* u, void* v, void* w, void* stream);
+
+/* compute_momentum_rhs_fused_imex (operators.py:1037-1225) for branch TC_BRANCH_GPU, or
+ * diffusion_xy_* - advection_* (operators.py:126-247, 474-548) for TC_BRANCH_CPU.  Writes
+ * every entry of ru, rv, rw (zero outside the interior). */
+int tc_rhs_imex(const tc_grid* g, int branch, const void* u, const void* v, const void* w,
+                void* ru, void* rv, void* rw, void* stream);
+
+/* solve_implicit_diffusion_u / _v / _w (operators.py:563-811) for comp 0 / 1 / 2: the interior
+ * columns of q_out receive the theta-scheme solution for q_in; other entries of q_out are not
+ * written (the reference returns a clone, operators.py:580).  dt is a host value, or, when
+ * dt_dev is non-null, read from device memory (a 0-D tensor, solver.py:659). */
+int tc_cn_solve(const tc_grid* g, int comp, double dt, const double* dt_dev, double theta,
+                const void* q_in, void* q_out, void* stream);
+
+/* pcr_solve (tridiag.py:33-58) / solve_tridiagonal_batch (operators.py:551-560) /
+ * solve_tridiagonal (projection_fft.py:167-215): batch systems of n rows,
+ * a[i] x[i-1] + b[i] x[i] + c[i] x[i+1] = d[i].  a, b, c are real with a batch stride of 0
+ * (shared rows) or n; d and x are (batch, n), complex interleaved when is_complex.
+ * n in [1, 512]. */
+int tc_pcr_batched(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
+                   const void* c, int64_t abc_stride, const void* d, void* x, void* stream);
+
+/* compute_divergence (utils.py:226-240). */
+int tc_divergence(const tc_grid* g, const void* u, const void* v, const void* w, void* div,
+                  void* stream);
+
+/* project_velocity (projection.py:143-165), in place on the interior; dt from dt_dev when
+ * non-null, else the host value. */
+int tc_project(const tc_grid* g, void* u, void* v, void* w, const void* p, double dt,
+               const double* dt_dev, void* stream);
+
+/* compute_bulk_velocity (utils.py:207-223): sum(u[1:nx+1,1:ny+1,1:nz+1] * wgt) / total_volume
+ * with wgt a float64 (nx, ny, nz) view given by element strides (0 for expanded dims).
+ * Result (float64) to *out_dev. */
+int tc_bulk_velocity(const tc_grid* g, const void* u, const double* wgt, int64_t ws0, int64_t ws1,
+                     int64_t ws2, double total_volume, double* out_dev, void* stream);
+
+/* compute_cfl_fused (operators.py:1232-1333): the largest convective rate, to *out_dev. */
+int tc_cfl_rate(const tc_grid* g, const void* u, const void* v, const void* w, double* out_dev,
+                void* stream);
+
+/* compute_u_tau (utils.py:176-204) with dist = z_c[1]; result to *out_dev. */
+int tc_u_tau(const tc_grid* g, const void* u, double z_c1, double* out_dev, void* stream);
+
+/* ---- FFT + tridiagonal pressure Poisson solve ------------------------------------------
+ * initialize_fft_solver (projection_fft.py:3-99): plan with modified wavenumbers, per-mode
+ * tridiagonal rows and the spectral workspace.  tc_poisson_solve is solve_poisson_fft
+ * (projection_fft.py:101-165): p (ghosts included) from rhs (nx, ny, nz).  The singular
+ * (kx,ky)=(0,0) mode is gauge-fixed in float64 (its first unknown pinned to zero); p is defined
+ * up to an additive constant, as in the reference. */
+int tc_poisson_create(const tc_grid* g, tc_poisson** out);
+int tc_poisson_destroy(tc_poisson* pl);
+int tc_poisson_solve(tc_poisson* pl, const void* rhs, void* p, void* stream);
+
+/* ---- turbulence statistics (turbstats.py:101-228) ------------------------------------------
+ * Adds one sample to float64 running sums: prof[5*nz] = U, uu, vv, ww, uw profiles and
+ * spec[4*(nx/2)*(ny/2)] = E_uu, E_vv, E_ww, E_uw folded 2-D spectra sampled at the planes
+ * k_bot, k_top (1-based cell indices, turbstats.py:64-69). */
+int tc_stats_accumulate(tc_poisson* pl, const void* u, const void* v, const void* w, int k_bot,
+                        int k_top, double* prof, double* spec, void* stream);
+
+/* ---- fused IMEX step (ChannelFlow.step_imex, solver.py:580-710) ----------------------------
+ * The solver owns its workspaces and advances caller-owned state in place:
+ *   u, v, w, p   ghosted fields (updated every step, ghosts included);
+ *   rhs_a/rhs_b  two sets of three rhs arrays shaped like u, v, w (AB2 history, swapped each
+ *                step exactly like rhs_*_prev / rhs_*_curr, solver.py:629-651);
+ *   scal         float64[TC_NSCAL] device scalar block: dt, forcing F, u_bulk, time, cfl rate,
+ *                U_bulk target, step count (see TC_S_* in the library sources).
+ * body (optional, shaped like u) is an additive body force next to F in rhs_u.
+ * Volume V = Lx*Ly*Lz (solver.py:234).  branch selects the explicit-RHS semantics of the
+ * reference's device branches (TC_BRANCH_GPU: solver.py:593-600, TC_BRANCH_CPU: 601-622). */
+int tc_solver_create(const tc_grid* g, tc_poisson* pl, void* u, void* v, void* w, void* p,
+                     void* const* rhs_a, void* const* rhs_b, double* scal, double volume,
+                     int branch, tc_solver** out);
+int tc_solver_destroy(tc_solver* s);
+/* One IMEX step.  dt > 0 sets scal[dt] first; dt <= 0 keeps the device value.  first != 0 is
+ * the forward-Euler start (no rhs history).  Returns which rhs set now holds the newest rhs
+ * (0 = rhs_a, 1 = rhs_b) through *newest (may be null). */
+int tc_solver_step(tc_solver* s, double dt, int first, const void* body, int* newest, void* stream);
+/* The initial projection of ChannelFlow.__init__ (solver.py:342-369): div -> Poisson with
+ * dt = 1 -> project -> boundary conditions. */
+int tc_solver_project_initial(tc_solver* s, void* stream);
+/* Adaptive dt on the device (solver.py:465-496 and the 5% rule of 807-812): reads the state,
+ * writes the new dt into scal when it changed by more than 5%. */
+int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream);
+/* Capture `steps` IMEX steps (even; AB2 history must exist) into a HIP graph, with the adaptive
+ * dt update before every `dt_interval`-th step of the chunk when dt_interval > 0.  Replay
+ * advances the state by that many steps on `stream`. */
+int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_target, double dt_min,
+                      double dt_max, void* stream);
+int tc_solver_replay(tc_solver* s, void* stream);
+/* Launch one kernel of the step `reps` times back to back, for per-kernel timing (no reference
+ * counterpart): 0 CN tables, 1 explicit+AB2+CN, 2 forward x FFT (with divergence), 3 forward y
+ * FFT, 4 per-mode PCR, 5 inverse y FFT, 6 inverse x FFT (+ p ghosts), 7 projection + boundary +
+ * bulk, 8 CFL reduction.  Overwrites the step's intermediates; needs a step taken first. */
+int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TORCHANNEL_H */

@@ -1,0 +1,153 @@
+"""Whole IMEX steps on the HIP path against the oracle and against the reference's own
+trajectories (tests/golden): per step, per field, <= 1e-12 relative in float64."""
+import math
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import channel_oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel(a, b):
+    a = torch.as_tensor(a).detach().cpu().double()
+    b = torch.as_tensor(b).detach().cpu().double()
+    den = float(b.abs().max())
+    return float((a - b).abs().max()) / (den if den > 0 else 1.0)
+
+
+def cfg(nx, ny, nz, Lx=4 * math.pi, Ly=4 * math.pi / 3, dt=0.005, seed=0, branch="gpu", precision="fp64",
+        amp=0.1, top="dirichlet"):
+    return {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": Lx, "Ly": Ly, "Lz": 2.0},
+        "flow": {"Re": 2870.0, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": 2.6},
+        "time": {"dt": dt, "n_steps": 10, "CFL_target": 0.25},
+        "initialization": {"type": "parabolic", "perturbation_intensity": amp, "seed": seed},
+        "compute": {"device": "cuda", "precision": precision, "rhs_branch": branch},
+        "boundary_conditions": {"top_wall": {"type": top}},
+    }
+
+
+@pytest.fixture(scope="module")
+def tc():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import torchannel_amd
+    torchannel_amd._lib.load()
+    return torchannel_amd
+
+
+def _pmod(p):
+    p = torch.as_tensor(p).double().cpu()
+    return p - p.mean()
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 16), (32, 24, 48), (24, 18, 130)])
+@pytest.mark.parametrize("top", ["dirichlet", "neumann"])
+def test_steps_match_oracle(tc, shape, top):
+    nx, ny, nz = shape
+    print(f"[step] shape={shape} top={top}", flush=True)
+    ch = tc.ChannelFlow(config=cfg(nx, ny, nz, top=top))
+    ref = O.Channel(nx, ny, nz, 4 * math.pi, 4 * math.pi / 3, seed=0, branch="gpu", gauge_zero_mode=True,
+                    top_wall_bc_type=top)
+    for q, r in ((ch.u, ref.u), (ch.v, ref.v), (ch.w, ref.w)):
+        assert rel(q, r) < TOL
+    for s in range(6):
+        ub, f = ch.step_imex(0.005)
+        rub, rf = ref.step_imex(0.005)
+        for q, r, name in ((ch.u, ref.u, "u"), (ch.v, ref.v, "v"), (ch.w, ref.w, "w")):
+            assert rel(q, r) < TOL, (name, s)
+        assert rel(_pmod(ch.p), _pmod(ref.p)) < 1e-10, ("p", s)
+        assert abs(float(ub) - float(rub)) < 1e-13
+        assert abs(float(f) - float(rf)) < 1e-10 * max(1.0, abs(float(rf)))
+        assert rel(ch.rhs_u_prev, ref.rhs_prev[0]) < TOL
+        div = tc.compute_divergence(ch.u, ch.v, ch.w, nx, ny, nz, ch.dx, ch.dy, ch.dz_f)
+        assert float(div.abs().max()) < 1e-10
+
+
+def test_three_steps_match_reference_trajectory(tc, golden_dir):
+    """The reference's own ChannelFlow (CPU branch) on a seeded 16x12x16 channel."""
+    g = np.load(os.path.join(golden_dir, "step_trajectory.npz"))
+    ch = tc.ChannelFlow(config=cfg(16, 12, 16, branch="cpu"))
+    for s in range(4):
+        if s > 0:
+            ub, f = ch.step_imex(0.005)
+            assert abs(float(ub) - float(g[f"ubulk{s}"])) < 1e-13
+            assert abs(float(f) - float(g[f"forcing{s}"])) < 1e-10
+            assert rel(ch.rhs_u_prev, g[f"rhs_u_prev{s}"]) < 1e-11
+        for q, key in ((ch.u, "u"), (ch.v, "v"), (ch.w, "w")):
+            assert rel(q, g[f"{key}{s}"]) < TOL, (key, s)
+        assert rel(_pmod(ch.p), _pmod(g[f"p{s}"])) < 1e-9
+
+
+def test_c1_200_steps_match_reference(tc, golden_dir):
+    """BASELINE config 1 (64x64x48, dt = 0.005, 200 steps, CPU branch) against the reference's
+    own run: bulk velocity every step, mean and RMS profiles at the end (<= 1e-9)."""
+    g = np.load(os.path.join(golden_dir, "c1_run.npz"))
+    nx, ny, nz = 64, 64, 48
+    ch = tc.ChannelFlow(config=cfg(nx, ny, nz, branch="cpu"))
+    ubs = []
+    for _ in range(200):
+        ub, _ = ch.step_imex(0.005)
+        ubs.append(ub)
+    ubs = torch.stack(ubs).cpu().numpy()
+    assert np.abs(ubs - g["u_bulk"]).max() < 1e-12
+    u = ch.u[1:nx + 1, 1:ny + 1, 1:nz + 1].cpu()
+    w = ch.w[1:nx + 1, 1:ny + 1, 1:nz].cpu()
+    v = ch.v[1:nx + 1, 1:ny + 1, 1:nz + 1].cpu()
+    assert rel(u.mean(dim=(0, 1)), g["U_mean"]) < 1e-9
+    assert rel(u.std(dim=(0, 1), unbiased=False), g["u_rms"]) < 1e-9
+    assert rel(v.std(dim=(0, 1), unbiased=False), g["v_rms"]) < 1e-9
+    assert rel(w.std(dim=(0, 1), unbiased=False), g["w_rms"]) < 1e-9
+    div = tc.compute_divergence(ch.u, ch.v, ch.w, nx, ny, nz, ch.dx, ch.dy, ch.dz_f)
+    assert float(div.abs().max()) < 1e-10
+
+
+def test_graph_replay_equals_eager(tc):
+    nx, ny, nz = 32, 24, 48
+    a = tc.ChannelFlow(config=cfg(nx, ny, nz))
+    b = tc.ChannelFlow(config=cfg(nx, ny, nz))
+    for ch in (a, b):
+        ch.step_imex(0.005)
+    for _ in range(2):
+        a.run_graph(1, 10, dt_update_interval=5)
+        torch.cuda.synchronize()
+        for i in range(10):
+            if (i + 1) % 5 == 0:
+                b.update_dt_device()
+            b.step_imex(None)
+        for q, r in ((a.u, b.u), (a.v, b.v), (a.w, b.w), (a.p, b.p), (a._scal, b._scal)):
+            assert torch.equal(q, r)
+
+
+def test_adaptive_dt_rule_matches_oracle(tc):
+    nx, ny, nz = 32, 24, 48
+    ch = tc.ChannelFlow(config=cfg(nx, ny, nz, amp=0.5))
+    ref = O.Channel(nx, ny, nz, 4 * math.pi, 4 * math.pi / 3, seed=0, perturbation=0.5, gauge_zero_mode=True)
+    host = ch.compute_cfl_dt()
+    expect = ref.cfl_dt(0.25)
+    assert abs(host - expect) < 1e-13 * expect
+    ch.update_dt_device()
+    dev = ch.device_dt()
+    want = expect if abs(expect - 0.005) / 0.005 > 0.05 else 0.005
+    assert abs(dev - want) < 1e-13 * want
+
+
+def test_fp32_bulk_and_wall_shear_track_fp64(tc):
+    """FP32 stencils/FFT/PCR with float64 gauge mode, bulk reduction and forcing: u_b and u_tau
+    within 1e-5 of the float64 oracle after 50 steps (64x64x48)."""
+    nx, ny, nz = 64, 64, 48
+    ch = tc.ChannelFlow(config=cfg(nx, ny, nz, precision="fp32"))
+    ref = O.Channel(nx, ny, nz, 4 * math.pi, 4 * math.pi / 3, seed=0, gauge_zero_mode=True)
+    for _ in range(50):
+        ub, _ = ch.step_imex(0.005)
+        rub, _ = ref.step_imex(0.005)
+    assert abs(float(ub) - float(rub)) < 1e-5
+    ut = tc.compute_u_tau(ch.u, ch.z_c, ch.nu)
+    assert abs(float(ut) - float(O.compute_u_tau(ref.u, ref.z_c, ref.nu))) < 1e-5
+    assert rel(ch.u.double(), ref.u) < 1e-4

@@ -1,0 +1,191 @@
+"""ctypes binding of libtorchannel_hip.so (the C ABI declared in include/torchannel.h).
+
+Every product entry point in this package goes through here.  There is no CPU path: a tensor
+that is not on a HIP device, or a missing library, raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+import weakref
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtorchannel_hip.so")
+
+TC_TOP_DIRICHLET, TC_TOP_NEUMANN = 0, 1
+TC_BRANCH_GPU, TC_BRANCH_CPU = 0, 1
+TC_F64, TC_F32 = 0, 1
+TC_NSCAL = 8
+# scalar-block slots (tc_internal.h)
+S_DT, S_F, S_UBULK, S_TIME, S_CFL, S_UTARGET, S_STEP = 0, 1, 2, 3, 4, 5, 6
+
+_vp, _i, _d, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
+_pd = ctypes.POINTER(ctypes.c_double)
+
+# name -> argtypes (every entry point returns int)
+SIGNATURES = {
+    "tc_version": [],
+    "tc_grid_create": [_i, _i, _i, _d, _d, _d, _i, _i, _pd, _pd, ctypes.POINTER(_vp)],
+    "tc_grid_destroy": [_vp],
+    "tc_apply_bc": [_vp, _vp, _vp, _vp, _vp],
+    "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tc_cn_solve": [_vp, _i, _d, _vp, _d, _vp, _vp, _vp],
+    "tc_pcr_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
+    "tc_divergence": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "tc_project": [_vp, _vp, _vp, _vp, _vp, _d, _vp, _vp],
+    "tc_bulk_velocity": [_vp, _vp, _vp, _i64, _i64, _i64, _d, _vp, _vp],
+    "tc_cfl_rate": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "tc_u_tau": [_vp, _vp, _d, _vp, _vp],
+    "tc_poisson_create": [_vp, ctypes.POINTER(_vp)],
+    "tc_poisson_destroy": [_vp],
+    "tc_poisson_solve": [_vp, _vp, _vp, _vp],
+    "tc_stats_accumulate": [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp],
+    "tc_solver_create": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _d,
+                         _i, ctypes.POINTER(_vp)],
+    "tc_solver_destroy": [_vp],
+    "tc_solver_step": [_vp, _d, _i, _vp, ctypes.POINTER(_i), _vp],
+    "tc_solver_project_initial": [_vp, _vp],
+    "tc_solver_update_dt": [_vp, _d, _d, _d, _vp],
+    "tc_solver_capture": [_vp, _i, _i, _d, _d, _d, _vp],
+    "tc_solver_replay": [_vp, _vp],
+    "tc_solver_phase": [_vp, _i, _i, _vp],
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the HIP library once; raise if it was never built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(f"torchannel_amd: {LIB_PATH} is missing; build it with "
+                                   "`python -m torchannel_amd.build` (HIP/gfx950)")
+            lib = ctypes.CDLL(LIB_PATH)
+            for name, args in SIGNATURES.items():
+                fn = getattr(lib, name)
+                fn.argtypes = args
+                fn.restype = ctypes.c_int
+            lib.tc_strerror.argtypes = [ctypes.c_int]
+            lib.tc_strerror.restype = ctypes.c_char_p
+            _lib = lib
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().tc_strerror(rc).decode()
+        raise RuntimeError(f"{msg} (code {rc})")
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args))
+
+
+def ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def stream_of(t: torch.Tensor):
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def require_device(*tensors):
+    """The product path runs only on HIP devices, on contiguous float32/float64 tensors."""
+    dt = None
+    for t in tensors:
+        if t is None:
+            continue
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
+            raise RuntimeError("torchannel_amd: tensors must live on a HIP device (no CPU path)")
+        if not t.is_contiguous():
+            raise RuntimeError("torchannel_amd: tensors must be contiguous (reference C-order layout)")
+        if t.dtype not in (torch.float64, torch.float32):
+            raise RuntimeError(f"torchannel_amd: unsupported dtype {t.dtype}")
+        if dt is None:
+            dt = t.dtype
+        elif t.dtype != dt:
+            raise RuntimeError("torchannel_amd: mixed dtypes")
+    load()
+    return dt
+
+
+def dtype_code(dt):
+    return TC_F64 if dt == torch.float64 else TC_F32
+
+
+def top_code(top_wall_bc_type: str) -> int:
+    if top_wall_bc_type not in ("dirichlet", "neumann"):
+        raise ValueError(f"Invalid top wall BC type: {top_wall_bc_type}. Must be 'dirichlet' or 'neumann'")
+    return TC_TOP_NEUMANN if top_wall_bc_type == "neumann" else TC_TOP_DIRICHLET
+
+
+class Grid:
+    """Owns a tc_grid (extents, spacings, metric tables on the device)."""
+
+    def __init__(self, nx, ny, nz, dx, dy, nu, top, dtype, dz_f_host, dz_c_host, device):
+        lib = load()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dtype = dtype
+        self.device = device
+        f = (ctypes.c_double * nz)(*dz_f_host)
+        c = (ctypes.c_double * (nz + 1))(*dz_c_host)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(device):
+            check(lib.tc_grid_create(nx, ny, nz, float(dx), float(dy), float(nu), top, dtype_code(dtype),
+                                     f, c, ctypes.byref(h)))
+        self.handle = h
+        self._poisson = None
+
+    def poisson(self):
+        if self._poisson is None:
+            h = ctypes.c_void_p()
+            with torch.cuda.device(self.device):
+                check(load().tc_poisson_create(self.handle, ctypes.byref(h)))
+            self._poisson = h
+        return self._poisson
+
+    def __del__(self):
+        try:
+            lib = load()
+            if self._poisson is not None:
+                lib.tc_poisson_destroy(self._poisson)
+            lib.tc_grid_destroy(self.handle)
+        except Exception:
+            pass
+
+
+_grid_cache: dict = {}
+
+
+def _metric_host(t, n):
+    if t is None:
+        return [1.0] * n
+    return [float(x) for x in t.detach().to("cpu", torch.float64).reshape(-1)[:n].tolist()]
+
+
+def get_grid(nx, ny, nz, dx, dy, nu, top, dtype, device, dz_f=None, dz_c=None) -> Grid:
+    """Cached Grid for these arguments.  The metric tensors are identified by object (weak
+    reference) and version counter, so an in-place change or a new tensor rebuilds the grid."""
+    key = (int(nx), int(ny), int(nz), float(dx), float(dy), float(nu), int(top), dtype, str(device),
+           id(dz_f), getattr(dz_f, "_version", -1), id(dz_c), getattr(dz_c, "_version", -1))
+    ent = _grid_cache.get(key)
+    if ent is not None:
+        g, rf, rc = ent
+        if (dz_f is None or rf() is dz_f) and (dz_c is None or rc() is dz_c):
+            return g
+    g = Grid(int(nx), int(ny), int(nz), dx, dy, nu, top, dtype, _metric_host(dz_f, nz),
+             _metric_host(dz_c, nz + 1), device)
+    rf = weakref.ref(dz_f) if dz_f is not None else (lambda: None)
+    rc = weakref.ref(dz_c) if dz_c is not None else (lambda: None)
+    if len(_grid_cache) > 64:
+        _grid_cache.clear()
+    _grid_cache[key] = (g, rf, rc)
+    return g

@@ -1,0 +1,458 @@
+// C ABI (include/torchannel.h): grid, Poisson plan and solver objects, the per-operator entry
+// points and the fused IMEX step with optional HIP-graph capture.
+#include <new>
+
+#include "tc_internal.h"
+#include "tc_poisson.h"
+
+namespace tc {
+int launch_stats(const Poisson* pl, const void* u, const void* v, const void* w, int kb, int kt,
+                 double* prof, double* spec, double* means, double* partials, unsigned* counter,
+                 int nblocks, hipStream_t st);
+}
+
+using namespace tc;
+
+#define TC_HIP(x)                                 \
+  do {                                            \
+    hipError_t e_ = (x);                          \
+    if (e_ != hipSuccess) return -(int)e_;        \
+  } while (0)
+#define TC_RC(x)                                  \
+  do {                                            \
+    int r_ = (x);                                 \
+    if (r_ != TC_OK) return r_;                   \
+  } while (0)
+
+static const int kMaxRedBlocks = 1024;   // partial slots for the stand-alone reductions
+static const int kStatsBlocks = 256;
+
+struct tc_grid {
+  Grid g;
+};
+
+struct tc_poisson {
+  Poisson p;
+  double* means = nullptr;   // statistics: xy means, 3*nz
+};
+
+struct tc_solver {
+  const Grid* g = nullptr;
+  Poisson* pl = nullptr;
+  void *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
+  void* rs[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  double* scal = nullptr;
+  double inv_volume = 0.0;
+  void *su = nullptr, *sv = nullptr, *sw = nullptr;     // post-CN velocities
+  double* tabs = nullptr;                                // CN tables: cell, face
+  double* partials = nullptr;                            // K5 bulk partials
+  unsigned* counters = nullptr;                          // [0] K5, [1] CFL
+  double* cfl_partials = nullptr;
+  double* init_scal = nullptr;                           // dt = 1 block for the initial projection
+  int newest = -1;                                       // rhs set holding the newest rhs
+  int branch = TC_BRANCH_GPU;                            // explicit-RHS semantics
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_steps = 0;
+};
+
+extern "C" {
+
+const char* tc_strerror(int code) {
+  switch (code) {
+    case TC_OK: return "success";
+    case TC_ERR_ARG: return "torchannel: invalid argument";
+    case TC_ERR_SHAPE: return "torchannel: unsupported grid size (nx, ny even and 2^a*3^b*5^c; 4 <= nz <= 256)";
+    case TC_ERR_DTYPE: return "torchannel: unsupported dtype (float64 or float32)";
+    case TC_ERR_UNSUPPORTED: return "torchannel: size outside the compiled kernel range";
+    case TC_ERR_NOMEM: return "torchannel: device workspace allocation failed";
+    default: break;
+  }
+  if (code < 0) return hipGetErrorString((hipError_t)(-code));
+  return "torchannel: unknown status";
+}
+
+int tc_version(void) { return 10000; }
+
+static bool good_fft_size(int n) {
+  if (n < 4 || n > 1536 || (n & 1)) return false;
+  while (n % 2 == 0) n /= 2;
+  while (n % 3 == 0) n /= 3;
+  while (n % 5 == 0) n /= 5;
+  return n == 1;
+}
+
+int tc_grid_create(int nx, int ny, int nz, double dx, double dy, double nu, int top_bc, int dtype,
+                   const double* dzf, const double* dzc, tc_grid** out) {
+  if (!out || !dzf || !dzc) return TC_ERR_ARG;
+  if (dtype != TC_F64 && dtype != TC_F32) return TC_ERR_DTYPE;
+  if (nx < 2 || ny < 2 || nz < 4 || nz > TC_MAX_NZ) return TC_ERR_SHAPE;
+  if (top_bc != TC_TOP_DIRICHLET && top_bc != TC_TOP_NEUMANN) return TC_ERR_ARG;
+  tc_grid* h = new (std::nothrow) tc_grid();
+  if (!h) return TC_ERR_NOMEM;
+  Grid& g = h->g;
+  g.d = make_dims(nx, ny, nz, top_bc);
+  g.dtype = dtype;
+  g.dx = dx; g.dy = dy; g.nu = nu;
+  std::vector<double> rf(nz), rc(nz + 1), wb(nz);
+  std::vector<float> rf32(nz), rc32(nz + 1);
+  for (int k = 0; k < nz; ++k) {
+    rf[k] = 1.0 / dzf[k];
+    rf32[k] = (float)rf[k];
+    wb[k] = dx * dy * dzf[k];
+  }
+  for (int k = 0; k <= nz; ++k) {
+    rc[k] = 1.0 / dzc[k];
+    rc32[k] = (float)rc[k];
+  }
+  g.scratch_len = std::max(2 * kMaxRedBlocks, kStatsBlocks * 4 * nz);
+  bool ok = hipMalloc(&g.dzf64, sizeof(double) * nz) == hipSuccess &&
+            hipMalloc(&g.dzc64, sizeof(double) * (nz + 1)) == hipSuccess &&
+            hipMalloc(&g.rdzf64, sizeof(double) * nz) == hipSuccess &&
+            hipMalloc(&g.rdzc64, sizeof(double) * (nz + 1)) == hipSuccess &&
+            hipMalloc(&g.wbulk, sizeof(double) * nz) == hipSuccess &&
+            hipMalloc(&g.rdzf32, sizeof(float) * nz) == hipSuccess &&
+            hipMalloc(&g.rdzc32, sizeof(float) * (nz + 1)) == hipSuccess &&
+            hipMalloc(&g.scratch, sizeof(double) * g.scratch_len) == hipSuccess &&
+            hipMalloc(&g.counter, sizeof(unsigned) * 8) == hipSuccess &&
+            hipMalloc(&g.tabs, sizeof(double) * 2 * TC_TAB_STRIDE) == hipSuccess &&
+            hipMalloc(&g.scal, sizeof(double) * TC_NSCAL) == hipSuccess;
+  if (ok) {
+    ok = hipMemcpy(g.dzf64, dzf, sizeof(double) * nz, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g.dzc64, dzc, sizeof(double) * (nz + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g.rdzf64, rf.data(), sizeof(double) * nz, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g.rdzc64, rc.data(), sizeof(double) * (nz + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g.wbulk, wb.data(), sizeof(double) * nz, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g.rdzf32, rf32.data(), sizeof(float) * nz, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g.rdzc32, rc32.data(), sizeof(float) * (nz + 1), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(g.counter, 0, sizeof(unsigned) * 8) == hipSuccess &&
+         hipMemset(g.scal, 0, sizeof(double) * TC_NSCAL) == hipSuccess;
+  }
+  if (!ok) {
+    tc_grid_destroy(h);
+    return TC_ERR_NOMEM;
+  }
+  *out = h;
+  return TC_OK;
+}
+
+int tc_grid_destroy(tc_grid* h) {
+  if (!h) return TC_OK;
+  Grid& g = h->g;
+  hipFree(g.dzf64); hipFree(g.dzc64); hipFree(g.rdzf64); hipFree(g.rdzc64); hipFree(g.wbulk);
+  hipFree(g.rdzf32); hipFree(g.rdzc32); hipFree(g.scratch); hipFree(g.counter); hipFree(g.tabs);
+  hipFree(g.scal);
+  delete h;
+  return TC_OK;
+}
+
+int tc_apply_bc(const tc_grid* h, void* u, void* v, void* w, void* stream) {
+  if (!h || !u || !v || !w) return TC_ERR_ARG;
+  return launch_apply_bc(&h->g, u, v, w, (hipStream_t)stream);
+}
+
+int tc_rhs_imex(const tc_grid* h, int branch, const void* u, const void* v, const void* w, void* ru,
+                void* rv, void* rw, void* stream) {
+  if (!h || !u || !v || !w || !ru || !rv || !rw) return TC_ERR_ARG;
+  if (branch != TC_BRANCH_GPU && branch != TC_BRANCH_CPU) return TC_ERR_ARG;
+  const Grid& g = h->g;
+  return launch_k1(&g, u, v, w, nullptr, nullptr, nullptr, ru, rv, rw, nullptr, nullptr, nullptr,
+                   nullptr, nullptr, g.scal, nullptr, TC_K1_RHS, branch, 0.5, (hipStream_t)stream);
+}
+
+int tc_cn_solve(const tc_grid* h, int comp, double dt, const double* dt_dev, double theta,
+                const void* q_in, void* q_out, void* stream) {
+  if (!h || !q_in || !q_out || comp < 0 || comp > 2) return TC_ERR_ARG;
+  if (!dt_dev && !(dt > 0.0)) return TC_ERR_ARG;
+  const Grid& g = h->g;
+  hipStream_t st = (hipStream_t)stream;
+  double* scal = dt_dev ? const_cast<double*>(dt_dev) : g.scal;
+  TC_RC(launch_tables(&g, scal, dt_dev ? -1.0 : dt, theta, g.tabs, g.tabs + TC_TAB_STRIDE, st));
+  return launch_cn_only(&g, comp, q_in, q_out, comp < 2 ? g.tabs : g.tabs + TC_TAB_STRIDE, scal, theta, st);
+}
+
+int tc_pcr_batched(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
+                   const void* c, int64_t abc_stride, const void* d, void* x, void* stream) {
+  if (n < 1 || n > 512 || batch < 0 || !a || !b || !c || !d || !x) return TC_ERR_ARG;
+  if (dtype != TC_F64 && dtype != TC_F32) return TC_ERR_DTYPE;
+  if (batch == 0) return TC_OK;
+  return pcr_batched(n, batch, dtype, is_complex, a, b, c, (long long)abc_stride, d, x, (hipStream_t)stream);
+}
+
+int tc_divergence(const tc_grid* h, const void* u, const void* v, const void* w, void* div, void* stream) {
+  if (!h || !u || !v || !w || !div) return TC_ERR_ARG;
+  return launch_divergence(&h->g, u, v, w, div, (hipStream_t)stream);
+}
+
+int tc_project(const tc_grid* h, void* u, void* v, void* w, const void* p, double dt,
+               const double* dt_dev, void* stream) {
+  if (!h || !u || !v || !w || !p) return TC_ERR_ARG;
+  return launch_project(&h->g, u, v, w, p, dt, dt_dev, (hipStream_t)stream);
+}
+
+int tc_bulk_velocity(const tc_grid* h, const void* u, const double* wgt, int64_t ws0, int64_t ws1,
+                     int64_t ws2, double total_volume, double* out_dev, void* stream) {
+  if (!h || !u || !wgt || !out_dev || !(total_volume != 0.0)) return TC_ERR_ARG;
+  const Grid& g = h->g;
+  return launch_bulk(&g, u, wgt, ws0, ws1, ws2, 1.0 / total_volume, out_dev, g.scratch, g.counter,
+                     kMaxRedBlocks, (hipStream_t)stream);
+}
+
+int tc_cfl_rate(const tc_grid* h, const void* u, const void* v, const void* w, double* out_dev, void* stream) {
+  if (!h || !u || !v || !w || !out_dev) return TC_ERR_ARG;
+  const Grid& g = h->g;
+  // the rate lands in scal[TC_S_CFL]; point scal so that slot is out_dev
+  return launch_cfl(&g, u, v, w, out_dev - TC_S_CFL, g.scratch, g.counter + 1, 0, 0.0, 0.0, 0.0,
+                    kMaxRedBlocks, (hipStream_t)stream);
+}
+
+int tc_u_tau(const tc_grid* h, const void* u, double z_c1, double* out_dev, void* stream) {
+  if (!h || !u || !out_dev) return TC_ERR_ARG;
+  const Grid& g = h->g;
+  return launch_u_tau(&g, u, z_c1, out_dev, g.scratch, g.counter + 2, 2 * kMaxRedBlocks, (hipStream_t)stream);
+}
+
+int tc_poisson_create(const tc_grid* h, tc_poisson** out) {
+  if (!h || !out) return TC_ERR_ARG;
+  if (!good_fft_size(h->g.d.nx) || !good_fft_size(h->g.d.ny)) return TC_ERR_SHAPE;
+  tc_poisson* pl = new (std::nothrow) tc_poisson();
+  if (!pl) return TC_ERR_NOMEM;
+  const Grid& g = h->g;
+  std::vector<double> dzf(g.d.nz), dzc(g.d.nz + 1);
+  if (hipMemcpy(dzf.data(), g.dzf64, sizeof(double) * g.d.nz, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(dzc.data(), g.dzc64, sizeof(double) * (g.d.nz + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+    delete pl;
+    return TC_ERR_NOMEM;
+  }
+  int rc = poisson_init(&pl->p, &g, dzf.data(), dzc.data());
+  if (rc == TC_OK && hipMalloc(&pl->means, sizeof(double) * 3 * g.d.nz) != hipSuccess) rc = TC_ERR_NOMEM;
+  if (rc != TC_OK) {
+    poisson_free(&pl->p);
+    delete pl;
+    return rc;
+  }
+  *out = pl;
+  return TC_OK;
+}
+
+int tc_poisson_destroy(tc_poisson* pl) {
+  if (!pl) return TC_OK;
+  poisson_free(&pl->p);
+  hipFree(pl->means);
+  delete pl;
+  return TC_OK;
+}
+
+int tc_poisson_solve(tc_poisson* pl, const void* rhs, void* p, void* stream) {
+  if (!pl || !rhs || !p) return TC_ERR_ARG;
+  return poisson_solve(&pl->p, false, nullptr, nullptr, nullptr, rhs, nullptr, p, (hipStream_t)stream);
+}
+
+int tc_stats_accumulate(tc_poisson* pl, const void* u, const void* v, const void* w, int k_bot,
+                        int k_top, double* prof, double* spec, void* stream) {
+  if (!pl || !u || !v || !w || !prof || !spec) return TC_ERR_ARG;
+  const Grid* g = pl->p.g;
+  if (k_bot < 1 || k_bot > g->d.nz || k_top < 1 || k_top > g->d.nz) return TC_ERR_ARG;
+  return launch_stats(&pl->p, u, v, w, k_bot, k_top, prof, spec, pl->means, g->scratch, g->counter + 3,
+                      kStatsBlocks, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------------------------------
+// Solver
+// ------------------------------------------------------------------------------------------
+int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w, void* p,
+                     void* const* rhs_a, void* const* rhs_b, double* scal, double volume,
+                     int branch, tc_solver** out) {
+  if (!h || !pl || !u || !v || !w || !p || !rhs_a || !rhs_b || !scal || !out || !(volume > 0.0))
+    return TC_ERR_ARG;
+  if (branch != TC_BRANCH_GPU && branch != TC_BRANCH_CPU) return TC_ERR_ARG;
+  for (int c = 0; c < 3; ++c)
+    if (!rhs_a[c] || !rhs_b[c]) return TC_ERR_ARG;
+  tc_solver* s = new (std::nothrow) tc_solver();
+  if (!s) return TC_ERR_NOMEM;
+  const Grid& g = h->g;
+  s->g = &g;
+  s->pl = &pl->p;
+  s->u = u; s->v = v; s->w = w; s->p = p;
+  for (int c = 0; c < 3; ++c) { s->rs[0][c] = rhs_a[c]; s->rs[1][c] = rhs_b[c]; }
+  s->scal = scal;
+  s->inv_volume = 1.0 / volume;
+  s->branch = branch;
+  const Dims& d = g.d;
+  const size_t esz = g.dtype == TC_F64 ? 8 : 4;
+  const size_t nu_ = (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
+  const size_t nv_ = (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
+  const size_t nw_ = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
+  bool ok = hipMalloc(&s->su, esz * nu_) == hipSuccess && hipMalloc(&s->sv, esz * nv_) == hipSuccess &&
+            hipMalloc(&s->sw, esz * nw_) == hipSuccess &&
+            hipMalloc(&s->tabs, sizeof(double) * 2 * TC_TAB_STRIDE) == hipSuccess &&
+            hipMalloc(&s->partials, sizeof(double) * k5_blocks(&g)) == hipSuccess &&
+            hipMalloc(&s->counters, sizeof(unsigned) * 4) == hipSuccess &&
+            hipMalloc(&s->cfl_partials, sizeof(double) * kMaxRedBlocks) == hipSuccess &&
+            hipMalloc(&s->init_scal, sizeof(double) * TC_NSCAL) == hipSuccess &&
+            hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) == hipSuccess;
+  if (ok) {
+    double init[TC_NSCAL] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    ok = hipMemset(s->counters, 0, sizeof(unsigned) * 4) == hipSuccess &&
+         hipMemcpy(s->init_scal, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) {
+    tc_solver_destroy(s);
+    return TC_ERR_NOMEM;
+  }
+  *out = s;
+  return TC_OK;
+}
+
+int tc_solver_destroy(tc_solver* s) {
+  if (!s) return TC_OK;
+  if (s->gexec) hipGraphExecDestroy(s->gexec);
+  if (s->graph) hipGraphDestroy(s->graph);
+  if (s->cap_stream) hipStreamDestroy(s->cap_stream);
+  hipFree(s->su); hipFree(s->sv); hipFree(s->sw); hipFree(s->tabs); hipFree(s->partials);
+  hipFree(s->counters); hipFree(s->cfl_partials); hipFree(s->init_scal);
+  delete s;
+  return TC_OK;
+}
+
+static int solver_step_impl(tc_solver* s, double dt, int first, const void* body, hipStream_t st) {
+  const Grid* g = s->g;
+  if (!first && s->newest < 0) return TC_ERR_ARG;
+  const int cur = first ? 0 : 1 - s->newest;
+  const int prev = first ? -1 : s->newest;
+  TC_RC(launch_tables(g, s->scal, dt, 0.5, s->tabs, s->tabs + TC_TAB_STRIDE, st));
+  int mode = TC_K1_RHS | TC_K1_STEP | TC_K1_FORCE | (first ? TC_K1_FIRST : 0);
+  void* const* pr = prev >= 0 ? s->rs[prev] : s->rs[cur];
+  TC_RC(launch_k1(g, s->u, s->v, s->w, pr[0], pr[1], pr[2], s->rs[cur][0], s->rs[cur][1], s->rs[cur][2],
+                  s->su, s->sv, s->sw, s->tabs, s->tabs + TC_TAB_STRIDE, s->scal, body, mode,
+                  s->branch, 0.5, st));
+  TC_RC(poisson_solve(s->pl, true, s->su, s->sv, s->sw, nullptr, s->scal, s->p, st));
+  TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->scal, s->partials, s->counters,
+                  s->inv_volume, TC_K5_BULK | TC_K5_FORCING, st));
+  s->newest = cur;
+  return TC_OK;
+}
+
+int tc_solver_step(tc_solver* s, double dt, int first, const void* body, int* newest, void* stream) {
+  if (!s) return TC_ERR_ARG;
+  int rc = solver_step_impl(s, dt, first, body, (hipStream_t)stream);
+  if (newest) *newest = s->newest;
+  return rc;
+}
+
+// Rescale u to the target bulk velocity when |u_b| > 1e-9 (solver.py:271-276), then the
+// initial projection with dt = 1 (solver.py:345-369).
+__global__ void k_rescale_u(double* __restrict__ u64, float* __restrict__ u32, size_t n,
+                            const double* __restrict__ scal) {
+  const double ub = scal[TC_S_UBULK];
+  if (!(fabs(ub) > 1e-9)) return;
+  const double f = scal[TC_S_UTARGET] / ub;
+  for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (size_t)gridDim.x * 256) {
+    if (u64) u64[t] *= f;
+    else u32[t] = (float)((double)u32[t] * f);
+  }
+}
+
+int tc_solver_project_initial(tc_solver* s, void* stream) {
+  if (!s) return TC_ERR_ARG;
+  const Grid* g = s->g;
+  const Dims& d = g->d;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nu_ = (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
+  const size_t nv_ = (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
+  const size_t nw_ = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
+  const size_t esz = g->dtype == TC_F64 ? 8 : 4;
+  // u_b of the initial field into scal[TC_S_UBULK] (cell volume dx*dy*dz_f expanded, solver.py:231)
+  TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
+                    g->counter, kMaxRedBlocks, st));
+  hipLaunchKernelGGL(k_rescale_u, dim3(1024), dim3(256), 0, st,
+                     g->dtype == TC_F64 ? (double*)s->u : nullptr, g->dtype == TC_F64 ? nullptr : (float*)s->u,
+                     nu_, s->scal);
+  TC_CHECK_LAUNCH();
+  TC_HIP(hipMemcpyAsync(s->su, s->u, esz * nu_, hipMemcpyDeviceToDevice, st));
+  TC_HIP(hipMemcpyAsync(s->sv, s->v, esz * nv_, hipMemcpyDeviceToDevice, st));
+  TC_HIP(hipMemcpyAsync(s->sw, s->w, esz * nw_, hipMemcpyDeviceToDevice, st));
+  TC_RC(poisson_solve(s->pl, true, s->su, s->sv, s->sw, nullptr, s->init_scal, s->p, st));
+  TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->init_scal, s->partials,
+                  s->counters, s->inv_volume, 0, st));
+  return TC_OK;
+}
+
+int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream) {
+  if (!s) return TC_ERR_ARG;
+  return launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, s->counters + 1, 1, cfl_target,
+                    dt_min, dt_max, kMaxRedBlocks, (hipStream_t)stream);
+}
+
+int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_target, double dt_min,
+                      double dt_max, void* stream) {
+  if (!s || steps < 2 || (steps & 1) || s->newest < 0) return TC_ERR_ARG;
+  if (s->gexec) { hipGraphExecDestroy(s->gexec); s->gexec = nullptr; }
+  if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+  hipStream_t cs = s->cap_stream;
+  // order the capture stream after the caller's work (not captured)
+  TC_HIP(hipStreamSynchronize((hipStream_t)stream));
+  const int newest0 = s->newest;
+  TC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int rc = TC_OK;
+  for (int i = 0; i < steps && rc == TC_OK; ++i) {
+    if (dt_interval > 0 && (i + 1) % dt_interval == 0)
+      rc = launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, s->counters + 1, 1, cfl_target,
+                      dt_min, dt_max, kMaxRedBlocks, cs);
+    if (rc == TC_OK) rc = solver_step_impl(s, -1.0, 0, nullptr, cs);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(cs, &graph);
+  s->newest = newest0;   // capture recorded the launches; state is advanced only by replays
+  if (rc != TC_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return -(int)e;
+  s->graph = graph;
+  TC_HIP(hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0));
+  s->graph_steps = steps;
+  return TC_OK;
+}
+
+int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream) {
+  if (!s || s->newest < 0 || reps < 1) return TC_ERR_ARG;
+  const Grid* g = s->g;
+  hipStream_t st = (hipStream_t)stream;
+  const int cur = 1 - s->newest, prev = s->newest;
+  for (int r = 0; r < reps; ++r) {
+    switch (phase) {
+      case 0:
+        TC_RC(launch_tables(g, s->scal, -1.0, 0.5, s->tabs, s->tabs + TC_TAB_STRIDE, st));
+        break;
+      case 1:
+        TC_RC(launch_k1(g, s->u, s->v, s->w, s->rs[prev][0], s->rs[prev][1], s->rs[prev][2], s->rs[cur][0],
+                        s->rs[cur][1], s->rs[cur][2], s->su, s->sv, s->sw, s->tabs, s->tabs + TC_TAB_STRIDE,
+                        s->scal, nullptr, TC_K1_RHS | TC_K1_STEP | TC_K1_FORCE, s->branch, 0.5, st));
+        break;
+      case 2: case 3: case 4: case 5: case 6:
+        TC_RC(poisson_phase(s->pl, phase - 2, true, s->su, s->sv, s->sw, nullptr, s->scal, s->p, st));
+        break;
+      case 7:
+        TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->scal, s->partials, s->counters,
+                        s->inv_volume, TC_K5_BULK, st));
+        break;
+      case 8:
+        TC_RC(launch_cfl(g, s->u, s->v, s->w, s->scal, s->cfl_partials, s->counters + 1, 0, 0.25, 1e-4, 0.01,
+                         kMaxRedBlocks, st));
+        break;
+      default:
+        return TC_ERR_ARG;
+    }
+  }
+  return TC_OK;
+}
+
+int tc_solver_replay(tc_solver* s, void* stream) {
+  if (!s || !s->gexec) return TC_ERR_ARG;
+  TC_HIP(hipGraphLaunch(s->gexec, (hipStream_t)stream));
+  return TC_OK;
+}
+
+}  // extern "C"

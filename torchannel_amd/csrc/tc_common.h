@@ -1,0 +1,178 @@
+// Shared device helpers for the torChannel IMEX kernels (gfx950 / CDNA4).
+//
+// Array conventions (reference index space, z-contiguous C order, 0-based):
+//   u (nx+1, ny+2, nz+2)   v (nx+2, ny+1, nz+2)   w (nx+2, ny+2, nz+1)   p (nx+2, ny+2, nz+2)
+// initflow.py:6-9 allocates these shapes; every kernel here indexes them the same way.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/torchannel.h"
+
+#define TC_WAVE 64
+#define TC_MAX_NZ 256        // rows per wall-normal column held by one wavefront (4 per lane)
+#define TC_MAX_STAGES 8      // ceil(log2(TC_MAX_NZ))
+#define TC_TAB_STRIDE (TC_MAX_NZ * (2 * TC_MAX_STAGES + 1))
+
+#define TC_CHECK_LAUNCH()                                      \
+  do {                                                         \
+    hipError_t e_ = hipGetLastError();                         \
+    if (e_ != hipSuccess) return -(int)e_;                     \
+  } while (0)
+
+// Extents and strides of the four ghosted arrays.
+struct Dims {
+  int nx, ny, nz;
+  int top;               // +1: free-slip top wall (ghost = +interior), -1: no-slip (ghost = -interior)
+  int syu, sxu;          // u strides (elements)
+  int syv, sxv;
+  int syw, sxw;
+  int syp, sxp;
+};
+
+static inline Dims make_dims(int nx, int ny, int nz, int top_bc) {
+  Dims d;
+  d.nx = nx; d.ny = ny; d.nz = nz;
+  d.top = top_bc == TC_TOP_NEUMANN ? 1 : -1;
+  d.syu = nz + 2; d.sxu = (ny + 2) * (nz + 2);
+  d.syv = nz + 2; d.sxv = (ny + 1) * (nz + 2);
+  d.syw = nz + 1; d.sxw = (ny + 2) * (nz + 1);
+  d.syp = nz + 2; d.sxp = (ny + 2) * (nz + 2);
+  return d;
+}
+
+// Per-grid wall-normal metric, precomputed once (tc_grid_create) in float64 and in the
+// kernel's working type.  Index ranges: rdzf/dzf [0, nz), rdzc/dzc [0, nz].
+template <typename T>
+struct Metric {
+  const T* rdzf;         // 1/dz_f
+  const T* rdzc;         // 1/dz_c
+  const double* dzf;     // dz_f (float64; coefficient tables are built from these)
+  const double* dzc;     // dz_c
+  const double* wbulk;   // dx*dy*dz_f[k] (float64 bulk weights, solver.py:231)
+  T rdx, rdy;            // 1/dx, 1/dy
+  T kxx, kyy;            // nu/dx^2, nu/dy^2 (operators.py:1064-1067)
+  double nu;
+};
+
+// Blocks are dealt round-robin to the 8 XCDs; remap so that each XCD walks a contiguous range
+// of logical tiles and its private L2 sees the neighbouring columns of the tiles it runs.
+__device__ __forceinline__ int xcd_remap(int b, int nb) {
+  const int q = nb >> 3, rem = nb & 7;
+  const int x = b & 7, idx = b >> 3;
+  return x < rem ? x * (q + 1) + idx : rem * (q + 1) + (x - rem) * q + idx;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (TC_WAVE - 1); }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, TC_WAVE);
+  return x;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, TC_WAVE));
+  return x;
+}
+
+// Rows of a wall-normal column live in registers: row m = r*64 + lane, r in [0, R).
+// shift_dn: out[r] = row (m - s), shift_up: out[r] = row (m + s); rows outside [0, 64R) -> pad.
+// S is a compile-time shift so all register indexing stays static.
+template <int S, int R, typename T>
+__device__ __forceinline__ void shift_dn(const T (&x)[R], T (&out)[R], T pad, int lane) {
+  if constexpr (S % TC_WAVE == 0) {
+    constexpr int q = S / TC_WAVE;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r] = (r - q >= 0) ? x[(r - q >= 0) ? r - q : 0] : pad;
+  } else {
+    constexpr int q = S / TC_WAVE, s = S % TC_WAVE;
+    T t[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) t[r] = __shfl(x[r], (lane - s) & (TC_WAVE - 1), TC_WAVE);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int ra = r - q, rb = r - q - 1;            // same wrap row / previous wrap row
+      const T a = ra >= 0 ? t[ra >= 0 ? ra : 0] : pad;
+      const T b = rb >= 0 ? t[rb >= 0 ? rb : 0] : pad;
+      out[r] = lane >= s ? a : b;
+    }
+  }
+}
+
+template <int S, int R, typename T>
+__device__ __forceinline__ void shift_up(const T (&x)[R], T (&out)[R], T pad, int lane) {
+  if constexpr (S % TC_WAVE == 0) {
+    constexpr int q = S / TC_WAVE;
+#pragma unroll
+    for (int r = 0; r < R; ++r) out[r] = (r + q < R) ? x[(r + q < R) ? r + q : 0] : pad;
+  } else {
+    constexpr int q = S / TC_WAVE, s = S % TC_WAVE;
+    T t[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) t[r] = __shfl(x[r], (lane + s) & (TC_WAVE - 1), TC_WAVE);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int ra = r + q, rb = r + q + 1;
+      const T a = ra < R ? t[ra < R ? ra : 0] : pad;
+      const T b = rb < R ? t[rb < R ? rb : 0] : pad;
+      out[r] = lane + s < TC_WAVE ? a : b;
+    }
+  }
+}
+
+// Parallel cyclic reduction of one column whose matrix is shared by every column (the CN
+// systems), so the whole a/b/c reduction is a precomputed table: per stage st, alpha[m] at
+// tab[st*2*MAXNZ + m], gamma[m] at tab[(st*2+1)*MAXNZ + m]; final diagonal b at
+// tab[2*STAGES*MAXNZ + m].  Rows >= n carry alpha = gamma = 0, b = 1, d = 0.
+// Recurrence of tridiag.py:50-58 restricted to d: d += alpha*d[m-s] + gamma*d[m+s]; x = d/b.
+template <int ST, int R, typename T>
+__device__ __forceinline__ void pcr_shared_stage(T (&dv)[R], const double* __restrict__ tab, int n, int lane) {
+  constexpr int s = 1 << ST;
+  if (s >= n) return;
+  T dn[R], up[R];
+  shift_dn<s, R, T>(dv, dn, T(0), lane);
+  shift_up<s, R, T>(dv, up, T(0), lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    const T al = (T)tab[(2 * ST) * TC_MAX_NZ + m];
+    const T ga = (T)tab[(2 * ST + 1) * TC_MAX_NZ + m];
+    dv[r] = dv[r] + al * dn[r] + ga * up[r];
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void pcr_shared_solve(T (&dv)[R], const double* __restrict__ tab, int n, int lane) {
+  pcr_shared_stage<0, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<1, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<2, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<3, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<4, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<5, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<6, R, T>(dv, tab, n, lane);
+  pcr_shared_stage<7, R, T>(dv, tab, n, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    dv[r] = dv[r] / (T)tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m];
+  }
+}
+
+// Complex number in the working precision.
+template <typename T>
+struct cplx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.re - b.re, a.im - b.im}; }

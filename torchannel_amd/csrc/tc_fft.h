@@ -1,0 +1,120 @@
+// LDS-resident batched complex FFT (Stockham autosort, radices 4, 2, 3, 5; N = 2^a 3^b 5^c).
+//
+// A block of 256 threads transforms `batch` sequences of length N held in LDS (sequence b at
+// buf[b*ldb + 0..N)), batch*N <= TC_FFT_ELEMS.  Pass with radix R and span p (p = product of the
+// radices already applied): butterfly idx in [0, N/R), k = idx mod p, inputs x[idx + r*N/R],
+// twiddles exp(sign*2*pi*i*r*k/(p*R)), outputs at (idx-k)*R + k + r*p.
+#pragma once
+#include "tc_common.h"
+
+#define TC_FFT_ELEMS 2048
+#define TC_FFT_MAXPASS 12
+
+struct FftDesc {
+  int n;
+  int npass;
+  int radix[TC_FFT_MAXPASS];
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
+  if constexpr (R == 2) {
+    const cplx<T> a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const cplx<T> a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    const cplx<T> c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+    // s*i*d with s = sign
+    const cplx<T> sid = sign < 0 ? cplx<T>{d.im, -d.re} : cplx<T>{-d.im, d.re};
+    x[0] = cadd(a, c);
+    x[2] = csub(a, c);
+    x[1] = cadd(b, sid);
+    x[3] = csub(b, sid);
+  } else if constexpr (R == 5) {
+    const T c1 = T(0.30901699437494742410229341718282);    // cos(2 pi/5)
+    const T c2 = T(-0.80901699437494742410229341718282);   // cos(4 pi/5)
+    const T s1 = T(0.95105651629515357211643933337938);    // sin(2 pi/5)
+    const T s2 = T(0.58778525229247312916870595463907);    // sin(4 pi/5)
+    const cplx<T> t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+    const cplx<T> t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
+    const cplx<T> a1 = {x[0].re + c1 * t1.re + c2 * t2.re, x[0].im + c1 * t1.im + c2 * t2.im};
+    const cplx<T> a2 = {x[0].re + c2 * t1.re + c1 * t2.re, x[0].im + c2 * t1.im + c1 * t2.im};
+    const cplx<T> b1 = {s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im};
+    const cplx<T> b2 = {s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im};
+    // sign*i*b
+    const cplx<T> ib1 = sign < 0 ? cplx<T>{b1.im, -b1.re} : cplx<T>{-b1.im, b1.re};
+    const cplx<T> ib2 = sign < 0 ? cplx<T>{b2.im, -b2.re} : cplx<T>{-b2.im, b2.re};
+    x[0] = cadd(x[0], cadd(t1, t2));
+    x[1] = cadd(a1, ib1);
+    x[4] = csub(a1, ib1);
+    x[2] = cadd(a2, ib2);
+    x[3] = csub(a2, ib2);
+  } else {  // R == 3
+    const T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
+    const cplx<T> t1 = cadd(x[1], x[2]);
+    const cplx<T> t2 = {x[0].re - T(0.5) * t1.re, x[0].im - T(0.5) * t1.im};
+    const cplx<T> df = csub(x[1], x[2]);
+    const T sh = sign < 0 ? -h : h;
+    const cplx<T> rot = {-sh * df.im, sh * df.re};  // i*sign*(sqrt3/2)*(x1-x2)
+    x[0] = cadd(x[0], t1);
+    x[1] = cadd(t2, rot);
+    x[2] = csub(t2, rot);
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N, int p,
+                                         const double2* __restrict__ tw, int sign) {
+  constexpr int MB = (TC_FFT_ELEMS / R + 255) / 256;
+  const int M = N / R;
+  const int total = batch * M;
+  const int tstride = N / (p * R);
+  cplx<T> x[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < total) {
+      const int bb = t / M, idx = t - bb * M;
+      const cplx<T>* base = buf + bb * ldb;
+#pragma unroll
+      for (int mm = 0; mm < R; ++mm) x[q][mm] = base[idx + mm * M];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < total) {
+      const int bb = t / M, idx = t - bb * M;
+      const int k = idx % p;
+#pragma unroll
+      for (int mm = 1; mm < R; ++mm) {
+        const double2 wv = tw[mm * k * tstride];
+        const cplx<T> wc = {(T)wv.x, sign < 0 ? (T)wv.y : (T)(-wv.y)};
+        x[q][mm] = cmul(x[q][mm], wc);
+      }
+      fft_butterfly<T, R>(x[q], sign);
+      cplx<T>* base = buf + bb * ldb;
+      const int j0 = (idx - k) * R + k;
+#pragma unroll
+      for (int mm = 0; mm < R; ++mm) base[j0 + mm * p] = x[q][mm];
+    }
+  }
+  __syncthreads();
+}
+
+// tw[t] = exp(-2*pi*i*t/N), t in [0, N), float64.
+template <typename T>
+__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const double2* __restrict__ tw,
+                        int sign) {
+  int p = 1;
+  for (int ps = 0; ps < f.npass; ++ps) {
+    const int R = f.radix[ps];
+    if (R == 4) fft_pass<T, 4>(buf, ldb, batch, f.n, p, tw, sign);
+    else if (R == 2) fft_pass<T, 2>(buf, ldb, batch, f.n, p, tw, sign);
+    else if (R == 3) fft_pass<T, 3>(buf, ldb, batch, f.n, p, tw, sign);
+    else fft_pass<T, 5>(buf, ldb, batch, f.n, p, tw, sign);
+    p *= R;
+  }
+}

@@ -1,0 +1,101 @@
+// Library-internal declarations: the grid object, scalar-block layout, kernel mode bits and the
+// launchers shared between the per-operator entry points and the fused step.
+#pragma once
+#include "tc_common.h"
+
+// Layout of the float64 scalar block a solver keeps in device memory (length TC_NSCAL).
+#define TC_S_DT 0        // current time step
+#define TC_S_F 1         // bulk forcing (lagged P-controller state, solver.py:707-708)
+#define TC_S_UBULK 2     // bulk velocity after the last step
+#define TC_S_TIME 3      // simulated time
+#define TC_S_CFL 4       // last convective rate (1/dt) from the CFL reduction
+#define TC_S_UTARGET 5   // target bulk velocity U_bulk
+#define TC_S_STEP 6      // steps taken (as a double)
+#define TC_S_SPARE 7
+
+// k_explicit_cn modes
+#define TC_K1_RHS 1
+#define TC_K1_STEP 2
+#define TC_K1_FIRST 4
+#define TC_K1_FORCE 8
+
+// k_project_bc flags
+#define TC_K5_BULK 1
+#define TC_K5_FORCING 2
+
+namespace tc {
+
+// Host-side grid object: extents, spacings and device copies of the wall-normal metric.
+struct Grid {
+  Dims d;
+  int dtype;
+  double dx, dy, nu;
+  // float64 device tables
+  double* dzf64 = nullptr;   // nz
+  double* dzc64 = nullptr;   // nz+1
+  double* rdzf64 = nullptr;
+  double* rdzc64 = nullptr;
+  double* wbulk = nullptr;   // dx*dy*dz_f
+  // float32 copies of the reciprocals
+  float* rdzf32 = nullptr;
+  float* rdzc32 = nullptr;
+  // scratch for the stand-alone reductions and CN tables
+  double* scratch = nullptr;      // partial sums / maxima
+  int scratch_len = 0;
+  unsigned* counter = nullptr;    // last-block counters (several)
+  double* tabs = nullptr;         // CN tables: cell, face
+  double* scal = nullptr;         // scalar block for stand-alone ops
+
+  template <typename T>
+  Metric<T> metric() const;
+};
+
+template <>
+inline Metric<double> Grid::metric<double>() const {
+  Metric<double> m;
+  m.rdzf = rdzf64; m.rdzc = rdzc64; m.dzf = dzf64; m.dzc = dzc64; m.wbulk = wbulk;
+  m.rdx = 1.0 / dx; m.rdy = 1.0 / dy;
+  m.kxx = nu * (1.0 / (dx * dx)); m.kyy = nu * (1.0 / (dy * dy)); m.nu = nu;
+  return m;
+}
+
+template <>
+inline Metric<float> Grid::metric<float>() const {
+  Metric<float> m;
+  m.rdzf = rdzf32; m.rdzc = rdzc32; m.dzf = dzf64; m.dzc = dzc64; m.wbulk = wbulk;
+  m.rdx = (float)(1.0 / dx); m.rdy = (float)(1.0 / dy);
+  m.kxx = (float)(nu * (1.0 / (dx * dx))); m.kyy = (float)(nu * (1.0 / (dy * dy))); m.nu = nu;
+  return m;
+}
+
+// tc_step.hip
+int launch_tables(const Grid* gr, double* scal, double dt_set, double theta, double* tab_cell,
+                  double* tab_face, hipStream_t st);
+int launch_k1(const Grid* gr, const void* u, const void* v, const void* w, const void* pu,
+              const void* pv, const void* pw, void* ru, void* rv, void* rw, void* su, void* sv,
+              void* sw, const double* tab_cell, const double* tab_face, const double* scal,
+              const void* body, int mode, int branch, double theta, hipStream_t st);
+int launch_cn_only(const Grid* gr, int comp, const void* q, void* out, const double* tab,
+                   const double* scal, double theta, hipStream_t st);
+int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p, void* u,
+              void* v, void* w, double* scal, double* partials, unsigned* counter, double inv_volume,
+              int flags, hipStream_t st);
+int k5_blocks(const Grid* gr);
+
+// tc_ops.hip
+int launch_apply_bc(const Grid* gr, void* u, void* v, void* w, hipStream_t st);
+int launch_divergence(const Grid* gr, const void* u, const void* v, const void* w, void* div,
+                      hipStream_t st);
+int launch_project(const Grid* gr, void* u, void* v, void* w, const void* p, double dt_host,
+                   const double* dt_dev, hipStream_t st);
+int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0, long long ws1,
+                long long ws2, double inv_vol, double* out, double* partials, unsigned* counter,
+                int nblocks_max, hipStream_t st);
+int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double* partials,
+                 unsigned* counter, int nblocks_max, hipStream_t st);
+int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, double* scal,
+               double* partials, unsigned* counter, int update_dt, double cfl_target, double dt_min,
+               double dt_max, int nblocks_max, hipStream_t st);
+int cfl_blocks(const Grid* gr);
+
+}  // namespace tc

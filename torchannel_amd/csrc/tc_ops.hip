@@ -1,0 +1,454 @@
+// Stand-alone operators of the IMEX path and the device-side reductions: boundary conditions,
+// divergence, projection, bulk velocity, CFL rate (with the adaptive-dt rule), wall shear and the
+// turbulence-statistics accumulation.  Reductions write per-block partials and the last block
+// to finish folds them in a fixed order, so results do not depend on scheduling.
+#include "tc_internal.h"
+#include "tc_poisson.h"
+
+namespace tc {
+
+__device__ __forceinline__ double block_reduce(double v, bool is_max, double* sh) {
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
+  v = is_max ? wave_max(v) : wave_sum(v);
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    r = sh[0];
+    for (int q = 1; q < nw; ++q) r = is_max ? fmax(r, sh[q]) : r + sh[q];
+  }
+  __syncthreads();
+  return r;
+}
+
+// Thread 0 publishes `val` into partials[slot]; returns true (in every thread) in the block that
+// finishes last, after which all partials are visible to it.
+__device__ __forceinline__ bool publish_last(double* partials, int slot, double val, unsigned* counter,
+                                             int nb) {
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partials[slot], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    last = atomicAdd(counter, 1u) == (unsigned)(nb - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ __forceinline__ double fold_partials(const double* partials, int nb, bool is_max, double* sh) {
+  double s = is_max ? -1.0e308 : 0.0;
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+    const double x = __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = is_max ? fmax(s, x) : s + x;
+  }
+  return block_reduce(s, is_max, sh);
+}
+
+// ---------------------------------------------------------------------------------------------
+// apply_bc_all (solver.py:23-79).  After the reference's x, y, z sequence every ghost entry holds
+// the periodic image / wall reflection of an interior entry; each wavefront fills one column
+// from its source column.  Interior columns only receive their wall-normal ghosts.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_bc(Dims d, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w) {
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int comp = g % 3, col = g / 3;
+  const int Y = d.ny + 2;
+  const int i = col / Y, j = col % Y;
+  if (i >= d.nx + 2) return;
+  if (comp == 0 && i > d.nx) return;
+  if (comp == 1 && j > d.ny) return;
+  const int si = (i == 0) ? d.nx : (i == d.nx + 1 ? 1 : i);
+  const int sj = (j == 0) ? d.ny : (j == d.ny + 1 ? 1 : j);
+  const bool ghost_col = si != i || sj != j;
+  if (comp < 2) {
+    T* q = comp == 0 ? u : v;
+    const int sx = comp == 0 ? d.sxu : d.sxv, sy = comp == 0 ? d.syu : d.syv;
+    const int dst = i * sx + j * sy, src = si * sx + sj * sy;
+    for (int k = 1 + lane; k <= d.nz; k += TC_WAVE) {
+      const T val = q[src + k];
+      if (ghost_col) q[dst + k] = val;
+      if (k == 1) q[dst] = -val;
+      if (k == d.nz) q[dst + d.nz + 1] = d.top > 0 ? val : -val;
+    }
+  } else {
+    const int dst = i * d.sxw + j * d.syw, src = si * d.sxw + sj * d.syw;
+    if (ghost_col)
+      for (int k = 1 + lane; k < d.nz; k += TC_WAVE) w[dst + k] = w[src + k];
+    if (lane == 0) { w[dst] = T(0); w[dst + d.nz] = T(0); }
+  }
+}
+
+// compute_divergence (utils.py:235-240), reading the arrays as stored.
+template <typename T>
+__global__ __launch_bounds__(256) void k_divergence(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                                    const T* __restrict__ v, const T* __restrict__ w,
+                                                    T* __restrict__ div) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  if (t >= N) return;
+  const int k = (int)(t % d.nz);
+  const long long ij = t / d.nz;
+  const int j = (int)(ij % d.ny), i = (int)(ij / d.ny);
+  const int cu = (i + 1) * d.sxu + (j + 1) * d.syu + k + 1;
+  const int cv = (i + 1) * d.sxv + (j + 1) * d.syv + k + 1;
+  const int cw = (i + 1) * d.sxw + (j + 1) * d.syw + k + 1;
+  div[t] = (u[cu] - u[cu - d.sxu]) * mt.rdx + (v[cv] - v[cv - d.syv]) * mt.rdy + (w[cw] - w[cw - 1]) * mt.rdzf[k];
+}
+
+// project_velocity (projection.py:154-163) on the interior, in place.
+template <typename T>
+__global__ __launch_bounds__(256) void k_project(Dims d, Metric<T> mt, T* __restrict__ u, T* __restrict__ v,
+                                                 T* __restrict__ w, const T* __restrict__ p,
+                                                 double dt_host, const double* __restrict__ dt_dev) {
+  const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  if (t >= N) return;
+  const T dt = (T)(dt_dev ? *dt_dev : dt_host);
+  const int k = (int)(t % d.nz) + 1;
+  const long long ij = t / d.nz;
+  const int j = (int)(ij % d.ny) + 1, i = (int)(ij / d.ny) + 1;
+  const int cp = i * d.sxp + j * d.syp + k;
+  const T pc = p[cp];
+  u[i * d.sxu + j * d.syu + k] -= dt * ((p[cp + d.sxp] - pc) * mt.rdx);
+  v[i * d.sxv + j * d.syv + k] -= dt * ((p[cp + d.syp] - pc) * mt.rdy);
+  if (k < d.nz) w[i * d.sxw + j * d.syw + k] -= dt * ((p[cp + 1] - pc) * mt.rdzc[k]);
+}
+
+// compute_bulk_velocity (utils.py:220-223) with an arbitrary strided float64 weight view.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, const double* __restrict__ wgt,
+                                              long long ws0, long long ws1, long long ws2, double inv_vol,
+                                              double* __restrict__ out, double* __restrict__ partials,
+                                              unsigned* __restrict__ counter) {
+  __shared__ double sh[4];
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  double s = 0.0;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long long)gridDim.x * 256) {
+    const int k = (int)(t % d.nz);
+    const long long ij = t / d.nz;
+    const int j = (int)(ij % d.ny), i = (int)(ij / d.ny);
+    s += (double)u[(i + 1) * d.sxu + (j + 1) * d.syu + k + 1] * wgt[i * ws0 + j * ws1 + k * ws2];
+  }
+  const double bs = block_reduce(s, false, sh);
+  if (!publish_last(partials, blockIdx.x, bs, counter, gridDim.x)) return;
+  const double tot = fold_partials(partials, gridDim.x, false, sh);
+  if (threadIdx.x == 0) {
+    *out = tot * inv_vol;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// compute_cfl_fused (operators.py:1262-1331): per-cell convective rates in x, y, z; maximum.
+// With update_dt, the last block also applies compute_cfl_dt (solver.py:483-494) and the 5% rule
+// (solver.py:811-812) to the scalar block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_cfl(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                             const T* __restrict__ v, const T* __restrict__ w,
+                                             double* __restrict__ scal, double* __restrict__ partials,
+                                             unsigned* __restrict__ counter, int update_dt, double cfl,
+                                             double dt_min, double dt_max) {
+  __shared__ double sh[4];
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  double best = 0.0;
+  const T q = T(0.25);
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long long)gridDim.x * 256) {
+    const int k = (int)(t % d.nz) + 1;
+    const long long ij = t / d.nz;
+    const int j = (int)(ij % d.ny) + 1, i = (int)(ij / d.ny) + 1;
+    const int cu = i * d.sxu + j * d.syu + k, cv = i * d.sxv + j * d.syv + k, cw = i * d.sxw + j * d.syw + k;
+    const T rzf = mt.rdzf[k - 1];
+    const T ex = fabs(u[cu]) * mt.rdx +
+                 q * fabs(v[cv] + v[cv - d.syv] + v[cv + d.sxv] + v[cv + d.sxv - d.syv]) * mt.rdy +
+                 q * fabs(w[cw] + w[cw - 1] + w[cw + d.sxw] + w[cw + d.sxw - 1]) * rzf;
+    const T ey = q * fabs(u[cu] + u[cu + d.syu] + u[cu - d.sxu + d.syu] + u[cu - d.sxu]) * mt.rdx +
+                 fabs(v[cv]) * mt.rdy +
+                 q * fabs(w[cw] + w[cw + d.syw] + w[cw + d.syw - 1] + w[cw - 1]) * rzf;
+    double m = fmax((double)ex, (double)ey);
+    if (k < d.nz) {
+      const T ez = q * fabs(u[cu] + u[cu - d.sxu] + u[cu - d.sxu + 1] + u[cu + 1]) * mt.rdx +
+                   q * fabs(v[cv] + v[cv - d.syv] + v[cv - d.syv + 1] + v[cv + 1]) * mt.rdy +
+                   fabs(w[cw]) * mt.rdzc[k];
+      m = fmax(m, (double)ez);
+    }
+    best = fmax(best, m);
+  }
+  const double bm = block_reduce(best, true, sh);
+  if (!publish_last(partials, blockIdx.x, bm, counter, gridDim.x)) return;
+  const double rate = fold_partials(partials, gridDim.x, true, sh);
+  if (threadIdx.x == 0) {
+    scal[TC_S_CFL] = rate;
+    if (update_dt) {
+      const double dti = rate < 1e-10 ? 1.0 : rate;
+      double dt_new = cfl * (1.0 / dti);
+      dt_new = fmin(fmax(dt_new, dt_min), dt_max);
+      const double dt = scal[TC_S_DT];
+      if (fabs(dt_new - dt) / dt > 0.05) scal[TC_S_DT] = dt_new;
+    }
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// compute_u_tau (utils.py:187-204): means over the whole k=1 and k=nz planes of u (ghosts too).
+template <typename T>
+__global__ __launch_bounds__(256) void k_u_tau(Dims d, const T* __restrict__ u, double nu, double dist,
+                                               double* __restrict__ out, double* __restrict__ partials,
+                                               unsigned* __restrict__ counter) {
+  __shared__ double sh[4];
+  const int P = (d.nx + 1) * (d.ny + 2);
+  double sb = 0.0, st = 0.0;
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < P; t += gridDim.x * 256) {
+    sb += (double)u[t * d.syu + 1];
+    st += (double)u[t * d.syu + d.nz];
+  }
+  const double b1 = block_reduce(sb, false, sh);
+  const double b2 = block_reduce(st, false, sh);
+  __shared__ bool last;
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&partials[2 * blockIdx.x], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&partials[2 * blockIdx.x + 1], b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (threadIdx.x == 0) {
+    double tb = 0.0, tt = 0.0;
+    for (unsigned q = 0; q < gridDim.x; ++q) {
+      tb += __hip_atomic_load(&partials[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      tt += __hip_atomic_load(&partials[2 * q + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const double ub = sqrt(fabs(nu * (tb / P) / dist));
+    if (d.top > 0) *out = ub;
+    else *out = 0.5 * (ub + sqrt(fabs(nu * (tt / P) / dist)));
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Turbulence statistics (turbstats.py:114-228).  Pass A: xy means of the cell-centred u, v, w per
+// z.  Pass B: xy means of the fluctuation products, and the fluctuation planes at the two
+// sampled heights (top-wall w negated).  Both passes: block b accumulates columns b, b+nb, ...
+// with one thread per z row, then the last block folds the per-block rows in order.
+// ---------------------------------------------------------------------------------------------
+template <typename T, bool PASS_B>
+__global__ __launch_bounds__(256) void k_stats(Dims d, const T* __restrict__ u, const T* __restrict__ v,
+                                               const T* __restrict__ w, double* __restrict__ means,
+                                               double* __restrict__ prof, double* __restrict__ planes,
+                                               int kb, int kt, double* __restrict__ partials,
+                                               unsigned* __restrict__ counter) {
+  const int k = threadIdx.x;
+  const int nz = d.nz, nx = d.nx, ny = d.ny;
+  constexpr int NQ = PASS_B ? 4 : 3;
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  double U = 0.0, V = 0.0, W = 0.0;
+  if (PASS_B && k < nz) { U = means[k]; V = means[nz + k]; W = means[2 * nz + k]; }
+  if (k < nz) {
+    for (int c = blockIdx.x; c < nx * ny; c += gridDim.x) {
+      const int i = c / ny, j = c % ny;
+      const double uc = 0.5 * ((double)u[i * d.sxu + (j + 1) * d.syu + k + 1] + (double)u[(i + 1) * d.sxu + (j + 1) * d.syu + k + 1]);
+      const double vc = 0.5 * ((double)v[(i + 1) * d.sxv + j * d.syv + k + 1] + (double)v[(i + 1) * d.sxv + (j + 1) * d.syv + k + 1]);
+      const double wc = 0.5 * ((double)w[(i + 1) * d.sxw + (j + 1) * d.syw + k] + (double)w[(i + 1) * d.sxw + (j + 1) * d.syw + k + 1]);
+      if constexpr (!PASS_B) {
+        acc[0] += uc; acc[1] += vc; acc[2] += wc;
+      } else {
+        const double uf = uc - U, vf = vc - V, wf = wc - W;
+        acc[0] += uf * uf; acc[1] += vf * vf; acc[2] += wf * wf; acc[3] += uf * wf;
+        if (k == kb - 1) {
+          double* pl = planes + ((size_t)i * ny + j) * 6;
+          pl[0] = uf; pl[1] = vf; pl[2] = wf;
+        }
+        if (k == kt - 1) {
+          double* pl = planes + ((size_t)i * ny + j) * 6;
+          pl[3] = uf; pl[4] = vf; pl[5] = -wf;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) partials[((size_t)blockIdx.x * NQ + q) * nz + k] = acc[q];
+  }
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (k < nz) {
+    const double inv = 1.0 / ((double)nx * (double)ny);
+    double tot[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
+    for (unsigned b = 0; b < gridDim.x; ++b)
+#pragma unroll
+      for (int q = 0; q < NQ; ++q)
+        tot[q] += __hip_atomic_load(&partials[((size_t)b * NQ + q) * nz + k], __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    if constexpr (!PASS_B) {
+      means[k] = tot[0] * inv; means[nz + k] = tot[1] * inv; means[2 * nz + k] = tot[2] * inv;
+    } else {
+      prof[k] += means[k];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) prof[(q + 1) * nz + k] += tot[q] * inv;
+    }
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Folded, wall-averaged spectra (turbstats.py:176-225) from the plane spectra (nkx, ny, 6): our
+// half spectrum keeps kx >= 0 and all ky, so the reference's E(-kx, ky) is E(kx, ny-ky) here.
+__global__ __launch_bounds__(256) void k_spectra(int nx, int ny, const double2* __restrict__ ps,
+                                                 double* __restrict__ spec) {
+  const int hx = nx / 2, hy = ny / 2;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= hx * hy) return;
+  const int a = t / hy, bq = t % hy;
+  const int kx = a + 1, ky = bq + 1;
+  const double nrm = ((double)nx * (double)ny) * ((double)nx * (double)ny);
+  double e[2][4];
+  const int jy[2] = {ky, ny - ky};
+  for (int s = 0; s < 2; ++s) {
+    const double2* z = ps + ((size_t)kx * ny + jy[s]) * 6;
+    double eb[4], et[4];
+    for (int side = 0; side < 2; ++side) {
+      const double2 U = z[3 * side], V = z[3 * side + 1], W = z[3 * side + 2];
+      double* o = side == 0 ? eb : et;
+      o[0] = (U.x * U.x + U.y * U.y) / nrm;
+      o[1] = (V.x * V.x + V.y * V.y) / nrm;
+      o[2] = (W.x * W.x + W.y * W.y) / nrm;
+      o[3] = (U.x * W.x + U.y * W.y) / nrm;
+    }
+    for (int q = 0; q < 4; ++q) e[s][q] = 0.5 * (eb[q] + et[q]);
+  }
+  for (int q = 0; q < 4; ++q) spec[(size_t)q * hx * hy + t] += e[0][q] + e[1][q];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------------
+int launch_apply_bc(const Grid* gr, void* u, void* v, void* w, hipStream_t st) {
+  const Dims& d = gr->d;
+  const int blocks = (3 * (d.nx + 2) * (d.ny + 2) + 3) / 4;
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_apply_bc<double>, dim3(blocks), dim3(256), 0, st, d, (double*)u, (double*)v, (double*)w);
+  else
+    hipLaunchKernelGGL(k_apply_bc<float>, dim3(blocks), dim3(256), 0, st, d, (float*)u, (float*)v, (float*)w);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_divergence(const Grid* gr, const void* u, const void* v, const void* w, void* div, hipStream_t st) {
+  const Dims& d = gr->d;
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  const int blocks = (int)((N + 255) / 256);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_divergence<double>, dim3(blocks), dim3(256), 0, st, d, gr->metric<double>(),
+                       (const double*)u, (const double*)v, (const double*)w, (double*)div);
+  else
+    hipLaunchKernelGGL(k_divergence<float>, dim3(blocks), dim3(256), 0, st, d, gr->metric<float>(),
+                       (const float*)u, (const float*)v, (const float*)w, (float*)div);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_project(const Grid* gr, void* u, void* v, void* w, const void* p, double dt_host,
+                   const double* dt_dev, hipStream_t st) {
+  const Dims& d = gr->d;
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  const int blocks = (int)((N + 255) / 256);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_project<double>, dim3(blocks), dim3(256), 0, st, d, gr->metric<double>(),
+                       (double*)u, (double*)v, (double*)w, (const double*)p, dt_host, dt_dev);
+  else
+    hipLaunchKernelGGL(k_project<float>, dim3(blocks), dim3(256), 0, st, d, gr->metric<float>(),
+                       (float*)u, (float*)v, (float*)w, (const float*)p, dt_host, dt_dev);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int cfl_blocks(const Grid* gr) {
+  const long long N = (long long)gr->d.nx * gr->d.ny * gr->d.nz;
+  return (int)std::min<long long>(1024, (N + 255) / 256);
+}
+
+int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, double* scal, double* partials,
+               unsigned* counter, int update_dt, double cfl_target, double dt_min, double dt_max,
+               int nblocks_max, hipStream_t st) {
+  const int blocks = std::min(cfl_blocks(gr), nblocks_max);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_cfl<double>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<double>(),
+                       (const double*)u, (const double*)v, (const double*)w, scal, partials, counter,
+                       update_dt, cfl_target, dt_min, dt_max);
+  else
+    hipLaunchKernelGGL(k_cfl<float>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<float>(),
+                       (const float*)u, (const float*)v, (const float*)w, scal, partials, counter,
+                       update_dt, cfl_target, dt_min, dt_max);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0, long long ws1,
+                long long ws2, double inv_vol, double* out, double* partials, unsigned* counter,
+                int nblocks_max, hipStream_t st) {
+  const long long N = (long long)gr->d.nx * gr->d.ny * gr->d.nz;
+  const int blocks = (int)std::min<long long>(nblocks_max, (N + 255) / 256);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_bulk<double>, dim3(blocks), dim3(256), 0, st, gr->d, (const double*)u, wgt, ws0,
+                       ws1, ws2, inv_vol, out, partials, counter);
+  else
+    hipLaunchKernelGGL(k_bulk<float>, dim3(blocks), dim3(256), 0, st, gr->d, (const float*)u, wgt, ws0, ws1,
+                       ws2, inv_vol, out, partials, counter);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double* partials,
+                 unsigned* counter, int nblocks_max, hipStream_t st) {
+  const int P = (gr->d.nx + 1) * (gr->d.ny + 2);
+  const int blocks = std::min(nblocks_max / 2, (P + 255) / 256);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_u_tau<double>, dim3(blocks), dim3(256), 0, st, gr->d, (const double*)u, gr->nu, dist,
+                       out, partials, counter);
+  else
+    hipLaunchKernelGGL(k_u_tau<float>, dim3(blocks), dim3(256), 0, st, gr->d, (const float*)u, gr->nu, dist,
+                       out, partials, counter);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+template <typename T>
+static int stats_t(const Poisson* pl, const void* u, const void* v, const void* w, int kb, int kt,
+                   double* prof, double* spec, double* means, double* partials, unsigned* counter,
+                   int nblocks, hipStream_t st) {
+  const Grid* gr = pl->g;
+  const Dims& d = gr->d;
+  hipLaunchKernelGGL((k_stats<T, false>), dim3(nblocks), dim3(256), 0, st, d, (const T*)u, (const T*)v,
+                     (const T*)w, means, prof, pl->planes, kb, kt, partials, counter);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_stats<T, true>), dim3(nblocks), dim3(256), 0, st, d, (const T*)u, (const T*)v,
+                     (const T*)w, means, prof, pl->planes, kb, kt, partials, counter);
+  TC_CHECK_LAUNCH();
+  int rc = plane_fft(pl, st);
+  if (rc != TC_OK) return rc;
+  const int n = (d.nx / 2) * (d.ny / 2);
+  hipLaunchKernelGGL(k_spectra, dim3((n + 255) / 256), dim3(256), 0, st, d.nx, d.ny,
+                     (const double2*)pl->plane_spec, spec);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_stats(const Poisson* pl, const void* u, const void* v, const void* w, int kb, int kt,
+                 double* prof, double* spec, double* means, double* partials, unsigned* counter,
+                 int nblocks, hipStream_t st) {
+  if (pl->g->dtype == TC_F64)
+    return stats_t<double>(pl, u, v, w, kb, kt, prof, spec, means, partials, counter, nblocks, st);
+  return stats_t<float>(pl, u, v, w, kb, kt, prof, spec, means, partials, counter, nblocks, st);
+}
+
+}  // namespace tc

@@ -1,0 +1,38 @@
+// Poisson plan (host object) shared by the stand-alone solve, the fused step and the statistics.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tc_fft.h"
+#include "tc_internal.h"
+
+namespace tc {
+
+struct Poisson {
+  const Grid* g = nullptr;
+  int nkx = 0;                 // nx/2 + 1
+  FftDesc fx, fy;
+  double2* twx = nullptr;      // exp(-2 pi i t/nx)
+  double2* twy = nullptr;
+  double* cl = nullptr;        // 1/(dz_c[k] dz_f[k])     (projection_fft.py:48)
+  double* cr = nullptr;        // 1/(dz_c[k+1] dz_f[k])   (projection_fft.py:49)
+  double* K2 = nullptr;        // kx~^2 + ky~^2 per (m, j) (projection_fft.py:30-39)
+  void* spec = nullptr;        // S[nkx][ny][nz] complex, working precision
+  int pairs = 1;               // z-row pairs per x-transform block
+  int kzy = 1;                 // z-rows per y-transform block
+  double* planes = nullptr;    // statistics: (nx, ny, 6) float64
+  void* plane_spec = nullptr;  // statistics: (nkx, ny, 6) complex float64
+};
+
+int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const double* dzc_host);
+void poisson_free(Poisson* pl);
+int poisson_solve(const Poisson* pl, bool from_vel, const void* u, const void* v, const void* w,
+                  const void* rin, const double* scal, void* p, hipStream_t st);
+int poisson_phase(const Poisson* pl, int ph, bool from_vel, const void* u, const void* v, const void* w,
+                  const void* rin, const double* scal, void* p, hipStream_t st);
+int plane_fft(const Poisson* pl, hipStream_t st);
+int pcr_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
+                long long s, const void* d, void* x, hipStream_t st);
+
+}  // namespace tc

@@ -1,0 +1,522 @@
+// FFT + wall-normal tridiagonal pressure Poisson solve (projection_fft.py:3-165).
+//
+// Spectral layout: S[m][j][k] complex, m = kx in [0, nx/2] (real-to-half transform along x),
+// j = ky index in [0, ny) (complex transform along y), k in [0, nz) contiguous.
+//   P1 k_fwd_x : div (or div/dt computed from the velocities) -> x half spectra, two z-rows per
+//                complex transform (real/imag packing)
+//   P2 k_fft_y : forward complex FFT along y
+//   P3 k_poisson_pcr : one wavefront per (kx, ky) system, PCR along z, (0,0) gauge-fixed in f64
+//   P4 k_fft_y : inverse complex FFT along y
+//   P5 k_inv_x : half spectra -> real rows, 1/(nx*ny), p written with every ghost image
+#include "tc_fft.h"
+#include "tc_internal.h"
+#include "tc_poisson.h"
+
+namespace tc {
+
+// ---------------------------------------------------------------------------------------------
+// P1: real rows along x -> half spectra.  Block = (y row j, chunk of KZ = 2*pairs z-rows).
+// from_vel: rhs = div(q)/dt with div of utils.py:235-240 on the post-CN velocities (whose
+// ghost columns are not stored: x-face 0 is the image of face nx, y-face 0 of face ny, and the
+// w walls are zero).  Otherwise rhs is read from a (nx, ny, nz_in) array (planes of nz_in).
+// ---------------------------------------------------------------------------------------------
+template <typename T, bool FROM_VEL>
+__global__ __launch_bounds__(256) void k_fwd_x(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                               const T* __restrict__ v, const T* __restrict__ w,
+                                               const T* __restrict__ rin, int nz_in,
+                                               const double* __restrict__ scal, FftDesc fx,
+                                               const double2* __restrict__ twx, cplx<T>* __restrict__ S,
+                                               int pairs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = d.nx, ny = d.ny;
+  const int KZ = 2 * pairs;
+  const int nch = (nz_in + KZ - 1) / KZ;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int j = blk / nch, k0 = (blk - j * nch) * KZ;
+  const int ldb = nx + 1;
+  T rdt = T(1);
+  if (FROM_VEL) rdt = (T)(1.0 / scal[TC_S_DT]);
+  for (int t = threadIdx.x; t < nx * KZ; t += 256) {
+    const int kk = t % KZ, i = t / KZ;
+    const int k = k0 + kk;
+    T val = T(0);
+    if (k < nz_in) {
+      if constexpr (FROM_VEL) {
+        const int iul = (i == 0) ? nx : i;
+        const int jvl = (j == 0) ? ny : j;
+        const T ur = u[(i + 1) * d.sxu + (j + 1) * d.syu + k + 1];
+        const T ul = u[iul * d.sxu + (j + 1) * d.syu + k + 1];
+        const T vr = v[(i + 1) * d.sxv + (j + 1) * d.syv + k + 1];
+        const T vl = v[(i + 1) * d.sxv + jvl * d.syv + k + 1];
+        const int cw = (i + 1) * d.sxw + (j + 1) * d.syw;
+        const T wt = (k + 1 < d.nz) ? w[cw + k + 1] : T(0);
+        const T wb = (k > 0) ? w[cw + k] : T(0);
+        val = ((ur - ul) * mt.rdx + (vr - vl) * mt.rdy + (wt - wb) * mt.rdzf[k]) * rdt;
+      } else {
+        val = rin[((size_t)i * ny + j) * nz_in + k];
+      }
+    }
+    T* zf = reinterpret_cast<T*>(&Z[(kk >> 1) * ldb + i]);
+    zf[kk & 1] = val;
+  }
+  __syncthreads();
+  lds_fft<T>(Z, ldb, pairs, fx, twx, -1);
+  const int nkx = nx / 2 + 1;
+  for (int t = threadIdx.x; t < nkx * KZ; t += 256) {
+    const int kk = t % KZ, m = t / KZ;
+    const int k = k0 + kk;
+    if (k >= nz_in) continue;
+    const cplx<T> z1 = Z[(kk >> 1) * ldb + m];
+    const cplx<T> z2 = Z[(kk >> 1) * ldb + (m == 0 ? 0 : nx - m)];
+    cplx<T> o;
+    if ((kk & 1) == 0) o = {T(0.5) * (z1.re + z2.re), T(0.5) * (z1.im - z2.im)};
+    else o = {T(0.5) * (z1.im + z2.im), T(-0.5) * (z1.re - z2.re)};
+    S[((size_t)m * ny + j) * nz_in + k] = o;
+  }
+}
+
+// P2 / P4: complex FFT along y for every (kx, z) column.  Block = (kx, chunk of KZ z-rows).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDesc fy,
+                                               const double2* __restrict__ twy,
+                                               cplx<T>* __restrict__ S, int KZ, int sign) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nch = (nz_in + KZ - 1) / KZ;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int m = blk / nch, k0 = (blk - m * nch) * KZ;
+  const int ldb = ny + 1;
+  for (int t = threadIdx.x; t < ny * KZ; t += 256) {
+    const int kk = t % KZ, j = t / KZ;
+    const int k = k0 + kk;
+    Y[kk * ldb + j] = k < nz_in ? S[((size_t)m * ny + j) * nz_in + k] : cplx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  lds_fft<T>(Y, ldb, KZ, fy, twy, sign);
+  for (int t = threadIdx.x; t < ny * KZ; t += 256) {
+    const int kk = t % KZ, j = t / KZ;
+    const int k = k0 + kk;
+    if (k < nz_in) S[((size_t)m * ny + j) * nz_in + k] = Y[kk * ldb + j];
+  }
+}
+
+// P5: half spectra -> real rows (irfft semantics: the imaginary parts of the kx = 0 and Nyquist
+// bins are dropped), scaled by 1/(nx*ny), written to p with the ghost images of
+// projection_fft.py:150-163 (x copies, y copies, Neumann z walls).
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_x(Dims d, FftDesc fx, const double2* __restrict__ twx,
+                                               const cplx<T>* __restrict__ S, T* __restrict__ p,
+                                               T scale, int pairs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int KZ = 2 * pairs;
+  const int nch = (nz + KZ - 1) / KZ;
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int j = blk / nch, k0 = (blk - j * nch) * KZ;
+  const int ldb = nx + 1;
+  const int h = nx / 2;
+  for (int t = threadIdx.x; t < (h + 1) * pairs; t += 256) {
+    const int q = t % pairs, m = t / pairs;
+    const int ka = k0 + 2 * q, kb = ka + 1;
+    cplx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
+    if (ka < nz) A = S[((size_t)m * ny + j) * nz + ka];
+    if (kb < nz) B = S[((size_t)m * ny + j) * nz + kb];
+    if (m == 0 || m == h) { A.im = T(0); B.im = T(0); }
+    Z[q * ldb + m] = {A.re - B.im, A.im + B.re};
+    if (m > 0 && m < h) Z[q * ldb + nx - m] = {A.re + B.im, B.re - A.im};
+  }
+  __syncthreads();
+  lds_fft<T>(Z, ldb, pairs, fx, twx, +1);
+  for (int t = threadIdx.x; t < nx * KZ; t += 256) {
+    const int kk = t % KZ, i = t / KZ;
+    const int k = k0 + kk;
+    if (k >= nz) continue;
+    const cplx<T> z = Z[(kk >> 1) * ldb + i];
+    const T val = ((kk & 1) ? z.im : z.re) * scale;
+    const int I = i + 1, J = j + 1, K = k + 1;
+    int xs[3], ys[3], zs[3], ni = 0, nj = 0, nk = 0;
+    xs[ni++] = I;
+    if (I == nx) xs[ni++] = 0;
+    if (I == 1) xs[ni++] = nx + 1;
+    ys[nj++] = J;
+    if (J == ny) ys[nj++] = 0;
+    if (J == 1) ys[nj++] = ny + 1;
+    zs[nk++] = K;
+    if (K == 1) zs[nk++] = 0;
+    if (K == nz) zs[nk++] = nz + 1;
+    for (int a = 0; a < ni; ++a)
+      for (int bq = 0; bq < nj; ++bq)
+        for (int c = 0; c < nk; ++c) p[xs[a] * d.sxp + ys[bq] * d.syp + zs[c]] = val;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// General PCR (tridiag.py:40-58) on registers: a, b, c, d rows m = r*64 + lane; rows >= n are
+// padding (a = c = d = 0, b = 1), the same values the reference pads out-of-range rows with.
+// ---------------------------------------------------------------------------------------------
+template <int ST, int R, typename U, bool CPLX>
+__device__ __forceinline__ void pcr_stage(U (&a)[R], U (&b)[R], U (&c)[R], U (&dr)[R], U (&di)[R], int n,
+                                          int lane) {
+  constexpr int s = 1 << ST;
+  if (s >= n) return;
+  U adn[R], aup[R], bdn[R], bup[R], cdn[R], cup[R], rdn[R], rup[R], idn[R], iup[R];
+  shift_dn<s, R, U>(a, adn, U(0), lane);
+  shift_up<s, R, U>(a, aup, U(0), lane);
+  shift_dn<s, R, U>(b, bdn, U(1), lane);
+  shift_up<s, R, U>(b, bup, U(1), lane);
+  shift_dn<s, R, U>(c, cdn, U(0), lane);
+  shift_up<s, R, U>(c, cup, U(0), lane);
+  shift_dn<s, R, U>(dr, rdn, U(0), lane);
+  shift_up<s, R, U>(dr, rup, U(0), lane);
+  if constexpr (CPLX) {
+    shift_dn<s, R, U>(di, idn, U(0), lane);
+    shift_up<s, R, U>(di, iup, U(0), lane);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const U al = -a[r] / bdn[r];
+    const U ga = -c[r] / bup[r];
+    b[r] = b[r] + al * cdn[r] + ga * aup[r];
+    dr[r] = dr[r] + al * rdn[r] + ga * rup[r];
+    if constexpr (CPLX) di[r] = di[r] + al * idn[r] + ga * iup[r];
+    a[r] = al * adn[r];
+    c[r] = ga * cup[r];
+  }
+}
+
+template <int R, typename U, bool CPLX>
+__device__ __forceinline__ void pcr_general(U (&a)[R], U (&b)[R], U (&c)[R], U (&dr)[R], U (&di)[R], int n,
+                                            int lane) {
+  pcr_stage<0, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<1, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<2, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<3, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<4, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<5, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<6, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<7, R, U, CPLX>(a, b, c, dr, di, n, lane);
+  pcr_stage<8, R, U, CPLX>(a, b, c, dr, di, n, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    dr[r] = dr[r] / b[r];
+    if constexpr (CPLX) di[r] = di[r] / b[r];
+  }
+}
+
+// P3 body for one (kx, ky) system in compute type U (float64 for the gauge mode).
+// Rows of projection_fft.py:44-82: a[k] = cl[k] (k >= 1), c[k] = cr[k] (k <= nz-2),
+// b[k] = -((cl[k] + cr[k]) + K2) (+ cl[0] at k = 0, + cr[nz-1] at k = nz-1).
+template <typename T, typename U, int R>
+__device__ __forceinline__ void poisson_system(int nz, const double* __restrict__ cl,
+                                               const double* __restrict__ cr, double k2,
+                                               cplx<T>* __restrict__ col, bool gauge, int lane) {
+  U a[R], b[R], c[R], dr[R], di[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m < nz) {
+      const double l = cl[m], rr = cr[m];
+      double bb = -((l + rr) + k2);
+      if (m == 0) bb += cl[0];
+      if (m == nz - 1) bb += cr[nz - 1];
+      a[r] = m >= 1 ? (U)l : U(0);
+      c[r] = m <= nz - 2 ? (U)rr : U(0);
+      b[r] = (U)bb;
+      const cplx<T> z = col[m];
+      dr[r] = (U)z.re;
+      di[r] = (U)z.im;
+      if (gauge && m == 0) { a[r] = U(0); b[r] = U(1); c[r] = U(0); dr[r] = U(0); di[r] = U(0); }
+      if (gauge && m == 1) a[r] = U(0);
+    } else {
+      a[r] = U(0); b[r] = U(1); c[r] = U(0); dr[r] = U(0); di[r] = U(0);
+    }
+  }
+  pcr_general<R, U, true>(a, b, c, dr, di, nz, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m < nz) col[m] = {(T)dr[r], (T)di[r]};
+  }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_poisson_pcr(int nkx, int ny, int nz, const double* __restrict__ cl,
+                                                     const double* __restrict__ cr,
+                                                     const double* __restrict__ K2,
+                                                     cplx<T>* __restrict__ S) {
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int sys = b * 4 + (threadIdx.x >> 6);
+  if (sys >= nkx * ny) return;
+  const int lane = lane_id();
+  cplx<T>* col = S + (size_t)sys * nz;
+  if (sys == 0) poisson_system<T, double, R>(nz, cl, cr, K2[0], col, true, lane);
+  else poisson_system<T, T, R>(nz, cl, cr, K2[sys], col, false, lane);
+}
+
+// Generic batched PCR for the stand-alone tridiagonal entry points.
+template <typename T, int R, bool CPLX>
+__global__ __launch_bounds__(256) void k_pcr_general(int n, int batch, const T* __restrict__ A,
+                                                     const T* __restrict__ B, const T* __restrict__ C,
+                                                     long long abc_stride, const T* __restrict__ D,
+                                                     T* __restrict__ X) {
+  const int sys = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (sys >= batch) return;
+  const int lane = lane_id();
+  const T* a0 = A + abc_stride * sys;
+  const T* b0 = B + abc_stride * sys;
+  const T* c0 = C + abc_stride * sys;
+  T a[R], b[R], c[R], dr[R], di[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m < n) {
+      a[r] = m > 0 ? a0[m] : T(0);
+      b[r] = b0[m];
+      c[r] = m < n - 1 ? c0[m] : T(0);
+      if (CPLX) {
+        dr[r] = D[2 * ((size_t)sys * n + m)];
+        di[r] = D[2 * ((size_t)sys * n + m) + 1];
+      } else {
+        dr[r] = D[(size_t)sys * n + m];
+        di[r] = T(0);
+      }
+    } else {
+      a[r] = T(0); b[r] = T(1); c[r] = T(0); dr[r] = T(0); di[r] = T(0);
+    }
+  }
+  pcr_general<R, T, CPLX>(a, b, c, dr, di, n, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m < n) {
+      if (CPLX) {
+        X[2 * ((size_t)sys * n + m)] = dr[r];
+        X[2 * ((size_t)sys * n + m) + 1] = di[r];
+      } else {
+        X[(size_t)sys * n + m] = dr[r];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------------------------
+static bool factor_fft(int n, FftDesc& f) {
+  f.n = n;
+  f.npass = 0;
+  int m = n;
+  while (m % 4 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 4; m /= 4; }
+  while (m % 2 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 2; m /= 2; }
+  while (m % 3 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 3; m /= 3; }
+  while (m % 5 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 5; m /= 5; }
+  return m == 1;
+}
+
+static int pow2_floor(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const double* dzc_host) {
+  pl->g = g;
+  const Dims& d = g->d;
+  pl->nkx = d.nx / 2 + 1;
+  if (!factor_fft(d.nx, pl->fx) || !factor_fft(d.ny, pl->fy)) return TC_ERR_SHAPE;
+  if (d.nx > TC_FFT_ELEMS || d.ny > TC_FFT_ELEMS) return TC_ERR_SHAPE;
+  pl->pairs = std::min(8, pow2_floor(std::max(1, TC_FFT_ELEMS / d.nx)));
+  pl->pairs = std::min(pl->pairs, pow2_floor(std::max(1, (d.nz + 1) / 2)));
+  pl->kzy = std::min(16, pow2_floor(std::max(1, TC_FFT_ELEMS / d.ny)));
+  pl->kzy = std::min(pl->kzy, pow2_floor(d.nz));
+  // twiddles exp(-2 pi i t / N) in float64
+  auto make_tw = [](int n, double2** out) -> int {
+    std::vector<double2> h(n);
+    for (int t = 0; t < n; ++t) {
+      const double ang = -2.0 * M_PI * (double)t / (double)n;
+      h[t].x = std::cos(ang);
+      h[t].y = std::sin(ang);
+    }
+    if (hipMalloc(out, sizeof(double2) * n) != hipSuccess) return TC_ERR_NOMEM;
+    if (hipMemcpy(*out, h.data(), sizeof(double2) * n, hipMemcpyHostToDevice) != hipSuccess) return TC_ERR_NOMEM;
+    return TC_OK;
+  };
+  int rc;
+  if ((rc = make_tw(d.nx, &pl->twx)) != TC_OK) return rc;
+  if ((rc = make_tw(d.ny, &pl->twy)) != TC_OK) return rc;
+  // wall-normal rows (projection_fft.py:44-49) and modified wavenumbers (26-39)
+  const int nz = d.nz;
+  std::vector<double> cl(nz), cr(nz);
+  for (int k = 0; k < nz; ++k) {
+    cl[k] = 1.0 / (dzc_host[k] * dzf_host[k]);
+    cr[k] = 1.0 / (dzc_host[k + 1] * dzf_host[k]);
+  }
+  const int nkx = pl->nkx, ny = d.ny, nx = d.nx;
+  std::vector<double> k2((size_t)nkx * ny);
+  const double two_pi = 2.0 * M_PI;
+  for (int m = 0; m < nkx; ++m) {
+    const double kx = (double)m / ((double)nx * (g->dx / two_pi));
+    const double kxm = (2.0 / g->dx) * std::sin(kx * g->dx / 2.0);
+    for (int j = 0; j < ny; ++j) {
+      const int jj = j <= ny / 2 ? j : ny - j;
+      const double ky = (double)jj / ((double)ny * (g->dy / two_pi));
+      const double kym = (2.0 / g->dy) * std::sin(ky * g->dy / 2.0);
+      k2[(size_t)m * ny + j] = kxm * kxm + kym * kym;
+    }
+  }
+  if (hipMalloc(&pl->cl, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->cr, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->K2, sizeof(double) * k2.size()) != hipSuccess) return TC_ERR_NOMEM;
+  hipMemcpy(pl->cl, cl.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
+  hipMemcpy(pl->cr, cr.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
+  hipMemcpy(pl->K2, k2.data(), sizeof(double) * k2.size(), hipMemcpyHostToDevice);
+  const size_t esz = g->dtype == TC_F64 ? 16 : 8;
+  if (hipMalloc(&pl->spec, esz * (size_t)nkx * ny * nz) != hipSuccess) return TC_ERR_NOMEM;
+  // statistics workspace: 6 planes (u, v, w at the two sampled heights), float64
+  if (hipMalloc(&pl->planes, sizeof(double) * (size_t)nx * ny * 6) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->plane_spec, 16 * (size_t)nkx * ny * 6) != hipSuccess) return TC_ERR_NOMEM;
+  return TC_OK;
+}
+
+void poisson_free(Poisson* pl) {
+  hipFree(pl->twx); hipFree(pl->twy); hipFree(pl->cl); hipFree(pl->cr); hipFree(pl->K2);
+  hipFree(pl->spec); hipFree(pl->planes); hipFree(pl->plane_spec);
+}
+
+template <typename T>
+int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* v, const void* w,
+                   const void* rin, int nz_in, const double* scal, void* spec, int pairs, hipStream_t st) {
+  const Grid* g = pl->g;
+  const Dims& d = g->d;
+  const int KZ = 2 * pairs;
+  const int blocks = d.ny * ((nz_in + KZ - 1) / KZ);
+  const size_t lds = sizeof(cplx<T>) * (size_t)pairs * (d.nx + 1);
+  const Metric<T> mt = g->metric<T>();
+  if (from_vel)
+    hipLaunchKernelGGL((k_fwd_x<T, true>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,
+                       (const T*)v, (const T*)w, (const T*)rin, nz_in, scal, pl->fx, pl->twx,
+                       (cplx<T>*)spec, pairs);
+  else
+    hipLaunchKernelGGL((k_fwd_x<T, false>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,
+                       (const T*)v, (const T*)w, (const T*)rin, nz_in, scal, pl->fx, pl->twx,
+                       (cplx<T>*)spec, pairs);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+template <typename T>
+int launch_fft_y_t(const Poisson* pl, void* spec, int nz_in, int kz, int sign, hipStream_t st) {
+  const Dims& d = pl->g->d;
+  const int blocks = pl->nkx * ((nz_in + kz - 1) / kz);
+  const size_t lds = sizeof(cplx<T>) * (size_t)kz * (d.ny + 1);
+  hipLaunchKernelGGL((k_fft_y<T>), dim3(blocks), dim3(256), lds, st, pl->nkx, d.ny, nz_in, pl->fy,
+                     pl->twy, (cplx<T>*)spec, kz, sign);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+// One stage of the Poisson solve: 0 = P1 forward x, 1 = P2 forward y, 2 = P3 PCR, 3 = P4 inverse y,
+// 4 = P5 inverse x + ghosts.
+template <typename T>
+static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void* u, const void* v,
+                           const void* w, const void* rin, const double* scal, void* p, hipStream_t st) {
+  const Grid* g = pl->g;
+  const Dims& d = g->d;
+  switch (ph) {
+    case 0: return launch_fwd_x_t<T>(pl, from_vel, u, v, w, rin, d.nz, scal, pl->spec, pl->pairs, st);
+    case 1: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, -1, st);
+    case 2: {
+      const int nsys = pl->nkx * d.ny;
+      const int blocks = (nsys + 3) / 4;
+      const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
+#define TC_P3(RR)                                                                                 \
+  hipLaunchKernelGGL((k_poisson_pcr<T, RR>), dim3(blocks), dim3(256), 0, st, pl->nkx, d.ny, d.nz, \
+                     pl->cl, pl->cr, pl->K2, (cplx<T>*)pl->spec)
+      switch (R) {
+        case 1: TC_P3(1); break;
+        case 2: TC_P3(2); break;
+        case 3: TC_P3(3); break;
+        case 4: TC_P3(4); break;
+        default: return TC_ERR_UNSUPPORTED;
+      }
+#undef TC_P3
+      TC_CHECK_LAUNCH();
+      return TC_OK;
+    }
+    case 3: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, +1, st);
+    case 4: {
+      const int KZ = 2 * pl->pairs;
+      const int blocks5 = d.ny * ((d.nz + KZ - 1) / KZ);
+      const size_t lds = sizeof(cplx<T>) * (size_t)pl->pairs * (d.nx + 1);
+      const T scale = (T)(1.0 / ((double)d.nx * (double)d.ny));
+      hipLaunchKernelGGL((k_inv_x<T>), dim3(blocks5), dim3(256), lds, st, d, pl->fx, pl->twx,
+                         (const cplx<T>*)pl->spec, (T*)p, scale, pl->pairs);
+      TC_CHECK_LAUNCH();
+      return TC_OK;
+    }
+    default: return TC_ERR_ARG;
+  }
+}
+
+int poisson_phase(const Poisson* pl, int ph, bool from_vel, const void* u, const void* v, const void* w,
+                  const void* rin, const double* scal, void* p, hipStream_t st) {
+  if (pl->g->dtype == TC_F64) return poisson_phase_t<double>(pl, ph, from_vel, u, v, w, rin, scal, p, st);
+  return poisson_phase_t<float>(pl, ph, from_vel, u, v, w, rin, scal, p, st);
+}
+
+int poisson_solve(const Poisson* pl, bool from_vel, const void* u, const void* v, const void* w,
+                  const void* rin, const double* scal, void* p, hipStream_t st) {
+  for (int ph = 0; ph < 5; ++ph) {
+    const int rc = poisson_phase(pl, ph, from_vel, u, v, w, rin, scal, p, st);
+    if (rc != TC_OK) return rc;
+  }
+  return TC_OK;
+}
+
+// Plane FFTs for the statistics: planes (nx, ny, 6) float64 -> folded spectra inputs.
+int plane_fft(const Poisson* pl, hipStream_t st) {
+  int rc;
+  const int pairs = std::min(3, pow2_floor(std::max(1, TC_FFT_ELEMS / pl->g->d.nx)));
+  if ((rc = launch_fwd_x_t<double>(pl, false, nullptr, nullptr, nullptr, pl->planes, 6, nullptr,
+                                   pl->plane_spec, pairs, st)) != TC_OK)
+    return rc;
+  const int kz = std::min(pow2_floor(std::max(1, TC_FFT_ELEMS / pl->g->d.ny)), 4);
+  return launch_fft_y_t<double>(pl, pl->plane_spec, 6, kz, -1, st);
+}
+
+template <typename T, int R, bool CPLX>
+static void launch_pcr_gen(int n, int batch, const void* a, const void* b, const void* c, long long s,
+                           const void* d, void* x, hipStream_t st) {
+  hipLaunchKernelGGL((k_pcr_general<T, R, CPLX>), dim3((batch + 3) / 4), dim3(256), 0, st, n, batch,
+                     (const T*)a, (const T*)b, (const T*)c, s, (const T*)d, (T*)x);
+}
+
+template <typename T, bool CPLX>
+static int pcr_dispatch_r(int n, int batch, const void* a, const void* b, const void* c, long long s,
+                          const void* d, void* x, hipStream_t st) {
+  const int R = (n + TC_WAVE - 1) / TC_WAVE;
+  switch (R) {
+    case 1: launch_pcr_gen<T, 1, CPLX>(n, batch, a, b, c, s, d, x, st); break;
+    case 2: launch_pcr_gen<T, 2, CPLX>(n, batch, a, b, c, s, d, x, st); break;
+    case 3: launch_pcr_gen<T, 3, CPLX>(n, batch, a, b, c, s, d, x, st); break;
+    case 4: launch_pcr_gen<T, 4, CPLX>(n, batch, a, b, c, s, d, x, st); break;
+    case 5: case 6: launch_pcr_gen<T, 6, CPLX>(n, batch, a, b, c, s, d, x, st); break;
+    case 7: case 8: launch_pcr_gen<T, 8, CPLX>(n, batch, a, b, c, s, d, x, st); break;
+    default: return TC_ERR_UNSUPPORTED;
+  }
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int pcr_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
+                long long s, const void* d, void* x, hipStream_t st) {
+  if (dtype == TC_F64)
+    return cplx_d ? pcr_dispatch_r<double, true>(n, batch, a, b, c, s, d, x, st)
+                  : pcr_dispatch_r<double, false>(n, batch, a, b, c, s, d, x, st);
+  return cplx_d ? pcr_dispatch_r<float, true>(n, batch, a, b, c, s, d, x, st)
+                : pcr_dispatch_r<float, false>(n, batch, a, b, c, s, d, x, st);
+}
+
+}  // namespace tc

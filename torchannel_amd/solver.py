@@ -1,0 +1,274 @@
+"""ChannelFlow driver and apply_bc_all (solver.py) on the HIP library.
+
+The driver keeps the reference's configuration keys, attributes and step semantics; every step
+runs the fused HIP path:
+
+  k_cn_tables  -> CN/PCR coefficient tables for the current dt (device scalar)
+  k_explicit_cn-> explicit RHS + forcing, AB2, Crank-Nicolson z-diffusion      (solver.py:588-668)
+  k_fwd_x, k_fft_y, k_poisson_pcr, k_fft_y, k_inv_x -> pressure Poisson solve (solver.py:672-682)
+  k_project_bc -> projection, boundary images, bulk velocity, forcing update     (solver.py:692-708)
+
+dt, the forcing and the bulk velocity live in a float64 device scalar block, so steps need no
+host synchronisation and can be captured in a HIP graph (run_graph).
+"""
+import ctypes
+import math
+import os
+import time as _time
+
+import numpy as np
+import torch
+import yaml
+
+from . import _lib as L
+from .initflow import initialize_flow
+from .projection_fft import initialize_fft_solver
+from .turbstats import TurbulenceStats
+from .utils import compute_divergence, compute_u_tau, generate_grid
+
+
+def apply_bc_all(u, v, w, top_wall_bc_type="dirichlet"):
+    """solver.py:23-79, in place."""
+    dt = L.require_device(u, v, w)
+    nx, ny, nz = u.shape[0] - 1, u.shape[1] - 2, u.shape[2] - 2
+    g = L.get_grid(nx, ny, nz, 1.0, 1.0, 0.0, L.top_code(top_wall_bc_type), dt, u.device)
+    L.call("tc_apply_bc", g.handle, L.ptr(u), L.ptr(v), L.ptr(w), L.stream_of(u))
+
+
+class ChannelFlow:
+    """Drop-in for solver.py:81-970's IMEX path.
+
+    Configuration: the reference's YAML keys (grid, domain, flow, initialization, time, compute,
+    output, statistics, boundary_conditions) plus ``compute.precision`` ('fp64' | 'fp32') and
+    ``initialization.seed`` (torch.manual_seed on the CPU generator before the noise draws) and
+    ``compute.rhs_branch`` ('gpu' = the fused RHS the reference runs on any HIP/CUDA device,
+    'cpu' = the separate kernels of its CPU branch, solver.py:593-622).
+    ``config`` may be given as a dict instead of a file path.
+    """
+
+    def __init__(self, config_file="config.yaml", config=None, verbose=False):
+        if config is None:
+            with open(config_file, "r") as f:
+                config = yaml.safe_load(f)
+        self.config = config
+        self.verbose = verbose
+        dev = config.get("compute", {}).get("device", "auto")
+        if dev == "cpu" or not torch.cuda.is_available():
+            raise RuntimeError("torchannel_amd.ChannelFlow runs on a HIP device only")
+        self.device = torch.device("cuda")
+        prec = config.get("compute", {}).get("precision", "fp64")
+        if prec not in ("fp64", "fp32"):
+            raise ValueError(f"Invalid precision: {prec}")
+        self.precision = prec
+        branch = config.get("compute", {}).get("rhs_branch", "gpu")
+        if branch not in ("gpu", "cpu"):
+            raise ValueError(f"Invalid rhs_branch: {branch}")
+        self.rhs_branch = branch
+        self.wdtype = torch.float64 if prec == "fp64" else torch.float32
+
+        g = config["grid"]
+        self.nx, self.ny, self.nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+        d = config["domain"]
+        self.Lx, self.Ly, self.Lz = float(d["Lx"]), float(d["Ly"]), float(d["Lz"])
+        self.stretching_type = d.get("stretching_type", "symmetric")
+        if self.stretching_type not in ("symmetric", "bottom"):
+            raise ValueError(f"Invalid stretching type: {self.stretching_type}")
+        fl = config["flow"]
+        self.nu = 1.0 / fl["Re"]
+        self.Re_tau = fl.get("Re_tau", 180.0)
+        self.U_bulk = float(fl["U_bulk"])
+        self.gamma = float(fl["gamma"])
+        self.top_wall_bc_type = config.get("boundary_conditions", {}).get("top_wall", {}).get("type", "dirichlet")
+        L.top_code(self.top_wall_bc_type)
+        t = config["time"]
+        self.dt = float(t["dt"])
+        self.n_steps = int(t["n_steps"])
+        self.t_max = t.get("t_max", 1000.0)
+        self.cfl_target = t.get("CFL_target", 0.25)
+        self.dt_update_interval = t.get("dt_update_interval", 0)
+        self.dt_max = t.get("dt_max", 0.01)
+        self.dt_min = t.get("dt_min", 0.0001)
+        self.time_scheme = t.get("scheme", "IMEX")
+        out = config.get("output", {})
+        self.results_folder = out.get("results_folder", "results")
+        self.n_out = out.get("n_out", out.get("print_interval", 10))
+        self.n_save = out.get("n_save", out.get("slice_interval", 100))
+
+        # grid (utils.py:5-45), built on the host in float64 and moved to the device
+        z_f, z_c, dz_f, dz_c = generate_grid(self.gamma, self.nz, self.Lz, device="cpu",
+                                             stretching_type=self.stretching_type)
+        self.z_f, self.z_c, self.dz_f, self.dz_c = (x.to(self.device) for x in (z_f, z_c, dz_f, dz_c))
+        self.dx, self.dy = self.Lx / self.nx, self.Ly / self.ny
+        self.cell_vol = (self.dx * self.dy * self.dz_f.view(1, 1, -1)).expand(self.nx, self.ny, self.nz)
+        self.cell_vol_ratio = self.cell_vol
+        self.total_volume = self.Lx * self.Ly * self.Lz
+
+        ini = config["initialization"]
+        self.u, self.v, self.w, self.p = initialize_flow(
+            self.nx, self.ny, self.nz, z_c, self.Ly, self.Lz, U_bulk=self.U_bulk,
+            init_type=ini.get("type", "parabolic"),
+            perturbation_intensity=ini.get("perturbation_intensity", 0.0),
+            n_vortices=ini.get("n_vortices", 4), device=self.device,
+            top_wall_bc_type=self.top_wall_bc_type, seed=ini.get("seed"), dtype=self.wdtype)
+
+        self.fft_data = initialize_fft_solver(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz_c,
+                                              self.dz_f, top_wall_bc_type=self.top_wall_bc_type,
+                                              dtype=self.wdtype)
+        self.p = self.fft_data["workspace_p"]
+        self._grid = L.get_grid(self.nx, self.ny, self.nz, self.dx, self.dy, self.nu,
+                                L.top_code(self.top_wall_bc_type), self.wdtype, self.device,
+                                dz_f=self.dz_f, dz_c=self.dz_c)
+        self._plan = self._grid.poisson()
+
+        st = config.get("statistics", {})
+        self.n_stats = st.get("n_stats", 0)
+        self.t_stats = st.get("t_stats", 10.0)
+        self.turbulence_stats = None
+        if self.n_stats > 0:
+            self.turbulence_stats = TurbulenceStats(
+                self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz, self.z_c, self.z_f, self.dz_c,
+                self.dz_f, self.dx, self.dy, self.nu, self.Re_tau,
+                z_plus_target=st.get("z_plus_target", 15.0), device=self.device)
+
+        # AB2 history (two rhs sets swapped each step) and the device scalar block
+        self._rhs = [[torch.zeros_like(q) for q in (self.u, self.v, self.w)] for _ in range(2)]
+        self._scal = torch.zeros(L.TC_NSCAL, dtype=torch.float64, device=self.device)
+        self._scal[L.S_DT] = self.dt
+        self._scal[L.S_UTARGET] = self.U_bulk
+        self._newest = -1
+        self.body_force = None
+        ra = (ctypes.c_void_p * 3)(*[q.data_ptr() for q in self._rhs[0]])
+        rb = (ctypes.c_void_p * 3)(*[q.data_ptr() for q in self._rhs[1]])
+        h = ctypes.c_void_p()
+        L.call("tc_solver_create", self._grid.handle, self._plan, L.ptr(self.u), L.ptr(self.v),
+               L.ptr(self.w), L.ptr(self.p), ra, rb, L.ptr(self._scal), float(self.total_volume),
+               L.TC_BRANCH_CPU if branch == "cpu" else L.TC_BRANCH_GPU, ctypes.byref(h))
+        self._solver = h
+        self._graph_key = None
+
+        # rescale to U_bulk (solver.py:268-276) and the initial projection (solver.py:342-369)
+        L.call("tc_solver_project_initial", self._solver, self._stream())
+        self.time = 0.0
+        self.current_step = 0
+        self.initial_step = 0
+
+    # -- plumbing --------------------------------------------------------------------------
+    def _stream(self):
+        return L.stream_of(self.u)
+
+    def __del__(self):
+        try:
+            if getattr(self, "_solver", None) is not None:
+                L.load().tc_solver_destroy(self._solver)
+        except Exception:
+            pass
+
+    @property
+    def forcing(self):
+        return self._scal[L.S_F]
+
+    @property
+    def rhs_u_prev(self):
+        return self._rhs[self._newest][0] if self._newest >= 0 else None
+
+    @property
+    def rhs_v_prev(self):
+        return self._rhs[self._newest][1] if self._newest >= 0 else None
+
+    @property
+    def rhs_w_prev(self):
+        return self._rhs[self._newest][2] if self._newest >= 0 else None
+
+    @property
+    def rhs_u_curr(self):
+        return self._rhs[1 - self._newest][0] if self._newest >= 0 else None
+
+    @property
+    def rhs_v_curr(self):
+        return self._rhs[1 - self._newest][1] if self._newest >= 0 else None
+
+    @property
+    def rhs_w_curr(self):
+        return self._rhs[1 - self._newest][2] if self._newest >= 0 else None
+
+    def device_dt(self):
+        return float(self._scal[L.S_DT].item())
+
+    # -- the step ------------------------------------------------------------------------------
+    def apply_bc_uvw(self):
+        apply_bc_all(self.u, self.v, self.w, self.top_wall_bc_type)
+
+    def step_imex(self, dt=None):
+        """One IMEX step (solver.py:580-710).  dt (host float) sets the device dt first; None keeps
+        the device value.  Returns (u_bulk, forcing) as 0-D float64 device tensors."""
+        newest = ctypes.c_int(-1)
+        first = 1 if self._newest < 0 else 0
+        body = L.ptr(self.body_force) if self.body_force is not None else None
+        L.call("tc_solver_step", self._solver, float(dt) if dt is not None else -1.0, first, body,
+               ctypes.byref(newest), self._stream())
+        self._newest = newest.value
+        if dt is not None:
+            self.dt = float(dt)
+        return self._scal[L.S_UBULK].clone(), self._scal[L.S_F].clone()
+
+    def compute_cfl_dt(self):
+        """solver.py:465-496 (host value; one device sync)."""
+        from .operators import compute_cfl_fused
+        dti = compute_cfl_fused(self.u, self.v, self.w, self.nx, self.ny, self.nz, self.dx, self.dy,
+                                self.dz_f, self.dz_c)
+        if dti < 1e-10:
+            dti = 1.0
+        return min(max(self.cfl_target * (1.0 / dti), self.dt_min), self.dt_max)
+
+    def update_dt_device(self):
+        """The adaptive-dt rule (solver.py:808-812) evaluated on the device, no host sync."""
+        L.call("tc_solver_update_dt", self._solver, float(self.cfl_target), float(self.dt_min),
+               float(self.dt_max), self._stream())
+
+    def run_graph(self, n_chunks, chunk_steps, dt_update_interval=0):
+        """Replay `n_chunks` HIP-graph chunks of `chunk_steps` IMEX steps each (even), with the
+        device-side adaptive dt before every dt_update_interval-th step of a chunk.  The AB2
+        history must exist (take at least one eager step first)."""
+        if self._newest < 0:
+            raise RuntimeError("run_graph needs one eager step first (forward-Euler start)")
+        key = (chunk_steps, dt_update_interval, self.cfl_target, self.dt_min, self.dt_max)
+        if self._graph_key != key:
+            L.call("tc_solver_capture", self._solver, int(chunk_steps), int(dt_update_interval),
+                   float(self.cfl_target), float(self.dt_min), float(self.dt_max), self._stream())
+            self._graph_key = key
+        for _ in range(n_chunks):
+            L.call("tc_solver_replay", self._solver, self._stream())
+        self.current_step += n_chunks * chunk_steps
+
+    def run_simulation(self):
+        """Stepping loop of solver.py:738-859: adaptive dt every dt_update_interval steps (on the
+        device), diagnostics every n_out steps, statistics every n_stats steps."""
+        if self.time_scheme != "IMEX":
+            raise ValueError(f"Unknown time stepping scheme: {self.time_scheme}. Use 'IMEX'.")
+        step = self.current_step
+        self._scal[L.S_TIME] = self.time
+        t0 = _time.time()
+        while step < self.n_steps and self.time < self.t_max:
+            step += 1
+            self.current_step = step
+            if self.dt_update_interval > 0 and step % self.dt_update_interval == 0:
+                self.update_dt_device()
+            ub, f = self.step_imex(None)
+            if step % self.n_out == 0 or (self.n_stats > 0 and step % self.n_stats == 0):
+                self.time = float(self._scal[L.S_TIME].item())
+                self.dt = float(self._scal[L.S_DT].item())
+            if step % self.n_out == 0:
+                div = compute_divergence(self.u, self.v, self.w, self.nx, self.ny, self.nz, self.dx, self.dy, self.dz_f)
+                ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
+                if self.verbose:
+                    print(f"{step:6d} {self.time:10.6f} {self.dt:10.6f} {float(div.abs().max()):12.3e} "
+                          f"{float(ub):10.6f} {float(ut):10.6f} {float(f):12.3e}", flush=True)
+            if (self.turbulence_stats is not None and self.n_stats > 0 and self.time >= self.t_stats
+                    and step % self.n_stats == 0):
+                self.turbulence_stats.accumulate_statistics(self.u, self.v, self.w)
+        self.time = float(self._scal[L.S_TIME].item())
+        self.dt = float(self._scal[L.S_DT].item())
+        if self.verbose:
+            print(f"Simulation complete: {step} steps, total time = {self.time:.6f}, "
+                  f"wall {(_time.time() - t0):.2f}s", flush=True)
+        return step

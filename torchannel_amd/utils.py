@@ -1,0 +1,67 @@
+"""Grid generation and the scalar reductions of utils.py, on the HIP library.
+
+generate_grid is one-off setup and uses the reference's own tensor formulas on the target
+device; the per-step functions (divergence, bulk velocity, wall shear) run HIP kernels.
+"""
+import torch
+
+from . import _lib as L
+
+
+def generate_grid(gamma, nz, Lz, device="cpu", stretching_type="symmetric", dtype=torch.float64):
+    """tanh-stretched wall-normal grid (utils.py:5-45): z_f[nz+1], z_c[nz+2], dz_f[nz], dz_c[nz+1]."""
+    k = torch.linspace(0, nz, nz + 1, device=device, dtype=dtype)
+    g = torch.tensor(gamma, device=device, dtype=dtype)
+    if stretching_type == "bottom":
+        xi = k / nz
+        z_f = Lz * (1.0 - torch.tanh(gamma * (1.0 - xi)) / torch.tanh(g))
+    else:
+        xi = (2 * k / nz) - 1
+        z_f = 0.5 * Lz * (1 + torch.tanh(gamma * xi) / torch.tanh(g))
+    mid = 0.5 * (z_f[:-1] + z_f[1:])
+    z_c = torch.cat([torch.tensor([-mid[0]], device=device, dtype=dtype), mid,
+                     torch.tensor([2 * z_f[-1] - mid[-1]], device=device, dtype=dtype)])
+    return z_f, z_c, z_f[1:] - z_f[:-1], z_c[1:] - z_c[:-1]
+
+
+def _dims_u(u):
+    return u.shape[0] - 1, u.shape[1] - 2, u.shape[2] - 2
+
+
+def compute_divergence(u, v, w, nx, ny, nz, dx, dy, dz_f):
+    """utils.py:226-240 -> new (nx, ny, nz) tensor."""
+    dt = L.require_device(u, v, w)
+    g = L.get_grid(nx, ny, nz, dx, dy, 0.0, L.TC_TOP_DIRICHLET, dt, u.device, dz_f=dz_f)
+    div = torch.empty((nx, ny, nz), dtype=dt, device=u.device)
+    L.call("tc_divergence", g.handle, L.ptr(u), L.ptr(v), L.ptr(w), L.ptr(div), L.stream_of(u))
+    return div
+
+
+def compute_bulk_velocity(u, cell_vol_ratio, total_volume):
+    """utils.py:207-223 -> 0-D float64 tensor on the device."""
+    dt = L.require_device(u)
+    nx, ny, nz = cell_vol_ratio.shape
+    wgt = cell_vol_ratio
+    if wgt.dtype != torch.float64 or wgt.device != u.device:
+        wgt = wgt.to(device=u.device, dtype=torch.float64)
+    g = L.get_grid(nx, ny, nz, 1.0, 1.0, 0.0, L.TC_TOP_DIRICHLET, dt, u.device)
+    out = torch.empty((), dtype=torch.float64, device=u.device)
+    s = wgt.stride()
+    L.call("tc_bulk_velocity", g.handle, L.ptr(u), ctypes_ptr(wgt), s[0], s[1], s[2],
+           float(total_volume), L.ptr(out), L.stream_of(u))
+    return out if dt == torch.float64 else out.to(dt)
+
+
+def ctypes_ptr(t):
+    import ctypes
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def compute_u_tau(u, z_c, nu, top_wall_bc_type="dirichlet"):
+    """utils.py:176-204 (means over the whole k=1 and k=nz planes) -> 0-D tensor."""
+    dt = L.require_device(u)
+    nx, ny, nz = _dims_u(u)
+    g = L.get_grid(nx, ny, nz, 1.0, 1.0, float(nu), L.top_code(top_wall_bc_type), dt, u.device)
+    out = torch.empty((), dtype=torch.float64, device=u.device)
+    L.call("tc_u_tau", g.handle, L.ptr(u), float(z_c[1]), L.ptr(out), L.stream_of(u))
+    return out if dt == torch.float64 else out.to(dt)
