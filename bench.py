@@ -156,6 +156,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (5 times), the reference "
+                         "kernel for scaling HBM byte counters")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -165,6 +168,16 @@ def main():
     steps = max(10, (args.steps // 10) * 10)
     warmup = max(10, (args.warmup // 10) * 10)
     print(f"[bench] {args.precision} {NX}x{NY}x{NZ} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
+    if args.copy_calib_mb > 0:
+        import torch
+        n = args.copy_calib_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: 5 x {args.copy_calib_mb} MiB read + write", file=sys.stderr, flush=True)
+        del src, dst
     main_run = run_gpu(args.precision, steps, warmup)
     print(f"[bench] {args.precision}: {main_run['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
     other = None

@@ -83,6 +83,12 @@ int tc_cn_solve(const tc_grid* g, int comp, double dt, const double* dt_dev, dou
 int tc_pcr_batched(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
                    const void* c, int64_t abc_stride, const void* d, void* x, void* stream);
 
+/* solve_tridiagonal (projection_fft.py:167-215): the same systems as tc_pcr_batched solved by the
+ * reference's Thomas sweep, one thread per system; work holds batch*n reals (c'). */
+int tc_thomas_batched(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
+                      const void* c, int64_t abc_stride, const void* d, void* x, void* work,
+                      void* stream);
+
 /* compute_divergence (utils.py:226-240). */
 int tc_divergence(const tc_grid* g, const void* u, const void* v, const void* w, void* div,
                   void* stream);
@@ -110,7 +116,8 @@ int tc_u_tau(const tc_grid* g, const void* u, double z_c1, double* out_dev, void
  * tridiagonal rows and the spectral workspace.  tc_poisson_solve is solve_poisson_fft
  * (projection_fft.py:101-165): p (ghosts included) from rhs (nx, ny, nz).  The singular
  * (kx,ky)=(0,0) mode is gauge-fixed in float64 (its first unknown pinned to zero); p is defined
- * up to an additive constant, as in the reference. */
+ * up to an additive constant, as in the reference.  The wall-normal systems are solved by the
+ * reference's Thomas sweep with the factorisation precomputed per plan. */
 int tc_poisson_create(const tc_grid* g, tc_poisson** out);
 int tc_poisson_destroy(tc_poisson* pl);
 int tc_poisson_solve(tc_poisson* pl, const void* rhs, void* p, void* stream);
@@ -128,7 +135,7 @@ int tc_stats_accumulate(tc_poisson* pl, const void* u, const void* v, const void
  *   rhs_a/rhs_b  two sets of three rhs arrays shaped like u, v, w (AB2 history, swapped each
  *                step exactly like rhs_*_prev / rhs_*_curr, solver.py:629-651);
  *   scal         float64[TC_NSCAL] device scalar block: dt, forcing F, u_bulk, time, cfl rate,
- *                U_bulk target, step count (see TC_S_* in the library sources).
+ *                U_bulk target, step count, non-finite flag (see TC_S_* in the library sources).
  * body (optional, shaped like u) is an additive body force next to F in rhs_u.
  * Volume V = Lx*Ly*Lz (solver.py:234).  branch selects the explicit-RHS semantics of the
  * reference's device branches (TC_BRANCH_GPU: solver.py:593-600, TC_BRANCH_CPU: 601-622). */
@@ -140,9 +147,10 @@ int tc_solver_destroy(tc_solver* s);
  * the forward-Euler start (no rhs history).  Returns which rhs set now holds the newest rhs
  * (0 = rhs_a, 1 = rhs_b) through *newest (may be null). */
 int tc_solver_step(tc_solver* s, double dt, int first, const void* body, int* newest, void* stream);
-/* The initial projection of ChannelFlow.__init__ (solver.py:342-369): div -> Poisson with
- * dt = 1 -> project -> boundary conditions. */
-int tc_solver_project_initial(tc_solver* s, void* stream);
+/* ChannelFlow.__init__'s start (solver.py:268-276, 342-369): when rescale != 0, u is scaled to
+ * the target bulk velocity (skipped on restart, solver.py:271); then div -> Poisson with dt = 1
+ * -> project -> boundary conditions. */
+int tc_solver_project_initial(tc_solver* s, int rescale, void* stream);
 /* Adaptive dt on the device (solver.py:465-496 and the 5% rule of 807-812): reads the state,
  * writes the new dt into scal when it changed by more than 5%. */
 int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream);

@@ -20,7 +20,7 @@ TC_BRANCH_GPU, TC_BRANCH_CPU = 0, 1
 TC_F64, TC_F32 = 0, 1
 TC_NSCAL = 8
 # scalar-block slots (tc_internal.h)
-S_DT, S_F, S_UBULK, S_TIME, S_CFL, S_UTARGET, S_STEP = 0, 1, 2, 3, 4, 5, 6
+S_DT, S_F, S_UBULK, S_TIME, S_CFL, S_UTARGET, S_STEP, S_NONFINITE = 0, 1, 2, 3, 4, 5, 6, 7
 
 _vp, _i, _d, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -34,6 +34,7 @@ SIGNATURES = {
     "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tc_cn_solve": [_vp, _i, _d, _vp, _d, _vp, _vp, _vp],
     "tc_pcr_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
+    "tc_thomas_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
     "tc_divergence": [_vp, _vp, _vp, _vp, _vp, _vp],
     "tc_project": [_vp, _vp, _vp, _vp, _vp, _d, _vp, _vp],
     "tc_bulk_velocity": [_vp, _vp, _vp, _i64, _i64, _i64, _d, _vp, _vp],
@@ -47,7 +48,7 @@ SIGNATURES = {
                          _i, ctypes.POINTER(_vp)],
     "tc_solver_destroy": [_vp],
     "tc_solver_step": [_vp, _d, _i, _vp, ctypes.POINTER(_i), _vp],
-    "tc_solver_project_initial": [_vp, _vp],
+    "tc_solver_project_initial": [_vp, _i, _vp],
     "tc_solver_update_dt": [_vp, _d, _d, _d, _vp],
     "tc_solver_capture": [_vp, _i, _i, _d, _d, _d, _vp],
     "tc_solver_replay": [_vp, _vp],

@@ -180,6 +180,15 @@ int tc_pcr_batched(int n, int batch, int dtype, int is_complex, const void* a, c
   return pcr_batched(n, batch, dtype, is_complex, a, b, c, (long long)abc_stride, d, x, (hipStream_t)stream);
 }
 
+int tc_thomas_batched(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
+                      const void* c, int64_t abc_stride, const void* d, void* x, void* work, void* stream) {
+  if (n < 1 || batch < 0 || !a || !b || !c || !d || !x || !work) return TC_ERR_ARG;
+  if (dtype != TC_F64 && dtype != TC_F32) return TC_ERR_DTYPE;
+  if (batch == 0) return TC_OK;
+  return thomas_batched(n, batch, dtype, is_complex, a, b, c, (long long)abc_stride, d, x, work,
+                        (hipStream_t)stream);
+}
+
 int tc_divergence(const tc_grid* h, const void* u, const void* v, const void* w, void* div, void* stream) {
   if (!h || !u || !v || !w || !div) return TC_ERR_ARG;
   return launch_divergence(&h->g, u, v, w, div, (hipStream_t)stream);
@@ -354,7 +363,7 @@ __global__ void k_rescale_u(double* __restrict__ u64, float* __restrict__ u32, s
   }
 }
 
-int tc_solver_project_initial(tc_solver* s, void* stream) {
+int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
   if (!s) return TC_ERR_ARG;
   const Grid* g = s->g;
   const Dims& d = g->d;
@@ -364,12 +373,14 @@ int tc_solver_project_initial(tc_solver* s, void* stream) {
   const size_t nw_ = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
   const size_t esz = g->dtype == TC_F64 ? 8 : 4;
   // u_b of the initial field into scal[TC_S_UBULK] (cell volume dx*dy*dz_f expanded, solver.py:231)
-  TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
-                    g->counter, kMaxRedBlocks, st));
-  hipLaunchKernelGGL(k_rescale_u, dim3(1024), dim3(256), 0, st,
-                     g->dtype == TC_F64 ? (double*)s->u : nullptr, g->dtype == TC_F64 ? nullptr : (float*)s->u,
-                     nu_, s->scal);
-  TC_CHECK_LAUNCH();
+  if (rescale) {
+    TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
+                      g->counter, kMaxRedBlocks, st));
+    hipLaunchKernelGGL(k_rescale_u, dim3(1024), dim3(256), 0, st,
+                       g->dtype == TC_F64 ? (double*)s->u : nullptr, g->dtype == TC_F64 ? nullptr : (float*)s->u,
+                       nu_, s->scal);
+    TC_CHECK_LAUNCH();
+  }
   TC_HIP(hipMemcpyAsync(s->su, s->u, esz * nu_, hipMemcpyDeviceToDevice, st));
   TC_HIP(hipMemcpyAsync(s->sv, s->v, esz * nv_, hipMemcpyDeviceToDevice, st));
   TC_HIP(hipMemcpyAsync(s->sw, s->w, esz * nw_, hipMemcpyDeviceToDevice, st));

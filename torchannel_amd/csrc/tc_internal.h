@@ -11,7 +11,7 @@
 #define TC_S_CFL 4       // last convective rate (1/dt) from the CFL reduction
 #define TC_S_UTARGET 5   // target bulk velocity U_bulk
 #define TC_S_STEP 6      // steps taken (as a double)
-#define TC_S_SPARE 7
+#define TC_S_NONFINITE 7   // set to 1 by the projection kernel when a non-finite velocity appears
 
 // k_explicit_cn modes
 #define TC_K1_RHS 1

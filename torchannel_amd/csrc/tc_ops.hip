@@ -22,28 +22,52 @@ __device__ __forceinline__ double block_reduce(double v, bool is_max, double* sh
   return r;
 }
 
-// Thread 0 publishes `val` into partials[slot]; returns true (in every thread) in the block that
-// finishes last, after which all partials are visible to it.
-__device__ __forceinline__ bool publish_last(double* partials, int slot, double val, unsigned* counter,
-                                             int nb) {
-  __shared__ bool last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[slot], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    last = atomicAdd(counter, 1u) == (unsigned)(nb - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ __forceinline__ double fold_partials(const double* partials, int nb, bool is_max, double* sh) {
+// Folds nb per-block partials (sum or max) in a fixed order inside one block.
+__device__ __forceinline__ double fold_partials(const double* __restrict__ partials, int nb, int stride,
+                                                int off, bool is_max, double* sh) {
   double s = is_max ? -1.0e308 : 0.0;
   for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-    const double x = __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double x = partials[(size_t)t * stride + off];
     s = is_max ? fmax(s, x) : s + x;
   }
   return block_reduce(s, is_max, sh);
+}
+
+#define TC_FIN_BULK 0
+#define TC_FIN_CFL 1
+#define TC_FIN_UTAU 2
+
+// Second pass of the stand-alone reductions: one block folds the per-block partials written by
+// k_bulk / k_cfl / k_u_tau and writes the result (and, for the CFL rate with update_dt, the
+// adaptive-dt rule of solver.py:483-494 and 811-812).
+__global__ __launch_bounds__(256) void k_finalize(int kind, const double* __restrict__ partials, int nb,
+                                                  double* __restrict__ out, double* __restrict__ scal,
+                                                  double a0, double a1, double a2, int flag) {
+  __shared__ double sh[4];
+  if (kind == TC_FIN_BULK) {
+    const double tot = fold_partials(partials, nb, 1, 0, false, sh);
+    if (threadIdx.x == 0) *out = tot * a0;
+  } else if (kind == TC_FIN_CFL) {
+    const double rate = fold_partials(partials, nb, 1, 0, true, sh);
+    if (threadIdx.x == 0) {
+      *out = rate;
+      if (flag) {
+        const double dti = rate < 1e-10 ? 1.0 : rate;
+        double dt_new = a0 * (1.0 / dti);
+        dt_new = fmin(fmax(dt_new, a1), a2);
+        const double dt = scal[TC_S_DT];
+        if (fabs(dt_new - dt) / dt > 0.05) scal[TC_S_DT] = dt_new;
+      }
+    }
+  } else {
+    const double tb = fold_partials(partials, nb, 2, 0, false, sh);
+    const double tt = fold_partials(partials, nb, 2, 1, false, sh);
+    if (threadIdx.x == 0) {
+      // a0 = nu, a1 = dist, a2 = plane size; flag = free-slip top (bottom wall only)
+      const double ub = sqrt(fabs(a0 * (tb / a2) / a1));
+      *out = flag ? ub : 0.5 * (ub + sqrt(fabs(a0 * (tt / a2) / a1)));
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -122,8 +146,7 @@ __global__ __launch_bounds__(256) void k_project(Dims d, Metric<T> mt, T* __rest
 template <typename T>
 __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, const double* __restrict__ wgt,
                                               long long ws0, long long ws1, long long ws2, double inv_vol,
-                                              double* __restrict__ out, double* __restrict__ partials,
-                                              unsigned* __restrict__ counter) {
+                                              double* __restrict__ partials) {
   __shared__ double sh[4];
   const long long N = (long long)d.nx * d.ny * d.nz;
   double s = 0.0;
@@ -134,12 +157,7 @@ __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, c
     s += (double)u[(i + 1) * d.sxu + (j + 1) * d.syu + k + 1] * wgt[i * ws0 + j * ws1 + k * ws2];
   }
   const double bs = block_reduce(s, false, sh);
-  if (!publish_last(partials, blockIdx.x, bs, counter, gridDim.x)) return;
-  const double tot = fold_partials(partials, gridDim.x, false, sh);
-  if (threadIdx.x == 0) {
-    *out = tot * inv_vol;
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = bs;
 }
 
 // compute_cfl_fused (operators.py:1262-1331): per-cell convective rates in x, y, z; maximum.
@@ -148,9 +166,7 @@ __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, c
 template <typename T>
 __global__ __launch_bounds__(256) void k_cfl(Dims d, Metric<T> mt, const T* __restrict__ u,
                                              const T* __restrict__ v, const T* __restrict__ w,
-                                             double* __restrict__ scal, double* __restrict__ partials,
-                                             unsigned* __restrict__ counter, int update_dt, double cfl,
-                                             double dt_min, double dt_max) {
+                                             double* __restrict__ partials) {
   __shared__ double sh[4];
   const long long N = (long long)d.nx * d.ny * d.nz;
   double best = 0.0;
@@ -177,26 +193,12 @@ __global__ __launch_bounds__(256) void k_cfl(Dims d, Metric<T> mt, const T* __re
     best = fmax(best, m);
   }
   const double bm = block_reduce(best, true, sh);
-  if (!publish_last(partials, blockIdx.x, bm, counter, gridDim.x)) return;
-  const double rate = fold_partials(partials, gridDim.x, true, sh);
-  if (threadIdx.x == 0) {
-    scal[TC_S_CFL] = rate;
-    if (update_dt) {
-      const double dti = rate < 1e-10 ? 1.0 : rate;
-      double dt_new = cfl * (1.0 / dti);
-      dt_new = fmin(fmax(dt_new, dt_min), dt_max);
-      const double dt = scal[TC_S_DT];
-      if (fabs(dt_new - dt) / dt > 0.05) scal[TC_S_DT] = dt_new;
-    }
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = bm;
 }
 
 // compute_u_tau (utils.py:187-204): means over the whole k=1 and k=nz planes of u (ghosts too).
 template <typename T>
-__global__ __launch_bounds__(256) void k_u_tau(Dims d, const T* __restrict__ u, double nu, double dist,
-                                               double* __restrict__ out, double* __restrict__ partials,
-                                               unsigned* __restrict__ counter) {
+__global__ __launch_bounds__(256) void k_u_tau(Dims d, const T* __restrict__ u, double* __restrict__ partials) {
   __shared__ double sh[4];
   const int P = (d.nx + 1) * (d.ny + 2);
   double sb = 0.0, st = 0.0;
@@ -206,26 +208,9 @@ __global__ __launch_bounds__(256) void k_u_tau(Dims d, const T* __restrict__ u, 
   }
   const double b1 = block_reduce(sb, false, sh);
   const double b2 = block_reduce(st, false, sh);
-  __shared__ bool last;
   if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[2 * blockIdx.x], b1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&partials[2 * blockIdx.x + 1], b2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (threadIdx.x == 0) {
-    double tb = 0.0, tt = 0.0;
-    for (unsigned q = 0; q < gridDim.x; ++q) {
-      tb += __hip_atomic_load(&partials[2 * q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      tt += __hip_atomic_load(&partials[2 * q + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    const double ub = sqrt(fabs(nu * (tb / P) / dist));
-    if (d.top > 0) *out = ub;
-    else *out = 0.5 * (ub + sqrt(fabs(nu * (tt / P) / dist)));
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    partials[2 * blockIdx.x] = b1;
+    partials[2 * blockIdx.x + 1] = b2;
   }
 }
 
@@ -239,8 +224,7 @@ template <typename T, bool PASS_B>
 __global__ __launch_bounds__(256) void k_stats(Dims d, const T* __restrict__ u, const T* __restrict__ v,
                                                const T* __restrict__ w, double* __restrict__ means,
                                                double* __restrict__ prof, double* __restrict__ planes,
-                                               int kb, int kt, double* __restrict__ partials,
-                                               unsigned* __restrict__ counter) {
+                                               int kb, int kt, double* __restrict__ partials) {
   const int k = threadIdx.x;
   const int nz = d.nz, nx = d.nx, ny = d.ny;
   constexpr int NQ = PASS_B ? 4 : 3;
@@ -273,32 +257,29 @@ __global__ __launch_bounds__(256) void k_stats(Dims d, const T* __restrict__ u, 
 #pragma unroll
     for (int q = 0; q < NQ; ++q) partials[((size_t)blockIdx.x * NQ + q) * nz + k] = acc[q];
   }
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (k < nz) {
-    const double inv = 1.0 / ((double)nx * (double)ny);
-    double tot[NQ];
+}
+
+// Folds the per-block z rows of k_stats in block order (one thread per z row).
+template <bool PASS_B>
+__global__ __launch_bounds__(256) void k_stats_fold(int nx, int ny, int nz, int nb, const double* __restrict__ partials,
+                                                    double* __restrict__ means, double* __restrict__ prof) {
+  constexpr int NQ = PASS_B ? 4 : 3;
+  const int k = threadIdx.x;
+  if (k >= nz) return;
+  const double inv = 1.0 / ((double)nx * (double)ny);
+  double tot[NQ];
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
-    for (unsigned b = 0; b < gridDim.x; ++b)
+  for (int q = 0; q < NQ; ++q) tot[q] = 0.0;
+  for (int b = 0; b < nb; ++b)
 #pragma unroll
-      for (int q = 0; q < NQ; ++q)
-        tot[q] += __hip_atomic_load(&partials[((size_t)b * NQ + q) * nz + k], __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    if constexpr (!PASS_B) {
-      means[k] = tot[0] * inv; means[nz + k] = tot[1] * inv; means[2 * nz + k] = tot[2] * inv;
-    } else {
-      prof[k] += means[k];
+    for (int q = 0; q < NQ; ++q) tot[q] += partials[((size_t)b * NQ + q) * nz + k];
+  if constexpr (!PASS_B) {
+    means[k] = tot[0] * inv; means[nz + k] = tot[1] * inv; means[2 * nz + k] = tot[2] * inv;
+  } else {
+    prof[k] += means[k];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) prof[(q + 1) * nz + k] += tot[q] * inv;
-    }
+    for (int q = 0; q < 4; ++q) prof[(q + 1) * nz + k] += tot[q] * inv;
   }
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Folded, wall-averaged spectra (turbstats.py:176-225) from the plane spectra (nkx, ny, 6): our
@@ -383,12 +364,13 @@ int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, doub
   const int blocks = std::min(cfl_blocks(gr), nblocks_max);
   if (gr->dtype == TC_F64)
     hipLaunchKernelGGL(k_cfl<double>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<double>(),
-                       (const double*)u, (const double*)v, (const double*)w, scal, partials, counter,
-                       update_dt, cfl_target, dt_min, dt_max);
+                       (const double*)u, (const double*)v, (const double*)w, partials);
   else
     hipLaunchKernelGGL(k_cfl<float>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<float>(),
-                       (const float*)u, (const float*)v, (const float*)w, scal, partials, counter,
-                       update_dt, cfl_target, dt_min, dt_max);
+                       (const float*)u, (const float*)v, (const float*)w, partials);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_CFL, (const double*)partials, blocks,
+                     scal + TC_S_CFL, scal, cfl_target, dt_min, dt_max, update_dt);
   TC_CHECK_LAUNCH();
   return TC_OK;
 }
@@ -400,10 +382,13 @@ int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0,
   const int blocks = (int)std::min<long long>(nblocks_max, (N + 255) / 256);
   if (gr->dtype == TC_F64)
     hipLaunchKernelGGL(k_bulk<double>, dim3(blocks), dim3(256), 0, st, gr->d, (const double*)u, wgt, ws0,
-                       ws1, ws2, inv_vol, out, partials, counter);
+                       ws1, ws2, inv_vol, partials);
   else
     hipLaunchKernelGGL(k_bulk<float>, dim3(blocks), dim3(256), 0, st, gr->d, (const float*)u, wgt, ws0, ws1,
-                       ws2, inv_vol, out, partials, counter);
+                       ws2, inv_vol, partials);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_BULK, (const double*)partials, blocks, out,
+                     (double*)nullptr, inv_vol, 0.0, 0.0, 0);
   TC_CHECK_LAUNCH();
   return TC_OK;
 }
@@ -413,11 +398,12 @@ int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double
   const int P = (gr->d.nx + 1) * (gr->d.ny + 2);
   const int blocks = std::min(nblocks_max / 2, (P + 255) / 256);
   if (gr->dtype == TC_F64)
-    hipLaunchKernelGGL(k_u_tau<double>, dim3(blocks), dim3(256), 0, st, gr->d, (const double*)u, gr->nu, dist,
-                       out, partials, counter);
+    hipLaunchKernelGGL(k_u_tau<double>, dim3(blocks), dim3(256), 0, st, gr->d, (const double*)u, partials);
   else
-    hipLaunchKernelGGL(k_u_tau<float>, dim3(blocks), dim3(256), 0, st, gr->d, (const float*)u, gr->nu, dist,
-                       out, partials, counter);
+    hipLaunchKernelGGL(k_u_tau<float>, dim3(blocks), dim3(256), 0, st, gr->d, (const float*)u, partials);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_UTAU, (const double*)partials, blocks, out,
+                     (double*)nullptr, gr->nu, dist, (double)P, gr->d.top > 0 ? 1 : 0);
   TC_CHECK_LAUNCH();
   return TC_OK;
 }
@@ -429,10 +415,16 @@ static int stats_t(const Poisson* pl, const void* u, const void* v, const void* 
   const Grid* gr = pl->g;
   const Dims& d = gr->d;
   hipLaunchKernelGGL((k_stats<T, false>), dim3(nblocks), dim3(256), 0, st, d, (const T*)u, (const T*)v,
-                     (const T*)w, means, prof, pl->planes, kb, kt, partials, counter);
+                     (const T*)w, means, prof, pl->planes, kb, kt, partials);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_stats_fold<false>, dim3(1), dim3(256), 0, st, d.nx, d.ny, d.nz, nblocks,
+                     (const double*)partials, means, prof);
   TC_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_stats<T, true>), dim3(nblocks), dim3(256), 0, st, d, (const T*)u, (const T*)v,
-                     (const T*)w, means, prof, pl->planes, kb, kt, partials, counter);
+                     (const T*)w, means, prof, pl->planes, kb, kt, partials);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_stats_fold<true>, dim3(1), dim3(256), 0, st, d.nx, d.ny, d.nz, nblocks,
+                     (const double*)partials, means, prof);
   TC_CHECK_LAUNCH();
   int rc = plane_fft(pl, st);
   if (rc != TC_OK) return rc;

@@ -21,6 +21,11 @@ struct Poisson {
   void* spec = nullptr;        // S[nkx][ny][nz] complex, working precision
   int pairs = 1;               // z-row pairs per x-transform block
   int kzy = 1;                 // z-rows per y-transform block
+  void* th_cp = nullptr;       // Thomas c'_k per system, [k][system], working precision
+  void* th_iden = nullptr;     // Thomas 1/pivot_k per system, [k][system]
+  double* th_cp0 = nullptr;    // float64 rows of the gauge-fixed (0,0) system
+  double* th_iden0 = nullptr;
+  int sb = 32;                 // systems per Thomas block (LDS-staged)
   double* planes = nullptr;    // statistics: (nx, ny, 6) float64
   void* plane_spec = nullptr;  // statistics: (nkx, ny, 6) complex float64
 };
@@ -32,6 +37,8 @@ int poisson_solve(const Poisson* pl, bool from_vel, const void* u, const void* v
 int poisson_phase(const Poisson* pl, int ph, bool from_vel, const void* u, const void* v, const void* w,
                   const void* rin, const double* scal, void* p, hipStream_t st);
 int plane_fft(const Poisson* pl, hipStream_t st);
+int thomas_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
+                   long long s, const void* d, void* x, void* work, hipStream_t st);
 int pcr_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
                 long long s, const void* d, void* x, hipStream_t st);
 

@@ -205,54 +205,83 @@ __device__ __forceinline__ void pcr_general(U (&a)[R], U (&b)[R], U (&c)[R], U (
   }
 }
 
-// P3 body for one (kx, ky) system in compute type U (float64 for the gauge mode).
-// Rows of projection_fft.py:44-82: a[k] = cl[k] (k >= 1), c[k] = cr[k] (k <= nz-2),
-// b[k] = -((cl[k] + cr[k]) + K2) (+ cl[0] at k = 0, + cr[nz-1] at k = nz-1).
-template <typename T, typename U, int R>
-__device__ __forceinline__ void poisson_system(int nz, const double* __restrict__ cl,
-                                               const double* __restrict__ cr, double k2,
-                                               cplx<T>* __restrict__ col, bool gauge, int lane) {
-  U a[R], b[R], c[R], dr[R], di[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    if (m < nz) {
-      const double l = cl[m], rr = cr[m];
-      double bb = -((l + rr) + k2);
-      if (m == 0) bb += cl[0];
-      if (m == nz - 1) bb += cr[nz - 1];
-      a[r] = m >= 1 ? (U)l : U(0);
-      c[r] = m <= nz - 2 ? (U)rr : U(0);
-      b[r] = (U)bb;
-      const cplx<T> z = col[m];
-      dr[r] = (U)z.re;
-      di[r] = (U)z.im;
-      if (gauge && m == 0) { a[r] = U(0); b[r] = U(1); c[r] = U(0); dr[r] = U(0); di[r] = U(0); }
-      if (gauge && m == 1) a[r] = U(0);
+// P3: one wall-normal Thomas solve per (kx, ky) system (projection_fft.py:184-215), the
+// reference's algorithm: PCR loses ~4 digits on the ill-conditioned low modes of this operator,
+// Thomas does not.  The factorisation depends only on the grid and (kx, ky), so c'_k and 1/pivot_k
+// are computed once per plan (extended precision on the host) and stored [k][system]; per step
+// each system is one forward and one backward sweep.  A block stages `sb` consecutive systems
+// (one contiguous chunk of S) in LDS; lane l of wave 0 sweeps system s0 + l.  The singular
+// (0,0) system is gauge-fixed (first unknown pinned to zero) and swept in float64.
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
+                                                        const T* __restrict__ cp, const T* __restrict__ iden,
+                                                        const double* __restrict__ cp0,
+                                                        const double* __restrict__ iden0,
+                                                        cplx<T>* __restrict__ S) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* col = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int s0 = xcd_remap(blockIdx.x, gridDim.x) * sb;
+  const int ns = min(sb, nsys - s0);
+  const int ld = nz + 1;
+  const cplx<T>* src = S + (size_t)s0 * nz;
+  for (int e = threadIdx.x; e < ns * nz; e += 256) {
+    const int sl = e / nz, k = e - sl * nz;
+    col[sl * ld + k] = src[e];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < ns) {
+    const int sl = threadIdx.x, s = s0 + sl;
+    cplx<T>* c = col + sl * ld;
+    if (s == 0) {
+      // gauge-fixed (0,0) mode in float64 (forward values kept in a float64 LDS row): row 0 is
+      // x0 = 0, row 1 is decoupled from it
+      double* fw = reinterpret_cast<double*>(col + sb * ld);
+      double pr = 0.0, pi = 0.0;
+      fw[0] = 0.0; fw[nz] = 0.0;
+      for (int k = 1; k < nz; ++k) {
+        const double a = (k == 1) ? 0.0 : cl[k];
+        const double id = iden0[k];
+        const cplx<T> d = c[k];
+        pr = ((double)d.re - a * pr) * id;
+        pi = ((double)d.im - a * pi) * id;
+        fw[k] = pr; fw[nz + k] = pi;
+      }
+      double xr = pr, xi = pi;
+      c[nz - 1] = {(T)xr, (T)xi};
+      for (int k = nz - 2; k >= 1; --k) {
+        const double q = cp0[k];
+        xr = fw[k] - q * xr;
+        xi = fw[nz + k] - q * xi;
+        c[k] = {(T)xr, (T)xi};
+      }
+      c[0] = {T(0), T(0)};
     } else {
-      a[r] = U(0); b[r] = U(1); c[r] = U(0); dr[r] = U(0); di[r] = U(0);
+      cplx<T> dp = c[0];
+      const T id0 = iden[s];
+      dp = {dp.re * id0, dp.im * id0};
+      c[0] = dp;
+      for (int k = 1; k < nz; ++k) {
+        const T a = (T)cl[k];
+        const T id = iden[(size_t)k * nsys + s];
+        const cplx<T> d = c[k];
+        dp = {(d.re - a * dp.re) * id, (d.im - a * dp.im) * id};
+        c[k] = dp;
+      }
+      cplx<T> x = dp;
+      for (int k = nz - 2; k >= 0; --k) {
+        const T q = cp[(size_t)k * nsys + s];
+        const cplx<T> d = c[k];
+        x = {d.re - q * x.re, d.im - q * x.im};
+        c[k] = x;
+      }
     }
   }
-  pcr_general<R, U, true>(a, b, c, dr, di, nz, lane);
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    if (m < nz) col[m] = {(T)dr[r], (T)di[r]};
+  __syncthreads();
+  cplx<T>* dst = S + (size_t)s0 * nz;
+  for (int e = threadIdx.x; e < ns * nz; e += 256) {
+    const int sl = e / nz, k = e - sl * nz;
+    dst[e] = col[sl * ld + k];
   }
-}
-
-template <typename T, int R>
-__global__ __launch_bounds__(256) void k_poisson_pcr(int nkx, int ny, int nz, const double* __restrict__ cl,
-                                                     const double* __restrict__ cr,
-                                                     const double* __restrict__ K2,
-                                                     cplx<T>* __restrict__ S) {
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int sys = b * 4 + (threadIdx.x >> 6);
-  if (sys >= nkx * ny) return;
-  const int lane = lane_id();
-  cplx<T>* col = S + (size_t)sys * nz;
-  if (sys == 0) poisson_system<T, double, R>(nz, cl, cr, K2[0], col, true, lane);
-  else poisson_system<T, T, R>(nz, cl, cr, K2[sys], col, false, lane);
 }
 
 // Generic batched PCR for the stand-alone tridiagonal entry points.
@@ -299,6 +328,61 @@ __global__ __launch_bounds__(256) void k_pcr_general(int n, int batch, const T* 
       }
     }
   }
+}
+
+// Batched Thomas sweep (projection_fft.py:184-215) for the stand-alone solve_tridiagonal: one
+// thread per system, rows of a system contiguous; c' goes to the caller's workspace.
+template <typename T, bool CPLX>
+__global__ __launch_bounds__(256) void k_thomas_general(int n, int batch, const T* __restrict__ A,
+                                                        const T* __restrict__ B, const T* __restrict__ C,
+                                                        long long abc_stride, const T* __restrict__ D,
+                                                        T* __restrict__ X, T* __restrict__ work) {
+  const int sys = blockIdx.x * 256 + threadIdx.x;
+  if (sys >= batch) return;
+  const T* a = A + abc_stride * sys;
+  const T* b = B + abc_stride * sys;
+  const T* c = C + abc_stride * sys;
+  T* cp = work + (size_t)sys * n;
+  const size_t base = (size_t)sys * n;
+  T pr, pi = T(0);
+  cp[0] = c[0] / b[0];
+  if (CPLX) { pr = D[2 * base] / b[0]; pi = D[2 * base + 1] / b[0]; X[2 * base] = pr; X[2 * base + 1] = pi; }
+  else { pr = D[base] / b[0]; X[base] = pr; }
+  for (int m = 1; m < n; ++m) {
+    const T den = b[m] - a[m] * cp[m - 1];
+    if (m < n - 1) cp[m] = c[m] / den;
+    if (CPLX) {
+      pr = (D[2 * (base + m)] - a[m] * pr) / den;
+      pi = (D[2 * (base + m) + 1] - a[m] * pi) / den;
+      X[2 * (base + m)] = pr; X[2 * (base + m) + 1] = pi;
+    } else {
+      pr = (D[base + m] - a[m] * pr) / den;
+      X[base + m] = pr;
+    }
+  }
+  for (int m = n - 2; m >= 0; --m) {
+    if (CPLX) {
+      pr = X[2 * (base + m)] - cp[m] * pr;
+      pi = X[2 * (base + m) + 1] - cp[m] * pi;
+      X[2 * (base + m)] = pr; X[2 * (base + m) + 1] = pi;
+    } else {
+      pr = X[base + m] - cp[m] * pr;
+      X[base + m] = pr;
+    }
+  }
+}
+
+int thomas_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
+                   long long s, const void* d, void* x, void* work, hipStream_t st) {
+  const int blocks = (batch + 255) / 256;
+#define TC_TH(TT, CC)                                                                            \
+  hipLaunchKernelGGL((k_thomas_general<TT, CC>), dim3(blocks), dim3(256), 0, st, n, batch,        \
+                     (const TT*)a, (const TT*)b, (const TT*)c, s, (const TT*)d, (TT*)x, (TT*)work)
+  if (dtype == TC_F64) { if (cplx_d) TC_TH(double, true); else TC_TH(double, false); }
+  else { if (cplx_d) TC_TH(float, true); else TC_TH(float, false); }
+#undef TC_TH
+  TC_CHECK_LAUNCH();
+  return TC_OK;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -372,7 +456,47 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   hipMemcpy(pl->cl, cl.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   hipMemcpy(pl->cr, cr.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   hipMemcpy(pl->K2, k2.data(), sizeof(double) * k2.size(), hipMemcpyHostToDevice);
+  // Thomas factorisation of every (kx, ky) system of projection_fft.py:56-82 (rows built in
+  // float64 exactly as the reference builds tri_a/b/c, factorised in long double), [k][system]
+  const int nsys = nkx * ny;
+  std::vector<double> cp64((size_t)nz * nsys), id64((size_t)nz * nsys);
+  std::vector<double> cp0(nz), id0(nz);
+  for (int sy = 0; sy < nsys; ++sy) {
+    const bool gauge = sy == 0;
+    const double K2 = k2[sy];
+    long double cprev = 0.0L;
+    for (int k = 0; k < nz; ++k) {
+      const double a = (k >= 1 && !(gauge && k == 1)) ? cl[k] : 0.0;
+      double b = -((cl[k] + cr[k]) + K2);
+      if (k == 0) b += cl[0];
+      if (k == nz - 1) b += cr[nz - 1];
+      double c = k <= nz - 2 ? cr[k] : 0.0;
+      if (gauge && k == 0) { b = 1.0; c = 0.0; }
+      const long double den = (long double)b - (long double)a * cprev;
+      const long double cpk = (long double)c / den;
+      cp64[(size_t)k * nsys + sy] = (double)cpk;
+      id64[(size_t)k * nsys + sy] = (double)(1.0L / den);
+      if (gauge) { cp0[k] = (double)cpk; id0[k] = (double)(1.0L / den); }
+      cprev = cpk;
+    }
+  }
   const size_t esz = g->dtype == TC_F64 ? 16 : 8;
+  const size_t rsz = esz / 2;
+  if (hipMalloc(&pl->th_cp, rsz * cp64.size()) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_iden, rsz * id64.size()) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_cp0, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_iden0, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
+  if (g->dtype == TC_F64) {
+    hipMemcpy(pl->th_cp, cp64.data(), 8 * cp64.size(), hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_iden, id64.data(), 8 * id64.size(), hipMemcpyHostToDevice);
+  } else {
+    std::vector<float> f1(cp64.begin(), cp64.end()), f2(id64.begin(), id64.end());
+    hipMemcpy(pl->th_cp, f1.data(), 4 * f1.size(), hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_iden, f2.data(), 4 * f2.size(), hipMemcpyHostToDevice);
+  }
+  hipMemcpy(pl->th_cp0, cp0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
+  hipMemcpy(pl->th_iden0, id0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
+  pl->sb = std::max(1, std::min(64, (int)(67584 / ((size_t)(nz + 1) * esz))));
   if (hipMalloc(&pl->spec, esz * (size_t)nkx * ny * nz) != hipSuccess) return TC_ERR_NOMEM;
   // statistics workspace: 6 planes (u, v, w at the two sampled heights), float64
   if (hipMalloc(&pl->planes, sizeof(double) * (size_t)nx * ny * 6) != hipSuccess) return TC_ERR_NOMEM;
@@ -383,6 +507,7 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
 void poisson_free(Poisson* pl) {
   hipFree(pl->twx); hipFree(pl->twy); hipFree(pl->cl); hipFree(pl->cr); hipFree(pl->K2);
   hipFree(pl->spec); hipFree(pl->planes); hipFree(pl->plane_spec);
+  hipFree(pl->th_cp); hipFree(pl->th_iden); hipFree(pl->th_cp0); hipFree(pl->th_iden0);
 }
 
 template <typename T>
@@ -429,19 +554,11 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 1: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, -1, st);
     case 2: {
       const int nsys = pl->nkx * d.ny;
-      const int blocks = (nsys + 3) / 4;
-      const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
-#define TC_P3(RR)                                                                                 \
-  hipLaunchKernelGGL((k_poisson_pcr<T, RR>), dim3(blocks), dim3(256), 0, st, pl->nkx, d.ny, d.nz, \
-                     pl->cl, pl->cr, pl->K2, (cplx<T>*)pl->spec)
-      switch (R) {
-        case 1: TC_P3(1); break;
-        case 2: TC_P3(2); break;
-        case 3: TC_P3(3); break;
-        case 4: TC_P3(4); break;
-        default: return TC_ERR_UNSUPPORTED;
-      }
-#undef TC_P3
+      const int blocks = (nsys + pl->sb - 1) / pl->sb;
+      const size_t lds = sizeof(cplx<T>) * (size_t)pl->sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
+      hipLaunchKernelGGL((k_poisson_thomas<T>), dim3(blocks), dim3(256), lds, st, nsys, d.nz, pl->sb, pl->cl,
+                         (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,
+                         (cplx<T>*)pl->spec);
       TC_CHECK_LAUNCH();
       return TC_OK;
     }

@@ -340,10 +340,8 @@ template <typename T, int R>
 __global__ __launch_bounds__(256) void k_project_bc(
     Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
-    double* __restrict__ scal, double* __restrict__ partials, unsigned* __restrict__ counter,
-    double inv_volume, int flags) {
+    double* __restrict__ scal, double* __restrict__ partials, int flags) {
   __shared__ double red[4];
-  __shared__ bool am_last;
   const int nb = gridDim.x;
   const int b = xcd_remap(blockIdx.x, nb);
   const int wv = threadIdx.x >> 6;
@@ -355,6 +353,7 @@ __global__ __launch_bounds__(256) void k_project_bc(
   const int i = col / Y, j = col - (col / Y) * Y;
   const T dt = (T)scal[TC_S_DT];
   double part = 0.0;
+  bool bad = false;
   bool live = i < d.nx + 2 && !(comp == 0 && i > d.nx) && !(comp == 1 && j > d.ny);
   if (live) {
     const int si = (i == 0) ? d.nx : (i == d.nx + 1 ? 1 : i);
@@ -374,6 +373,7 @@ __global__ __launch_bounds__(256) void k_project_bc(
         if (m < d.nz) {
           const int k = m + 1;
           const T val = s[src + k] - dt * ((p[pb + pstep + k] - p[pb + k]) * rd);
+          bad |= !isfinite(val);
           o[dst + k] = val;
           if (m == 0) o[dst] = -val;
           if (m == d.nz - 1) o[dst + k + 1] = d.top > 0 ? val : -val;
@@ -387,30 +387,32 @@ __global__ __launch_bounds__(256) void k_project_bc(
         const int m = r * TC_WAVE + lane;
         if (m < d.nz - 1) {
           const int k = m + 1;
-          w[dst + k] = sw[src + k] - dt * ((p[pb + k + 1] - p[pb + k]) * mt.rdzc[k]);
+          const T val = sw[src + k] - dt * ((p[pb + k + 1] - p[pb + k]) * mt.rdzc[k]);
+          bad |= !isfinite(val);
+          w[dst + k] = val;
         }
         if (m == 0) { w[dst] = T(0); w[dst + d.nz] = T(0); }
       }
     }
   }
+  // device-side non-finite flag (the reference scans u, v, w, p on the host at save points,
+  // solver.py:862-867); every writer stores the same value, so no atomic is needed
+  if (__any(bad) && lane == 0) scal[TC_S_NONFINITE] = 1.0;
   if (!(flags & TC_K5_BULK)) return;
   part = wave_sum(part);
   if (lane == 0) red[wv] = part;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    const double tot = (red[0] + red[1]) + (red[2] + red[3]);
-    __hip_atomic_store(&partials[b], tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    am_last = prev == (unsigned)(nb - 1);
-  }
-  __syncthreads();
-  if (!am_last) return;
-  __threadfence();
+  if (threadIdx.x == 0) partials[b] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Folds the per-block bulk partials of k_project_bc in a fixed order (one block), then the
+// forcing controller F += 0.1*(U_b - u_b)/dt (solver.py:701-708), time and step count.
+__global__ __launch_bounds__(256) void k_step_finalize(const double* __restrict__ partials, int nb,
+                                                       double* __restrict__ scal, double inv_volume,
+                                                       int flags) {
   __shared__ double acc[256];
   double s = 0.0;
-  for (int t = threadIdx.x; t < nb; t += 256)
-    s += __hip_atomic_load(&partials[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int t = threadIdx.x; t < nb; t += 256) s += partials[t];
   acc[threadIdx.x] = s;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -426,7 +428,6 @@ __global__ __launch_bounds__(256) void k_project_bc(
       scal[TC_S_TIME] = scal[TC_S_TIME] + dtd;
       scal[TC_S_STEP] = scal[TC_S_STEP] + 1.0;
     }
-    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -520,7 +521,7 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
 #define TC_K5(RR)                                                                                \
   hipLaunchKernelGGL((k_project_bc<T, RR>), dim3(blocks), dim3(256), 0, st, d, mt, (const T*)su,  \
                      (const T*)sv, (const T*)sw, (const T*)p, (T*)u, (T*)v, (T*)w, scal, partials, \
-                     counter, inv_volume, flags)
+                     flags)
   switch (R) {
     case 1: TC_K5(1); break;
     case 2: TC_K5(2); break;
@@ -530,6 +531,10 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
   }
 #undef TC_K5
   TC_CHECK_LAUNCH();
+  if (flags & TC_K5_BULK) {
+    hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(256), 0, st, partials, blocks, scal, inv_volume, flags);
+    TC_CHECK_LAUNCH();
+  }
   return TC_OK;
 }
 

@@ -6,6 +6,8 @@ that a seeded start is the same on every machine.
 """
 import torch
 
+from .utils import load_flow_fields
+
 
 def initialize_flow(nx, ny, nz, z_c, Ly, Lz, U_bulk=1.0, init_type="parabolic", perturbation_intensity=0.0,
                     n_vortices=4, device="cpu", top_wall_bc_type="dirichlet", seed=None, dtype=torch.float64):
@@ -53,3 +55,20 @@ def initialize_flow(nx, ny, nz, z_c, Ly, Lz, U_bulk=1.0, init_type="parabolic", 
     w[:, :, 0] = 0.0
     w[:, :, -1] = 0.0
     return tuple(t.to(device=device, dtype=dtype) for t in (u, v, w, p))
+
+
+def initialize_flow_from_file(field_file, device="cpu", reset_time=False, dtype=None):
+    """initflow.py:106-167: fields from a saved npz, step/time kept or reset, wall ghosts
+    re-applied (no-slip reflections for u, v; zero w at the walls)."""
+    f = load_flow_fields(field_file, device="cpu")
+    u, v, w, p = (f[k] for k in ("u", "v", "w", "p"))
+    u[:, :, 0] = -u[:, :, 1]
+    u[:, :, -1] = -u[:, :, -2]
+    v[:, :, 0] = -v[:, :, 1]
+    v[:, :, -1] = -v[:, :, -2]
+    w[:, :, 0] = 0.0
+    w[:, :, -1] = 0.0
+    step, time = (0, 0.0) if reset_time else (int(f["step"]), float(f["time"]))
+    dt = dtype or u.dtype
+    u, v, w, p = (t.to(device=device, dtype=dt).contiguous() for t in (u, v, w, p))
+    return u, v, w, p, step, time

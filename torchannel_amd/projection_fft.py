@@ -9,7 +9,7 @@ import math
 import torch
 
 from . import _lib as L
-from .tridiag import pcr_solve
+from .tridiag import thomas_solve
 
 
 def initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f, top_wall_bc_type="dirichlet", dtype=None):
@@ -60,5 +60,5 @@ def solve_poisson_fft(div, fft_data):
 
 
 def solve_tridiagonal(a, b, c, d):
-    """projection_fft.py:167-215 (batched, per-row diagonals); solved by PCR on the device."""
-    return pcr_solve(a, b, c, d)
+    """projection_fft.py:167-215: batched Thomas sweep with per-row diagonals, on the device."""
+    return thomas_solve(a, b, c, d)

@@ -21,10 +21,11 @@ import torch
 import yaml
 
 from . import _lib as L
-from .initflow import initialize_flow
+from .initflow import initialize_flow, initialize_flow_from_file
 from .projection_fft import initialize_fft_solver
 from .turbstats import TurbulenceStats
-from .utils import compute_divergence, compute_u_tau, generate_grid
+from .utils import (append_timeseries, compute_bulk_velocity, compute_divergence, compute_u_tau, generate_grid,
+                    save_flow_fields)
 
 
 def apply_bc_all(u, v, w, top_wall_bc_type="dirichlet"):
@@ -104,17 +105,24 @@ class ChannelFlow:
         self.total_volume = self.Lx * self.Ly * self.Lz
 
         ini = config["initialization"]
-        self.u, self.v, self.w, self.p = initialize_flow(
-            self.nx, self.ny, self.nz, z_c, self.Ly, self.Lz, U_bulk=self.U_bulk,
-            init_type=ini.get("type", "parabolic"),
-            perturbation_intensity=ini.get("perturbation_intensity", 0.0),
-            n_vortices=ini.get("n_vortices", 4), device=self.device,
-            top_wall_bc_type=self.top_wall_bc_type, seed=ini.get("seed"), dtype=self.wdtype)
+        field_file = ini.get("field_file")
+        self.initial_step, self.initial_time = 0, 0.0
+        if field_file is not None:                                  # restart (solver.py:246-249)
+            self.u, self.v, self.w, p0, self.initial_step, self.initial_time = initialize_flow_from_file(
+                field_file, device=self.device, reset_time=ini.get("reset_time", False), dtype=self.wdtype)
+        else:
+            self.u, self.v, self.w, p0 = initialize_flow(
+                self.nx, self.ny, self.nz, z_c, self.Ly, self.Lz, U_bulk=self.U_bulk,
+                init_type=ini.get("type", "parabolic"),
+                perturbation_intensity=ini.get("perturbation_intensity", 0.0),
+                n_vortices=ini.get("n_vortices", 4), device=self.device,
+                top_wall_bc_type=self.top_wall_bc_type, seed=ini.get("seed"), dtype=self.wdtype)
 
         self.fft_data = initialize_fft_solver(self.nx, self.ny, self.nz, self.dx, self.dy, self.dz_c,
                                               self.dz_f, top_wall_bc_type=self.top_wall_bc_type,
                                               dtype=self.wdtype)
         self.p = self.fft_data["workspace_p"]
+        self.p.copy_(p0)
         self._grid = L.get_grid(self.nx, self.ny, self.nz, self.dx, self.dy, self.nu,
                                 L.top_code(self.top_wall_bc_type), self.wdtype, self.device,
                                 dz_f=self.dz_f, dz_c=self.dz_c)
@@ -124,11 +132,15 @@ class ChannelFlow:
         self.n_stats = st.get("n_stats", 0)
         self.t_stats = st.get("t_stats", 10.0)
         self.turbulence_stats = None
+        self.stats_output_path = os.path.join(self.results_folder, st.get("output_file", "turbulence_stats.npz"))
+        self.stats_state_path = os.path.join(self.results_folder, st.get("state_file", "turbulence_stats_state.npz"))
         if self.n_stats > 0:
             self.turbulence_stats = TurbulenceStats(
                 self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz, self.z_c, self.z_f, self.dz_c,
                 self.dz_f, self.dx, self.dy, self.nu, self.Re_tau,
                 z_plus_target=st.get("z_plus_target", 15.0), device=self.device)
+            if st.get("restart_state_file") is not None:
+                self.turbulence_stats.load_state(st["restart_state_file"])
 
         # AB2 history (two rhs sets swapped each step) and the device scalar block
         self._rhs = [[torch.zeros_like(q) for q in (self.u, self.v, self.w)] for _ in range(2)]
@@ -146,11 +158,12 @@ class ChannelFlow:
         self._solver = h
         self._graph_key = None
 
-        # rescale to U_bulk (solver.py:268-276) and the initial projection (solver.py:342-369)
-        L.call("tc_solver_project_initial", self._solver, self._stream())
-        self.time = 0.0
-        self.current_step = 0
-        self.initial_step = 0
+        # rescale to U_bulk unless restarting (solver.py:268-276) and the initial projection
+        # (solver.py:342-369); forcing and AB2 history start from zero (solver.py:249, 377-383)
+        L.call("tc_solver_project_initial", self._solver, 0 if field_file is not None else 1, self._stream())
+        self.time = self.initial_time
+        self._scal[L.S_TIME] = self.time
+        self.current_step = self.initial_step
 
     # -- plumbing --------------------------------------------------------------------------
     def _stream(self):
@@ -241,34 +254,67 @@ class ChannelFlow:
         self.current_step += n_chunks * chunk_steps
 
     def run_simulation(self):
-        """Stepping loop of solver.py:738-859: adaptive dt every dt_update_interval steps (on the
-        device), diagnostics every n_out steps, statistics every n_stats steps."""
+        """The loop of solver.py:738-970: adaptive dt every dt_update_interval steps (evaluated on
+        the device), diagnostics and the timeseries every n_out steps, statistics every n_stats
+        steps, and every n_save steps the non-finite check (a device flag set by the projection
+        kernel), timeseries.npz append, fields.npz and the statistics state.  Host syncs happen
+        only at those cadences; the t_max test uses the device time read at them."""
         if self.time_scheme != "IMEX":
             raise ValueError(f"Unknown time stepping scheme: {self.time_scheme}. Use 'IMEX'.")
+        os.makedirs(self.results_folder, exist_ok=True)
+        ts = {k: [] for k in ("step", "time", "u_bulk", "u_tau", "forcing")}
         step = self.current_step
-        self._scal[L.S_TIME] = self.time
         t0 = _time.time()
+        ub = f = None
         while step < self.n_steps and self.time < self.t_max:
             step += 1
             self.current_step = step
             if self.dt_update_interval > 0 and step % self.dt_update_interval == 0:
                 self.update_dt_device()
             ub, f = self.step_imex(None)
-            if step % self.n_out == 0 or (self.n_stats > 0 and step % self.n_stats == 0):
+            need_host = step % self.n_out == 0 or step % self.n_save == 0 or (
+                self.n_stats > 0 and step % self.n_stats == 0)
+            if need_host:
                 self.time = float(self._scal[L.S_TIME].item())
                 self.dt = float(self._scal[L.S_DT].item())
             if step % self.n_out == 0:
                 div = compute_divergence(self.u, self.v, self.w, self.nx, self.ny, self.nz, self.dx, self.dy, self.dz_f)
                 ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
+                for k, val in (("step", step), ("time", self.time), ("u_bulk", float(ub)), ("u_tau", float(ut)),
+                               ("forcing", float(f))):
+                    ts[k].append(val)
                 if self.verbose:
                     print(f"{step:6d} {self.time:10.6f} {self.dt:10.6f} {float(div.abs().max()):12.3e} "
                           f"{float(ub):10.6f} {float(ut):10.6f} {float(f):12.3e}", flush=True)
             if (self.turbulence_stats is not None and self.n_stats > 0 and self.time >= self.t_stats
                     and step % self.n_stats == 0):
                 self.turbulence_stats.accumulate_statistics(self.u, self.v, self.w)
+            if step % self.n_save == 0:
+                ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
+                if float(self._scal[L.S_NONFINITE].item()) != 0.0:          # solver.py:862-881
+                    if self.verbose:
+                        print(f"ERROR: NaN or Inf detected in solution at step {step}", flush=True)
+                    save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step,
+                                     self.time, ut, (self.U_bulk - float(ub)) / self.dt, self.results_folder,
+                                     "fields_error.npz")
+                    break
+                if ts["step"]:
+                    append_timeseries(os.path.join(self.results_folder, "timeseries.npz"), **ts)
+                    ts = {k: [] for k in ts}
+                save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step,
+                                 self.time, ut, f, self.results_folder, "fields.npz")
+                if self.turbulence_stats is not None and self.turbulence_stats.n_samples > 0:
+                    self.turbulence_stats.save_state(self.stats_state_path)
         self.time = float(self._scal[L.S_TIME].item())
         self.dt = float(self._scal[L.S_DT].item())
         if self.verbose:
-            print(f"Simulation complete: {step} steps, total time = {self.time:.6f}, "
+            print(f"Simulation complete: {step} steps, total time = {self.time:.6f}, final dt = {self.dt:.6f}, "
                   f"wall {(_time.time() - t0):.2f}s", flush=True)
+        if self.turbulence_stats is not None and self.turbulence_stats.n_samples > 0:
+            self.turbulence_stats.save_state(self.stats_state_path)
+            self.turbulence_stats.save_statistics(self.stats_output_path)
+        ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
+        ubf = compute_bulk_velocity(self.u, self.cell_vol_ratio, self.total_volume)
+        save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step, self.time, ut,
+                         (self.U_bulk - float(ubf)) / self.dt, self.results_folder, "fields.npz")
         return step

@@ -1,5 +1,7 @@
 """On-the-fly turbulence statistics (turbstats.py:20-303) with the reductions and plane FFTs on
 the HIP library; running sums are float64 on the device."""
+import os
+
 import numpy as np
 import torch
 
@@ -64,3 +66,41 @@ class TurbulenceStats:
             "nu": self.nu, "Re": 1.0 / self.nu,
             "u_tau": float(np.sqrt(self.nu * dudz)) if self.nu * dudz >= 0 else float("nan"),
         }
+
+    def save_statistics(self, filepath):
+        """turbstats.py:305-314."""
+        np.savez_compressed(filepath, **self.finalize_statistics())
+
+    @staticmethod
+    def load_statistics(filepath):
+        """turbstats.py:316-332."""
+        if not os.path.exists(filepath):
+            raise FileNotFoundError(f"Statistics file not found: {filepath}")
+        data = np.load(filepath, allow_pickle=False)
+        return {k: data[k] for k in data.files}
+
+    def save_state(self, filepath):
+        """turbstats.py:334-364: running sums, n_samples and the grid for validation."""
+        host = lambda t: np.asarray(t.detach().cpu().numpy())
+        np.savez_compressed(filepath, n_samples=self.n_samples,
+                            U_sum=host(self.U_sum), uu_sum=host(self.uu_sum), vv_sum=host(self.vv_sum),
+                            ww_sum=host(self.ww_sum), uw_sum=host(self.uw_sum),
+                            E_uu_2d_sum=host(self.E_uu_2d_sum), E_vv_2d_sum=host(self.E_vv_2d_sum),
+                            E_ww_2d_sum=host(self.E_ww_2d_sum), E_uw_2d_sum=host(self.E_uw_2d_sum),
+                            nx=self.nx, ny=self.ny, nz=self.nz)
+
+    def load_state(self, filepath):
+        """turbstats.py:366-402: restore the sums (grid must match)."""
+        if not os.path.exists(filepath):
+            raise FileNotFoundError(f"Statistics state file not found: {filepath}")
+        data = np.load(filepath, allow_pickle=False)
+        if data["nx"] != self.nx or data["ny"] != self.ny or data["nz"] != self.nz:
+            raise ValueError(
+                f"Grid dimension mismatch! State file has (nx={data['nx']}, ny={data['ny']}, nz={data['nz']}) "
+                f"but current simulation has (nx={self.nx}, ny={self.ny}, nz={self.nz})")
+        self.n_samples = int(data["n_samples"])
+        for dst, key in ((self.U_sum, "U_sum"), (self.uu_sum, "uu_sum"), (self.vv_sum, "vv_sum"),
+                         (self.ww_sum, "ww_sum"), (self.uw_sum, "uw_sum"), (self.E_uu_2d_sum, "E_uu_2d_sum"),
+                         (self.E_vv_2d_sum, "E_vv_2d_sum"), (self.E_ww_2d_sum, "E_ww_2d_sum"),
+                         (self.E_uw_2d_sum, "E_uw_2d_sum")):
+            dst.copy_(torch.as_tensor(data[key], dtype=torch.float64))

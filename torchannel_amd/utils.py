@@ -3,6 +3,9 @@
 generate_grid is one-off setup and uses the reference's own tensor formulas on the target
 device; the per-step functions (divergence, bulk velocity, wall shear) run HIP kernels.
 """
+import os
+
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -65,3 +68,40 @@ def compute_u_tau(u, z_c, nu, top_wall_bc_type="dirichlet"):
     out = torch.empty((), dtype=torch.float64, device=u.device)
     L.call("tc_u_tau", g.handle, L.ptr(u), float(z_c[1]), L.ptr(out), L.stream_of(u))
     return out if dt == torch.float64 else out.to(dt)
+
+
+def save_flow_fields(u, v, w, p, z_c, z_f, Lx, Ly, step, time, u_tau, forcing, results_folder,
+                     filename="fields.npz"):
+    """utils.py:369-400: same npz keys (u, v, w, p, z_c, z_f, Lx, Ly, step, time, u_tau, forcing)."""
+    def host(x):
+        return x.detach().cpu().numpy() if torch.is_tensor(x) else x
+
+    def scalar(x):
+        return x.item() if torch.is_tensor(x) else x
+
+    os.makedirs(results_folder, exist_ok=True)
+    np.savez(os.path.join(results_folder, filename), u=host(u), v=host(v), w=host(w), p=host(p),
+             z_c=host(z_c), z_f=host(z_f), Lx=Lx, Ly=Ly, step=step, time=time, u_tau=scalar(u_tau),
+             forcing=scalar(forcing))
+
+
+def load_flow_fields(filepath, device="cpu"):
+    """utils.py:402-444 (npz loaded without pickles)."""
+    if not os.path.exists(filepath):
+        raise FileNotFoundError(f"Field file not found: {filepath}")
+    data = np.load(filepath, allow_pickle=False)
+    out = {k: torch.tensor(data[k], device=device) for k in ("u", "v", "w", "p", "z_c", "z_f")}
+    out.update(Lx=float(data["Lx"]), Ly=float(data["Ly"]), step=int(data["step"]), time=float(data["time"]),
+               u_tau=float(data["u_tau"]), forcing=float(data["forcing"]))
+    return out
+
+
+def append_timeseries(path, step, time, u_bulk, u_tau, forcing):
+    """The timeseries.npz append of solver.py:889-929 (keys step, time, u_bulk, u_tau, forcing)."""
+    chunk = {"step": np.asarray(step, dtype=np.int32), "time": np.asarray(time, dtype=np.float64),
+             "u_bulk": np.asarray(u_bulk, dtype=np.float64), "u_tau": np.asarray(u_tau, dtype=np.float64),
+             "forcing": np.asarray(forcing, dtype=np.float64)}
+    if os.path.exists(path):
+        old = np.load(path, allow_pickle=False)
+        chunk = {k: np.concatenate([old[k], chunk[k]]) for k in chunk}
+    np.savez_compressed(path, **chunk)
