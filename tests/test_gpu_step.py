@@ -57,15 +57,20 @@ def test_steps_match_oracle(tc, shape, top):
                     top_wall_bc_type=top)
     for q, r in ((ch.u, ref.u), (ch.v, ref.v), (ch.w, ref.w)):
         assert rel(q, r) < TOL
+    f_used, rf_used = 0.0, 0.0
     for s in range(6):
         ub, f = ch.step_imex(0.005)
         rub, rf = ref.step_imex(0.005)
         for q, r, name in ((ch.u, ref.u, "u"), (ch.v, ref.v, "v"), (ch.w, ref.w, "w")):
             assert rel(q, r) < TOL, (name, s)
-        assert rel(_pmod(ch.p), _pmod(ref.p)) < 1e-10, ("p", s)
+        assert rel(_pmod(ch.p), _pmod(ref.p)) < TOL, ("p", s)
         assert abs(float(ub) - float(rub)) < 1e-13
-        assert abs(float(f) - float(rf)) < 1e-10 * max(1.0, abs(float(rf)))
-        assert rel(ch.rhs_u_prev, ref.rhs_prev[0]) < TOL
+        # F += 0.1*(U_b - u_b)/dt cancels: a last-bit difference in u_b (summation order) moves F
+        # by ~1e-14 absolute, so F is checked absolutely and the rhs without it relatively
+        assert abs(float(f) - float(rf)) < 1e-12
+        assert rel(ch.rhs_u_prev - f_used, ref.rhs_prev[0] - rf_used) < TOL
+        assert rel(ch.rhs_v_prev, ref.rhs_prev[1]) < TOL and rel(ch.rhs_w_prev, ref.rhs_prev[2]) < TOL
+        f_used, rf_used = float(f), float(rf)
         div = tc.compute_divergence(ch.u, ch.v, ch.w, nx, ny, nz, ch.dx, ch.dy, ch.dz_f)
         assert float(div.abs().max()) < 1e-10
 

@@ -63,6 +63,24 @@ __device__ __forceinline__ int xcd_remap(int b, int nb) {
   return x < rem ? x * (q + 1) + idx : rem * (q + 1) + (x - rem) * q + idx;
 }
 
+// Column order of the per-column kernels: wave g -> (component, i, j), component fastest, the
+// (i, j) plane walked in TC_TI x TC_TJ tiles so that the i+-1 and j+-1 neighbour columns a
+// stencil reads were loaded by waves only a tile earlier (still in the XCD's L2).
+#define TC_TI 8
+#define TC_TJ 8
+__host__ __device__ __forceinline__ int tiled_waves(int nxp, int nyp) {
+  return 3 * ((nxp + TC_TI - 1) / TC_TI) * TC_TI * ((nyp + TC_TJ - 1) / TC_TJ) * TC_TJ;
+}
+__device__ __forceinline__ bool tiled_column(int g, int nxp, int nyp, int& comp, int& i, int& j) {
+  comp = g % 3;
+  const int pos = g / 3;
+  const int tj_n = (nyp + TC_TJ - 1) / TC_TJ;
+  const int tile = pos / (TC_TI * TC_TJ), w = pos - tile * (TC_TI * TC_TJ);
+  i = (tile / tj_n) * TC_TI + w / TC_TJ;
+  j = (tile - (tile / tj_n) * tj_n) * TC_TJ + (w - (w / TC_TJ) * TC_TJ);
+  return i < nxp && j < nyp;
+}
+
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (TC_WAVE - 1); }
 
 template <typename T>

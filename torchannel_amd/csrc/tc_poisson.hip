@@ -256,23 +256,53 @@ __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb
       }
       c[0] = {T(0), T(0)};
     } else {
-      cplx<T> dp = c[0];
-      const T id0 = iden[s];
-      dp = {dp.re * id0, dp.im * id0};
-      c[0] = dp;
-      for (int k = 1; k < nz; ++k) {
-        const T a = (T)cl[k];
-        const T id = iden[(size_t)k * nsys + s];
-        const cplx<T> d = c[k];
-        dp = {(d.re - a * dp.re) * id, (d.im - a * dp.im) * id};
-        c[k] = dp;
+      // the sweeps walk k with the table loads issued a chunk of 8 rows ahead of their use
+      constexpr int CH = 8;
+      cplx<T> dp = {T(0), T(0)};
+      T idb[CH], idn[CH];
+#pragma unroll
+      for (int q = 0; q < CH; ++q) idb[q] = q < nz ? iden[(size_t)q * nsys + s] : T(0);
+      for (int k0 = 0; k0 < nz; k0 += CH) {
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int kn = k0 + CH + q;
+          idn[q] = kn < nz ? iden[(size_t)kn * nsys + s] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int k = k0 + q;
+          if (k < nz) {
+            const T a = k > 0 ? (T)cl[k] : T(0);
+            const cplx<T> d = c[k];
+            dp = {(d.re - a * dp.re) * idb[q], (d.im - a * dp.im) * idb[q]};
+            c[k] = dp;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) idb[q] = idn[q];
       }
       cplx<T> x = dp;
-      for (int k = nz - 2; k >= 0; --k) {
-        const T q = cp[(size_t)k * nsys + s];
-        const cplx<T> d = c[k];
-        x = {d.re - q * x.re, d.im - q * x.im};
-        c[k] = x;
+      T cpb[CH], cpn[CH];
+      const int top = nz - 2;
+#pragma unroll
+      for (int q = 0; q < CH; ++q) cpb[q] = top - q >= 0 ? cp[(size_t)(top - q) * nsys + s] : T(0);
+      for (int k0 = top; k0 >= 0; k0 -= CH) {
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int kn = k0 - CH - q;
+          cpn[q] = kn >= 0 ? cp[(size_t)kn * nsys + s] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+          const int k = k0 - q;
+          if (k >= 0) {
+            const cplx<T> d = c[k];
+            x = {d.re - cpb[q] * x.re, d.im - cpb[q] * x.im};
+            c[k] = x;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) cpb[q] = cpn[q];
       }
     }
   }

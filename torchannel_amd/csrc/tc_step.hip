@@ -208,11 +208,8 @@ __global__ __launch_bounds__(256) void k_explicit_cn(
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int g = b * 4 + (threadIdx.x >> 6);
   const int lane = lane_id();
-  const int comp = g % 3;
-  const int col = g / 3;
-  const int Y = d.ny + 2;
-  const int i = col / Y, j = col - (col / Y) * Y;
-  if (i >= d.nx + 2) return;
+  int comp, i, j;
+  if (!tiled_column(g, d.nx + 2, d.ny + 2, comp, i, j)) return;
   if (comp == 0 && i > d.nx) return;
   if (comp == 1 && j > d.ny) return;
 
@@ -347,14 +344,12 @@ __global__ __launch_bounds__(256) void k_project_bc(
   const int wv = threadIdx.x >> 6;
   const int g = b * 4 + wv;
   const int lane = lane_id();
-  const int comp = g % 3;
-  const int col = g / 3;
-  const int Y = d.ny + 2;
-  const int i = col / Y, j = col - (col / Y) * Y;
+  int comp, i, j;
+  const bool in_grid = tiled_column(g, d.nx + 2, d.ny + 2, comp, i, j);
   const T dt = (T)scal[TC_S_DT];
   double part = 0.0;
   bool bad = false;
-  bool live = i < d.nx + 2 && !(comp == 0 && i > d.nx) && !(comp == 1 && j > d.ny);
+  bool live = in_grid && !(comp == 0 && i > d.nx) && !(comp == 1 && j > d.ny);
   if (live) {
     const int si = (i == 0) ? d.nx : (i == d.nx + 1 ? 1 : i);
     const int sj = (j == 0) ? d.ny : (j == d.ny + 1 ? 1 : j);
@@ -441,7 +436,7 @@ static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void*
                        const double* scal, const void* body, int mode, int branch, double theta,
                        hipStream_t st) {
   const Dims& d = gr->d;
-  const int waves = 3 * (d.nx + 2) * (d.ny + 2);
+  const int waves = tiled_waves(d.nx + 2, d.ny + 2);
   const int blocks = (waves + 3) / 4;
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
@@ -514,7 +509,7 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
                        void* u, void* v, void* w, double* scal, double* partials, unsigned* counter,
                        double inv_volume, int flags, hipStream_t st) {
   const Dims& d = gr->d;
-  const int waves = 3 * (d.nx + 2) * (d.ny + 2);
+  const int waves = tiled_waves(d.nx + 2, d.ny + 2);
   const int blocks = (waves + 3) / 4;
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
@@ -538,7 +533,7 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
   return TC_OK;
 }
 
-int k5_blocks(const Grid* gr) { return (3 * (gr->d.nx + 2) * (gr->d.ny + 2) + 3) / 4; }
+int k5_blocks(const Grid* gr) { return (tiled_waves(gr->d.nx + 2, gr->d.ny + 2) + 3) / 4; }
 
 int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p, void* u,
               void* v, void* w, double* scal, double* partials, unsigned* counter, double inv_volume,
