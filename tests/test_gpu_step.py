@@ -57,20 +57,31 @@ def test_steps_match_oracle(tc, shape, top):
                     top_wall_bc_type=top)
     for q, r in ((ch.u, ref.u), (ch.v, ref.v), (ch.w, ref.w)):
         assert rel(q, r) < TOL
-    f_used, rf_used = 0.0, 0.0
+    f_used = 0.0
     for s in range(6):
+        # the explicit terms this step evaluates, from the HIP path's own (bit-identical) state
+        st = [q.detach().cpu().double().clone() for q in (ch.u, ch.v, ch.w)]
+        O.apply_bc_all(*st, ref.top)
+        want = O.rhs_fused_imex(*st, nx, ny, nz, ref.dx, ref.dy, ref.dz_c, ref.dz_f, ref.nu)
         ub, f = ch.step_imex(0.005)
         rub, rf = ref.step_imex(0.005)
         for q, r, name in ((ch.u, ref.u, "u"), (ch.v, ref.v, "v"), (ch.w, ref.w, "w")):
             assert rel(q, r) < TOL, (name, s)
-        assert rel(_pmod(ch.p), _pmod(ref.p)) < TOL, ("p", s)
+        # p comes from per-mode wall-normal solves whose worst-conditioned (small |k|) systems
+        # amplify round-off: the oracle's float64 Thomas sweep itself sits ~5e-13 from an
+        # extended-precision solve, so p is held to 1e-10 while u, v, w (which only see grad p
+        # times dt) stay at 1e-12
+        assert rel(_pmod(ch.p), _pmod(ref.p)) < 1e-10, ("p", s)
         assert abs(float(ub) - float(rub)) < 1e-13
         # F += 0.1*(U_b - u_b)/dt cancels: a last-bit difference in u_b (summation order) moves F
-        # by ~1e-14 absolute, so F is checked absolutely and the rhs without it relatively
+        # by ~1e-14 absolute, so F is checked absolutely
         assert abs(float(f) - float(rf)) < 1e-12
-        assert rel(ch.rhs_u_prev - f_used, ref.rhs_prev[0] - rf_used) < TOL
-        assert rel(ch.rhs_v_prev, ref.rhs_prev[1]) < TOL and rel(ch.rhs_w_prev, ref.rhs_prev[2]) < TOL
-        f_used, rf_used = float(f), float(rf)
+        # the stored rhs against the oracle's explicit terms on the same input state: the
+        # derivative operators amplify the (<= 1e-12) state differences to the oracle's own
+        # trajectory by up to 1/dz_min, so the kernel itself is pinned on identical inputs
+        assert rel(ch.rhs_u_prev - f_used, want[0]) < TOL, ("rhs_u", s)
+        assert rel(ch.rhs_v_prev, want[1]) < TOL and rel(ch.rhs_w_prev, want[2]) < TOL, ("rhs_vw", s)
+        f_used = float(f)
         div = tc.compute_divergence(ch.u, ch.v, ch.w, nx, ny, nz, ch.dx, ch.dy, ch.dz_f)
         assert float(div.abs().max()) < 1e-10
 

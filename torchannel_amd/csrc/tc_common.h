@@ -142,6 +142,76 @@ __device__ __forceinline__ void shift_up(const T (&x)[R], T (&out)[R], T pad, in
   }
 }
 
+// Neighbour exchange along a column through a per-wavefront LDS row buffer: rows live at
+// buf[64R + m] between two zero pads of 64R rows, so row m+-s (s <= 64R) is one ds_read with an
+// immediate offset and out-of-range rows read zero.  LDS operations of one wavefront execute in
+// order; the wave barriers keep the compiler from moving them across each other.
+template <int R, typename T>
+__device__ __forceinline__ void rows_zero_pads(T* buf, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    buf[r * TC_WAVE + lane] = T(0);
+    buf[2 * TC_WAVE * R + r * TC_WAVE + lane] = T(0);
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void rows_put(T* buf, const T (&x)[R], int lane) {
+  // the fences order this wavefront's LDS accesses for the compiler (one lane reads what its
+  // neighbours wrote); a wavefront's LDS instructions execute in order, so they emit no code
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int r = 0; r < R; ++r) buf[TC_WAVE * R + r * TC_WAVE + lane] = x[r];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int OFF, int R, typename T>
+__device__ __forceinline__ void rows_get(const T* buf, T (&out)[R], int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r] = buf[TC_WAVE * R + r * TC_WAVE + lane + OFF];
+}
+
+template <int ST, int R, typename T>
+__device__ __forceinline__ void pcr_shared_stage_lds(T (&dv)[R], T* buf, const double* __restrict__ tab, int n,
+                                                     int lane) {
+  constexpr int s = 1 << ST;
+  if (s >= n) return;
+  rows_put<R, T>(buf, dv, lane);
+  T dn[R], up[R];
+  rows_get<-s, R, T>(buf, dn, lane);
+  rows_get<s, R, T>(buf, up, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    const T al = (T)tab[(2 * ST) * TC_MAX_NZ + m];
+    const T ga = (T)tab[(2 * ST + 1) * TC_MAX_NZ + m];
+    dv[r] = dv[r] + al * dn[r] + ga * up[r];
+  }
+}
+
+// PCR with shared tables (see pcr_shared_solve) using the LDS row buffer for the exchanges.
+template <int R, typename T>
+__device__ __forceinline__ void pcr_shared_solve_lds(T (&dv)[R], T* buf, const double* __restrict__ tab, int n,
+                                                     int lane) {
+  pcr_shared_stage_lds<0, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<1, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<2, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<3, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<4, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<5, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<6, R, T>(dv, buf, tab, n, lane);
+  pcr_shared_stage_lds<7, R, T>(dv, buf, tab, n, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    dv[r] = dv[r] / (T)tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m];
+  }
+}
+
 // Parallel cyclic reduction of one column whose matrix is shared by every column (the CN
 // systems), so the whole a/b/c reduction is a precomputed table: per stage st, alpha[m] at
 // tab[st*2*MAXNZ + m], gamma[m] at tab[(st*2+1)*MAXNZ + m]; final diagonal b at

@@ -45,6 +45,7 @@ struct Grid {
   unsigned* counter = nullptr;    // last-block counters (several)
   double* tabs = nullptr;         // CN tables: cell, face
   double* scal = nullptr;         // scalar block for stand-alone ops
+  int k1_jb = 0;                  // j-columns per K1 pencil (0: default 8; set by the tuner)
 
   template <typename T>
   Metric<T> metric() const;

@@ -71,83 +71,21 @@ __global__ __launch_bounds__(TC_MAX_NZ) void k_cn_tables(const double* __restric
 
 // ---------------------------------------------------------------------------------------------
 // Explicit right-hand sides, compute_momentum_rhs_fused_imex (operators.py:1078-1223).
+// Each function receives wavefront-uniform column pointers (scalar registers) and the lane's row
+// index k, so every load is a scalar base plus one shared per-lane offset.
 // cpu_branch widens the xy-Laplacian to u-face nx / v-face ny with the periodic wrap of
-// diffusion_xy_u/v (operators.py:485-495, 516-521).
+// diffusion_xy_u/v (operators.py:485-495, 516-521): the caller passes the wrapped column.
 // ---------------------------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T rhs_u_at(const Dims& d, const Metric<T>& mt, const T* __restrict__ u,
-                                      const T* __restrict__ v, const T* __restrict__ w, int i, int j,
-                                      int k, bool adv) {
-  const int c = i * d.sxu + j * d.syu + k;
-  const T h = T(0.5);
-  const T uc = u[c], uw = u[c - d.sxu];
-  const T ue = (i == d.nx) ? u[d.sxu + j * d.syu + k] : u[c + d.sxu];
-  const T us = u[c - d.syu], un = u[c + d.syu];
-  const T lap = (ue - T(2) * uc + uw) * mt.kxx + (un - T(2) * uc + us) * mt.kyy;
-  if (!adv) return lap;
-  const T ub = u[c - 1], ut = u[c + 1];
-  const int cv = i * d.sxv + j * d.syv + k;
-  const T v00 = v[cv], v10 = v[cv + d.sxv], v0m = v[cv - d.syv], v1m = v[cv + d.sxv - d.syv];
-  const int cw = i * d.sxw + j * d.syw + k;
-  const T w00 = w[cw], w10 = w[cw + d.sxw], w0m = w[cw - 1], w1m = w[cw + d.sxw - 1];
-  const T ur = h * (uc + ue), ul = h * (uw + uc);
-  const T fx = (ur * ur - ul * ul) * mt.rdx;
-  const T fy = (h * (v00 + v10) * (h * (uc + un)) - h * (v0m + v1m) * (h * (us + uc))) * mt.rdy;
-  const T fz = (h * (w00 + w10) * (h * (uc + ut)) - h * (w0m + w1m) * (h * (ub + uc))) * mt.rdzf[k - 1];
-  return lap - (fx + fy + fz);
-}
-
-template <typename T>
-__device__ __forceinline__ T rhs_v_at(const Dims& d, const Metric<T>& mt, const T* __restrict__ u,
-                                      const T* __restrict__ v, const T* __restrict__ w, int i, int j,
-                                      int k, bool adv) {
-  const int c = i * d.sxv + j * d.syv + k;
-  const T h = T(0.5);
-  const T vc = v[c], vw = v[c - d.sxv], ve = v[c + d.sxv];
-  const T vs = v[c - d.syv];
-  const T vn = (j == d.ny) ? v[i * d.sxv + d.syv + k] : v[c + d.syv];
-  const T lap = (ve - T(2) * vc + vw) * mt.kxx + (vn - T(2) * vc + vs) * mt.kyy;
-  if (!adv) return lap;
-  const T vb = v[c - 1], vt = v[c + 1];
-  const int cu = i * d.sxu + j * d.syu + k;
-  const T u00 = u[cu], u01 = u[cu + d.syu], um0 = u[cu - d.sxu], um1 = u[cu - d.sxu + d.syu];
-  const int cw = i * d.sxw + j * d.syw + k;
-  const T w00 = w[cw], w01 = w[cw + d.syw], w0m = w[cw - 1], w1m = w[cw + d.syw - 1];
-  const T fx = (h * (u00 + u01) * (h * (vc + ve)) - h * (um0 + um1) * (h * (vw + vc))) * mt.rdx;
-  const T vr = h * (vc + vn), vl = h * (vs + vc);
-  const T fy = (vr * vr - vl * vl) * mt.rdy;
-  const T fz = (h * (w00 + w01) * (h * (vc + vt)) - h * (w0m + w1m) * (h * (vb + vc))) * mt.rdzf[k - 1];
-  return lap - (fx + fy + fz);
-}
-
-template <typename T>
-__device__ __forceinline__ T rhs_w_at(const Dims& d, const Metric<T>& mt, const T* __restrict__ u,
-                                      const T* __restrict__ v, const T* __restrict__ w, int i, int j,
-                                      int k) {
-  const int c = i * d.sxw + j * d.syw + k;
-  const T h = T(0.5);
-  const T wc = w[c], ww = w[c - d.sxw], we = w[c + d.sxw], ws = w[c - d.syw], wn = w[c + d.syw];
-  const T wb = w[c - 1], wt = w[c + 1];
-  const int cu = i * d.sxu + j * d.syu + k;
-  const T u00 = u[cu], u01 = u[cu + 1], um0 = u[cu - d.sxu], um1 = u[cu - d.sxu + 1];
-  const int cv = i * d.sxv + j * d.syv + k;
-  const T v00 = v[cv], v01 = v[cv + 1], vm0 = v[cv - d.syv], vm1 = v[cv - d.syv + 1];
-  const T lap = (we - T(2) * wc + ww) * mt.kxx + (wn - T(2) * wc + ws) * mt.kyy;
-  const T fx = (h * (u00 + u01) * (h * (wc + we)) - h * (um0 + um1) * (h * (ww + wc))) * mt.rdx;
-  const T fy = (h * (v00 + v01) * (h * (wc + wn)) - h * (vm0 + vm1) * (h * (ws + wc))) * mt.rdy;
-  const T wr = h * (wc + wt), wl = h * (wb + wc);
-  const T fz = (wr * wr - wl * wl) * mt.rdzc[k];
-  return lap - (fx + fy + fz);
-}
-
 // Right-hand side d = q + beta*D2(q) of the CN system for one u/v column (operators.py:627-644):
 // rows m = 0..n-1 hold cell k = m+1; the wall closures are the interior stencil with the
 // reflected neighbours X[-1] = -X[0] and X[n] = top*X[n-1].
 template <int R, typename T>
-__device__ __forceinline__ void cn_rhs_cell(T (&x)[R], const Metric<T>& mt, int n, int top, T beta, int lane) {
+__device__ __forceinline__ void cn_rhs_cell(T (&x)[R], T* rows, const Metric<T>& mt, int n, int top, T beta,
+                                            int lane) {
   T dn[R], up[R];
-  shift_dn<1, R, T>(x, dn, T(0), lane);
-  shift_up<1, R, T>(x, up, T(0), lane);
+  rows_put<R, T>(rows, x, lane);
+  rows_get<-1, R, T>(rows, dn, lane);
+  rows_get<1, R, T>(rows, up, lane);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int m = r * TC_WAVE + lane;
@@ -170,10 +108,12 @@ __device__ __forceinline__ void cn_rhs_cell(T (&x)[R], const Metric<T>& mt, int 
 // centred on face m, with the wall value W[0] (`wall`) below row 0, zero below that, and the
 // top row's upper difference taken against zero.
 template <int R, typename T>
-__device__ __forceinline__ void cn_rhs_face(T (&x)[R], const Metric<T>& mt, int n, T wall, T beta, int lane) {
+__device__ __forceinline__ void cn_rhs_face(T (&x)[R], T* rows, const Metric<T>& mt, int n, T wall, T beta,
+                                            int lane) {
   T d1[R], d2[R];
-  shift_dn<1, R, T>(x, d1, T(0), lane);
-  shift_dn<2, R, T>(x, d2, T(0), lane);
+  rows_put<R, T>(rows, x, lane);
+  rows_get<-1, R, T>(rows, d1, lane);
+  rows_get<-2, R, T>(rows, d2, lane);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int m = r * TC_WAVE + lane;
@@ -197,99 +137,505 @@ __device__ __forceinline__ void cn_rhs_face(T (&x)[R], const Metric<T>& mt, int 
 // K1: explicit RHS (+ forcing) -> rhs_curr; AB2 -> CN solve -> star (post-CN) velocities.
 // mode bits: TC_K1_RHS write rhs arrays; TC_K1_STEP do AB2 + CN; TC_K1_FIRST Euler start;
 // TC_K1_FORCE add the scalar forcing scal[TC_S_F] (and body force) to every rhs_u entry.
+//
+// Work decomposition: a block of TC_K1_WAVES wavefronts owns one component and a band of
+// TC_K1_WAVES consecutive i; each wavefront marches one (component, i) pencil along j over `jb`
+// columns, carrying in registers the columns the next j reuses (the j-1 / j neighbours of the
+// stencil), so each column of u, v, w is fetched about once per pencil instead of once per
+// stencil tap.  The CN reduction tables of the block's component are staged in LDS once per
+// block; wall-normal neighbour values (k +- 1, PCR strides < 64) go through a per-wavefront LDS
+// row buffer, PCR strides that are multiples of 64 rows are register moves.
+// Explicit terms: compute_momentum_rhs_fused_imex (operators.py:1078-1223); the CPU-branch
+// extension of the xy-Laplacian to u-face nx / v-face ny with the periodic wrap of
+// diffusion_xy_u/v (operators.py:485-495, 516-521); AB2 solver.py:639-646; CN rows
+// operators.py:627-644 (u, v) and 788-803 (w, shifted explicit half).
 // ---------------------------------------------------------------------------------------------
+#define TC_K1_WAVES 4
+
+template <int R, typename T>
+__device__ __forceinline__ void col_load(const T* __restrict__ col, T (&x)[R], int n, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    x[r] = (int)k <= n ? col[k] : T(0);
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void col_store(T* __restrict__ col, const T (&x)[R], int n, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    if ((int)k <= n) col[k] = x[r];
+  }
+}
+
+// dn = x one row below (row 0 takes `lo`), up = x one row above (row n-1 takes `hi`)
+template <int R, typename T>
+__device__ __forceinline__ void rows_pm(T* buf, const T (&x)[R], T (&dn)[R], T (&up)[R], T lo, T hi, int n,
+                                        int lane) {
+  rows_put<R, T>(buf, x, lane);
+  rows_get<-1, R, T>(buf, dn, lane);
+  rows_get<1, R, T>(buf, up, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m == 0) dn[r] = lo;
+    if (m == n - 1) up[r] = hi;
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void rows_dn(T* buf, const T (&x)[R], T (&dn)[R], T lo, int lane) {
+  rows_put<R, T>(buf, x, lane);
+  rows_get<-1, R, T>(buf, dn, lane);
+  if (lane == 0) dn[0] = lo;
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void rows_up(T* buf, const T (&x)[R], T (&up)[R], T hi, int n, int lane) {
+  rows_put<R, T>(buf, x, lane);
+  rows_get<1, R, T>(buf, up, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r)
+    if (r * TC_WAVE + lane == n - 1) up[r] = hi;
+}
+
+// One PCR stage with the block's LDS copy of the shared reduction table (layout of k_cn_tables
+// with row stride 64*R).
+template <int ST, int R, typename T>
+__device__ __forceinline__ void pcr_stage_tab(T (&dv)[R], T* buf, const T* tabl, int n, int lane) {
+  constexpr int s = 1 << ST;
+  if (s >= n) return;
+  T dn[R], up[R];
+  if constexpr (s >= TC_WAVE) {
+    constexpr int q = s / TC_WAVE;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      dn[r] = r >= q ? dv[r - q] : T(0);
+      up[r] = r + q < R ? dv[r + q] : T(0);
+    }
+  } else {
+    rows_put<R, T>(buf, dv, lane);
+    rows_get<-s, R, T>(buf, dn, lane);
+    rows_get<s, R, T>(buf, up, lane);
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    const T al = tabl[(2 * ST) * TC_WAVE * R + m];
+    const T ga = tabl[(2 * ST + 1) * TC_WAVE * R + m];
+    dv[r] = dv[r] + al * dn[r] + ga * up[r];
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void pcr_solve_tab(T (&dv)[R], T* buf, const T* tabl, int n, int lane) {
+  pcr_stage_tab<0, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<1, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<2, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<3, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<4, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<5, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<6, R, T>(dv, buf, tabl, n, lane);
+  pcr_stage_tab<7, R, T>(dv, buf, tabl, n, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    dv[r] = dv[r] / tabl[2 * TC_MAX_STAGES * TC_WAVE * R + m];
+  }
+}
+
+// Per-pencil constants of one wavefront.
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_explicit_cn(
+struct K1Ctx {
+  Dims d;
+  Metric<T> mt;
+  T* rows;              // this wavefront's LDS row buffer (3*64*R)
+  const T* tabl;        // block's LDS table
+  int lane, mode;
+  bool cpu;
+  T dt, beta, F;
+  const T* body;
+};
+
+// RHS column write (interior + the two wall-normal ghost rows), and AB2 + CN into `sout`.
+template <typename T, int R, bool FACE>
+__device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], const T (&q)[R], const T* rp,
+                                          T* rout, T* sout, const T* bcol, int n, int Kc, bool inner,
+                                          const T (&rz0)[R], const T (&rz1)[R], const T (&rz2)[R],
+                                          T wall) {
+  const int lane = c.lane;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    T val = rhs[r] + c.F;
+    if (bcol != nullptr && (int)k <= n) val = val + bcol[k];
+    rhs[r] = val;
+  }
+  if (c.mode & TC_K1_RHS) {
+    col_store<R, T>(rout, rhs, n, lane);
+    // wall-normal ghost entries of the rhs column carry only the forcing (solver.py:626)
+    if (lane == 0) rout[0] = c.F + (bcol != nullptr ? bcol[0] : T(0));
+    if (lane == 1) rout[Kc - 1] = c.F + (bcol != nullptr ? bcol[Kc - 1] : T(0));
+  }
+  if (!(c.mode & TC_K1_STEP) || !inner) return;
+  const bool first = (c.mode & TC_K1_FIRST) != 0;
+  T x[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    T val = T(0);
+    if ((int)k <= n) {
+      const T inc = first ? rhs[r] : (T(1.5) * rhs[r] - T(0.5) * rp[k]);
+      val = q[r] + c.dt * inc;
+    }
+    x[r] = val;
+  }
+  // CN right-hand side d = x + beta*D2(x) with the rows' metric held in registers:
+  // cells (u, v): rz0 = 1/dz_c[m], rz1 = 1/dz_c[m+1], rz2 = 1/dz_f[m], reflected neighbours;
+  // faces (w): rz0 = 1/dz_f[m], rz1 = 1/dz_f[m+1], rz2 = 1/dz_c[m+1], explicit half centred on
+  // face m with the wall value below row 0 and zero above the top row.
+  T dn[R], up[R];
+  if constexpr (!FACE) {
+    rows_put<R, T>(c.rows, x, lane);
+    rows_get<-1, R, T>(c.rows, dn, lane);
+    rows_get<1, R, T>(c.rows, up, lane);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = r * TC_WAVE + lane;
+      if (m == 0) dn[r] = -x[r];
+      if (m == n - 1) up[r] = c.d.top > 0 ? x[r] : -x[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = r * TC_WAVE + lane;
+      if (m < n) {
+        const T lap = ((up[r] - x[r]) * rz1[r] - (x[r] - dn[r]) * rz0[r]) * rz2[r];
+        x[r] = x[r] + c.beta * lap;
+      } else {
+        x[r] = T(0);
+      }
+    }
+  } else {
+    rows_put<R, T>(c.rows, x, lane);
+    rows_get<-1, R, T>(c.rows, dn, lane);
+    rows_get<-2, R, T>(c.rows, up, lane);       // two rows below
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = r * TC_WAVE + lane;
+      if (m == 0) dn[r] = wall;
+      if (m == 1) up[r] = wall;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int m = r * TC_WAVE + lane;
+      if (m < n) {
+        const T xp = (m == n - 1) ? T(0) : x[r];
+        const T lap = ((xp - dn[r]) * rz1[r] - (dn[r] - up[r]) * rz0[r]) * rz2[r];
+        x[r] = x[r] + c.beta * lap;
+      } else {
+        x[r] = T(0);
+      }
+    }
+  }
+  pcr_solve_tab<R, T>(x, c.rows, c.tabl, n, lane);
+  col_store<R, T>(sout, x, n, lane);
+}
+
+// u pencil: columns (i, j0..j1-1) of u and rhs_u.
+template <typename T, int R>
+__device__ __forceinline__ void k1_pencil_u(const K1Ctx<T, R>& c, int i, int j0, int j1, const T* __restrict__ u,
+                                            const T* __restrict__ v, const T* __restrict__ w,
+                                            const T* __restrict__ pu, T* __restrict__ ru, T* __restrict__ su) {
+  const Dims& d = c.d;
+  const int lane = c.lane, n = d.nz, Kc = d.nz + 2;
+  const T h = T(0.5);
+  const bool i_in = i >= 1 && i <= d.nx;
+  const bool adv = i_in && i <= d.nx - 1;
+  const bool lap_ok = adv || (c.cpu && i_in);
+  T rz0[R], rz1[R], rz2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    rz0[r] = m < n ? c.mt.rdzc[m] : T(0);
+    rz1[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
+    rz2[r] = m < n ? c.mt.rdzf[m] : T(0);
+  }
+  const T* ui = u + (long long)i * d.sxu;
+  const T* uw = ui - d.sxu;
+  const T* ue = (i == d.nx) ? u + d.sxu : ui + d.sxu;        // periodic wrap of the CPU branch
+  const T* vi = v + (long long)i * d.sxv;
+  const T* wi = w + (long long)i * d.sxw;
+  T uS[R], uC[R], v0m[R], v1m[R];
+  const int jin0 = j0 > 1 ? j0 : 1;
+  if (lap_ok && jin0 <= d.ny && jin0 < j1) {
+    col_load<R, T>(ui + (long long)(jin0 - 1) * d.syu, uS, n, lane);
+    col_load<R, T>(ui + (long long)jin0 * d.syu, uC, n, lane);
+    if (adv) {
+      col_load<R, T>(vi + (long long)(jin0 - 1) * d.syv, v0m, n, lane);
+      col_load<R, T>(vi + d.sxv + (long long)(jin0 - 1) * d.syv, v1m, n, lane);
+    }
+  }
+  for (int j = j0; j < j1; ++j) {
+    const bool inner = i_in && j >= 1 && j <= d.ny;
+    const long long cu = (long long)j * d.syu;
+    T rhs[R], q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rhs[r] = T(0);
+    if (inner && lap_ok) {
+      T uN[R], uW[R], uE[R];
+      col_load<R, T>(ui + cu + d.syu, uN, n, lane);
+      col_load<R, T>(uw + cu, uW, n, lane);
+      col_load<R, T>(ue + cu, uE, n, lane);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        rhs[r] = (uE[r] - T(2) * uC[r] + uW[r]) * c.mt.kxx + (uN[r] - T(2) * uC[r] + uS[r]) * c.mt.kyy;
+      if (adv) {
+        const T* w0 = wi + (long long)j * d.syw;
+        const T* w1 = w0 + d.sxw;
+        T v00[R], v10[R], w00[R], w10[R], ub[R], ut[R], w0m[R], w1m[R];
+        col_load<R, T>(vi + (long long)j * d.syv, v00, n, lane);
+        col_load<R, T>(vi + d.sxv + (long long)j * d.syv, v10, n, lane);
+        col_load<R, T>(w0, w00, n, lane);
+        col_load<R, T>(w1, w10, n, lane);
+        const T* uc = ui + cu;
+        rows_pm<R, T>(c.rows, uC, ub, ut, uc[0], uc[n + 1], n, lane);
+        rows_dn<R, T>(c.rows, w00, w0m, w0[0], lane);
+        rows_dn<R, T>(c.rows, w10, w1m, w1[0], lane);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const T ur = h * (uC[r] + uE[r]), ul = h * (uW[r] + uC[r]);
+          const T fx = (ur * ur - ul * ul) * c.mt.rdx;
+          const T fy = (h * (v00[r] + v10[r]) * (h * (uC[r] + uN[r])) - h * (v0m[r] + v1m[r]) * (h * (uS[r] + uC[r]))) * c.mt.rdy;
+          const T fz = (h * (w00[r] + w10[r]) * (h * (uC[r] + ut[r])) - h * (w0m[r] + w1m[r]) * (h * (ub[r] + uC[r]))) * rz2[r];
+          rhs[r] = rhs[r] - (fx + fy + fz);
+          v0m[r] = v00[r];
+          v1m[r] = v10[r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        q[r] = uC[r];
+        uS[r] = uC[r];
+        uC[r] = uN[r];
+      }
+    } else if (inner && (c.mode & TC_K1_STEP)) {
+      col_load<R, T>(ui + cu, q, n, lane);
+    }
+    const T* bcol = c.body != nullptr ? c.body + (long long)i * d.sxu + cu : nullptr;
+    k1_finish<T, R, false>(c, rhs, q, pu ? pu + (long long)i * d.sxu + cu : nullptr, ru + (long long)i * d.sxu + cu,
+                           su ? su + (long long)i * d.sxu + cu : nullptr, bcol, n, Kc, inner, rz0, rz1, rz2, T(0));
+  }
+}
+
+// v pencil: columns (i, j0..j1-1) of v and rhs_v.
+template <typename T, int R>
+__device__ __forceinline__ void k1_pencil_v(const K1Ctx<T, R>& c, int i, int j0, int j1, const T* __restrict__ u,
+                                            const T* __restrict__ v, const T* __restrict__ w,
+                                            const T* __restrict__ pv, T* __restrict__ rv, T* __restrict__ sv) {
+  const Dims& d = c.d;
+  const int lane = c.lane, n = d.nz, Kc = d.nz + 2;
+  const T h = T(0.5);
+  const bool i_in = i >= 1 && i <= d.nx;
+  T rz0[R], rz1[R], rz2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    rz0[r] = m < n ? c.mt.rdzc[m] : T(0);
+    rz1[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
+    rz2[r] = m < n ? c.mt.rdzf[m] : T(0);
+  }
+  const T* vi = v + (long long)i * d.sxv;
+  const T* ui = u + (long long)i * d.sxu;
+  const T* wi = w + (long long)i * d.sxw;
+  T vS[R], vC[R], u00[R], um0[R], w00[R], w0m[R];
+  const int jin0 = j0 > 1 ? j0 : 1;
+  if (i_in && jin0 <= d.ny && jin0 < j1) {
+    col_load<R, T>(vi + (long long)(jin0 - 1) * d.syv, vS, n, lane);
+    col_load<R, T>(vi + (long long)jin0 * d.syv, vC, n, lane);
+    col_load<R, T>(ui + (long long)jin0 * d.syu, u00, n, lane);
+    col_load<R, T>(ui - d.sxu + (long long)jin0 * d.syu, um0, n, lane);
+    const T* w0 = wi + (long long)jin0 * d.syw;
+    col_load<R, T>(w0, w00, n, lane);
+    rows_dn<R, T>(c.rows, w00, w0m, w0[0], lane);
+  }
+  for (int j = j0; j < j1; ++j) {
+    const bool inner = i_in && j >= 1 && j <= d.ny;
+    const bool adv = inner && j <= d.ny - 1;
+    const bool lap_ok = adv || (c.cpu && inner);
+    const long long cv = (long long)j * d.syv;
+    T rhs[R], q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rhs[r] = T(0);
+    if (lap_ok) {
+      T vN[R], vW[R], vE[R];
+      const T* vn = (j == d.ny) ? vi + d.syv : vi + cv + d.syv;    // periodic wrap of the CPU branch
+      col_load<R, T>(vn, vN, n, lane);
+      col_load<R, T>(vi - d.sxv + cv, vW, n, lane);
+      col_load<R, T>(vi + d.sxv + cv, vE, n, lane);
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        rhs[r] = (vE[r] - T(2) * vC[r] + vW[r]) * c.mt.kxx + (vN[r] - T(2) * vC[r] + vS[r]) * c.mt.kyy;
+      if (adv) {
+        const T* w1 = wi + (long long)(j + 1) * d.syw;
+        T u01[R], um1[R], w01[R], w1m[R], vb[R], vt[R];
+        col_load<R, T>(ui + (long long)(j + 1) * d.syu, u01, n, lane);
+        col_load<R, T>(ui - d.sxu + (long long)(j + 1) * d.syu, um1, n, lane);
+        col_load<R, T>(w1, w01, n, lane);
+        const T* vc = vi + cv;
+        rows_pm<R, T>(c.rows, vC, vb, vt, vc[0], vc[n + 1], n, lane);
+        rows_dn<R, T>(c.rows, w01, w1m, w1[0], lane);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const T fx = (h * (u00[r] + u01[r]) * (h * (vC[r] + vE[r])) - h * (um0[r] + um1[r]) * (h * (vW[r] + vC[r]))) * c.mt.rdx;
+          const T vr = h * (vC[r] + vN[r]), vl = h * (vS[r] + vC[r]);
+          const T fy = (vr * vr - vl * vl) * c.mt.rdy;
+          const T fz = (h * (w00[r] + w01[r]) * (h * (vC[r] + vt[r])) - h * (w0m[r] + w1m[r]) * (h * (vb[r] + vC[r]))) * rz2[r];
+          rhs[r] = rhs[r] - (fx + fy + fz);
+          u00[r] = u01[r];
+          um0[r] = um1[r];
+          w00[r] = w01[r];
+          w0m[r] = w1m[r];
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        q[r] = vC[r];
+        vS[r] = vC[r];
+        vC[r] = vN[r];
+      }
+    } else if (inner && (c.mode & TC_K1_STEP)) {
+      col_load<R, T>(vi + cv, q, n, lane);
+    }
+    k1_finish<T, R, false>(c, rhs, q, pv ? pv + (long long)i * d.sxv + cv : nullptr, rv + (long long)i * d.sxv + cv,
+                           sv ? sv + (long long)i * d.sxv + cv : nullptr, nullptr, n, Kc, inner, rz0, rz1, rz2, T(0));
+  }
+}
+
+// w pencil: columns (i, j0..j1-1) of w and rhs_w (rows k = 1..nz-1).
+template <typename T, int R>
+__device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0, int j1, const T* __restrict__ u,
+                                            const T* __restrict__ v, const T* __restrict__ w,
+                                            const T* __restrict__ pw, T* __restrict__ rw, T* __restrict__ sw) {
+  const Dims& d = c.d;
+  const int lane = c.lane, n = d.nz - 1, Kc = d.nz + 1;
+  const T h = T(0.5);
+  const bool i_in = i >= 1 && i <= d.nx;
+  T rz0[R], rz1[R], rz2[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    rz0[r] = m < n ? c.mt.rdzf[m] : T(0);
+    rz1[r] = m < n ? c.mt.rdzf[m + 1] : T(0);
+    rz2[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
+  }
+  const T* wi = w + (long long)i * d.sxw;
+  const T* ui = u + (long long)i * d.sxu;
+  const T* vi = v + (long long)i * d.sxv;
+  T wS[R], wC[R], vm0[R], vm1[R];
+  const int jin0 = j0 > 1 ? j0 : 1;
+  if (i_in && jin0 <= d.ny && jin0 < j1) {
+    col_load<R, T>(wi + (long long)(jin0 - 1) * d.syw, wS, n, lane);
+    col_load<R, T>(wi + (long long)jin0 * d.syw, wC, n, lane);
+    const T* vp = vi + (long long)(jin0 - 1) * d.syv;
+    col_load<R, T>(vp, vm0, n, lane);
+    rows_up<R, T>(c.rows, vm0, vm1, vp[n + 1], n, lane);
+  }
+  for (int j = j0; j < j1; ++j) {
+    const bool inner = i_in && j >= 1 && j <= d.ny;
+    const long long cw = (long long)j * d.syw;
+    T rhs[R], q[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rhs[r] = T(0);
+    if (inner) {
+      T wN[R], wW[R], wE[R], u00[R], um0[R], v00[R], u01[R], um1[R], v01[R], wb[R], wt[R];
+      const T* u0 = ui + (long long)j * d.syu;
+      const T* um = u0 - d.sxu;
+      const T* v0 = vi + (long long)j * d.syv;
+      col_load<R, T>(wi + cw + d.syw, wN, n, lane);
+      col_load<R, T>(wi - d.sxw + cw, wW, n, lane);
+      col_load<R, T>(wi + d.sxw + cw, wE, n, lane);
+      col_load<R, T>(u0, u00, n, lane);
+      col_load<R, T>(um, um0, n, lane);
+      col_load<R, T>(v0, v00, n, lane);
+      const T* wc = wi + cw;
+      rows_pm<R, T>(c.rows, wC, wb, wt, wc[0], wc[n + 1], n, lane);
+      rows_up<R, T>(c.rows, u00, u01, u0[n + 1], n, lane);
+      rows_up<R, T>(c.rows, um0, um1, um[n + 1], n, lane);
+      rows_up<R, T>(c.rows, v00, v01, v0[n + 1], n, lane);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T lap = (wE[r] - T(2) * wC[r] + wW[r]) * c.mt.kxx + (wN[r] - T(2) * wC[r] + wS[r]) * c.mt.kyy;
+        const T fx = (h * (u00[r] + u01[r]) * (h * (wC[r] + wE[r])) - h * (um0[r] + um1[r]) * (h * (wW[r] + wC[r]))) * c.mt.rdx;
+        const T fy = (h * (v00[r] + v01[r]) * (h * (wC[r] + wN[r])) - h * (vm0[r] + vm1[r]) * (h * (wS[r] + wC[r]))) * c.mt.rdy;
+        const T wr = h * (wC[r] + wt[r]), wl = h * (wb[r] + wC[r]);
+        const T fz = (wr * wr - wl * wl) * rz2[r];
+        rhs[r] = lap - (fx + fy + fz);
+        vm0[r] = v00[r];
+        vm1[r] = v01[r];
+        q[r] = wC[r];
+        wS[r] = wC[r];
+        wC[r] = wN[r];
+      }
+    }
+    k1_finish<T, R, true>(c, rhs, q, pw ? pw + (long long)i * d.sxw + cw : nullptr, rw + (long long)i * d.sxw + cw,
+                          sw ? sw + (long long)i * d.sxw + cw : nullptr, nullptr, n, Kc, inner, rz0, rz1, rz2, T(0));
+  }
+}
+
+// Pencil bands: a band is TC_K1_WAVES consecutive i (one per wavefront) by jb consecutive j, on
+// the common (nx+2) x (ny+2) column plane; block b serves component b % 3 of band b / 3, so the
+// three components of one band run back to back on the same XCD.
+__host__ __device__ __forceinline__ int k1_bands_i(int nx) { return (nx + 2 + TC_K1_WAVES - 1) / TC_K1_WAVES; }
+__host__ __device__ __forceinline__ int k1_bands_j(int ny, int jb) { return (ny + 2 + jb - 1) / jb; }
+
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
     Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
     T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
-    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta) {
+    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
+  __shared__ T lds_tab[(2 * TC_MAX_STAGES + 1) * TC_WAVE * R];
+  __shared__ T lds_rows[TC_K1_WAVES][3 * TC_WAVE * R];
   const int b = xcd_remap(blockIdx.x, gridDim.x);
-  const int g = b * 4 + (threadIdx.x >> 6);
+  const int comp = b % 3, band = b / 3;
+  const int nbj = k1_bands_j(d.ny, jb);
+  const int bi = band / nbj, bj = band - bi * nbj;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = lane_id();
-  int comp, i, j;
-  if (!tiled_column(g, d.nx + 2, d.ny + 2, comp, i, j)) return;
-  if (comp == 0 && i > d.nx) return;
-  if (comp == 1 && j > d.ny) return;
-
-  const bool cpu = branch == TC_BRANCH_CPU;
-  const bool inner_ij = i >= 1 && i <= d.nx && j >= 1 && j <= d.ny;
+  if (mode & TC_K1_STEP) {
+    const double* tab = comp < 2 ? tab_cell : tab_face;
+    for (int e = threadIdx.x; e < (2 * TC_MAX_STAGES + 1) * TC_WAVE * R; e += 64 * TC_K1_WAVES) {
+      const int row = e / (TC_WAVE * R), m = e - row * (TC_WAVE * R);
+      lds_tab[e] = (T)tab[row * TC_MAX_NZ + m];
+    }
+    __syncthreads();
+  }
+  const int i = bi * TC_K1_WAVES + wv;
+  const int imax = comp == 0 ? d.nx : d.nx + 1;
+  const int jmax = comp == 1 ? d.ny : d.ny + 1;
+  const int j0 = bj * jb;
+  const int j1 = min(j0 + jb, jmax + 1);
+  if (i > imax || j0 > jmax) return;
+  K1Ctx<T, R> c;
+  c.d = d;
+  c.mt = mt;
+  c.rows = lds_rows[wv];
+  c.tabl = lds_tab;
+  c.lane = lane;
+  c.mode = mode;
+  c.cpu = branch == TC_BRANCH_CPU;
   const double dt = scal[TC_S_DT];
-  const T Fu = (comp == 0 && (mode & TC_K1_FORCE)) ? (T)scal[TC_S_F] : T(0);
-
-  T* rout;
-  const T *q, *rp;
-  T* sout;
-  int base, Kc, n;
-  bool lap_ok, adv_ok;
-  if (comp == 0) {
-    base = i * d.sxu + j * d.syu; Kc = d.nz + 2; n = d.nz;
-    rout = ru; q = u; rp = pu; sout = su;
-    adv_ok = inner_ij && i <= d.nx - 1;
-    lap_ok = adv_ok || (cpu && inner_ij);
-  } else if (comp == 1) {
-    base = i * d.sxv + j * d.syv; Kc = d.nz + 2; n = d.nz;
-    rout = rv; q = v; rp = pv; sout = sv;
-    adv_ok = inner_ij && j <= d.ny - 1;
-    lap_ok = adv_ok || (cpu && inner_ij);
-  } else {
-    base = i * d.sxw + j * d.syw; Kc = d.nz + 1; n = d.nz - 1;
-    rout = rw; q = w; rp = pw; sout = sw;
-    adv_ok = inner_ij;
-    lap_ok = inner_ij;
-  }
-
-  T rhs[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    const int k = m + 1;
-    T val = T(0);
-    if (m < n && lap_ok) {
-      if (comp == 0) val = rhs_u_at<T>(d, mt, u, v, w, i, j, k, adv_ok);
-      else if (comp == 1) val = rhs_v_at<T>(d, mt, u, v, w, i, j, k, adv_ok);
-      else val = rhs_w_at<T>(d, mt, u, v, w, i, j, k);
-    }
-    val = val + Fu;
-    if (comp == 0 && body != nullptr && (mode & TC_K1_FORCE) && m < n) val = val + body[base + k];
-    rhs[r] = val;
-    if ((mode & TC_K1_RHS) && m < n) rout[base + k] = val;
-  }
-  if (mode & TC_K1_RHS) {
-    // wall-normal ghost entries of the rhs column carry only the forcing (solver.py:626)
-    if (lane == 0) rout[base] = Fu + ((comp == 0 && body && (mode & TC_K1_FORCE)) ? body[base] : T(0));
-    if (lane == 1) rout[base + Kc - 1] = Fu + ((comp == 0 && body && (mode & TC_K1_FORCE)) ? body[base + Kc - 1] : T(0));
-  }
-  if (!(mode & TC_K1_STEP) || !inner_ij) return;
-
-  // AB2 (solver.py:639-646): q* = q + dt*rhs (first step) or q + dt*(1.5 rhs - 0.5 rhs_prev)
-  T x[R];
-  const bool first = (mode & TC_K1_FIRST) != 0;
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    T val = T(0);
-    if (m < n) {
-      const T qv = q[base + m + 1];
-      const T inc = first ? rhs[r] : (T(1.5) * rhs[r] - T(0.5) * rp[base + m + 1]);
-      val = qv + (T)dt * inc;
-    }
-    x[r] = val;
-  }
-  const T beta = (T)((1.0 - theta) * dt * mt.nu);
-  if (comp < 2) {
-    cn_rhs_cell<R, T>(x, mt, n, d.top, beta, lane);
-    pcr_shared_solve<R, T>(x, tab_cell, n, lane);
-  } else {
-    cn_rhs_face<R, T>(x, mt, n, T(0), beta, lane);
-    pcr_shared_solve<R, T>(x, tab_face, n, lane);
-  }
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    if (m < n) sout[base + m + 1] = x[r];
-  }
+  c.dt = (T)dt;
+  c.beta = (T)((1.0 - theta) * dt * mt.nu);
+  c.F = (comp == 0 && (mode & TC_K1_FORCE)) ? (T)scal[TC_S_F] : T(0);
+  c.body = (comp == 0 && (mode & TC_K1_FORCE)) ? body : nullptr;
+  rows_zero_pads<R, T>(c.rows, lane);
+  if (comp == 0) k1_pencil_u<T, R>(c, i, j0, j1, u, v, w, pu, ru, su);
+  else if (comp == 1) k1_pencil_v<T, R>(c, i, j0, j1, u, v, w, pv, rv, sv);
+  else k1_pencil_w<T, R>(c, i, j0, j1, u, v, w, pw, rw, sw);
 }
 
 // Stand-alone CN solve of one component (solve_implicit_diffusion_u/v/w, operators.py:563-811):
@@ -299,94 +645,100 @@ template <typename T, int R>
 __global__ __launch_bounds__(256) void k_cn_only(Dims d, Metric<T> mt, int comp, const T* __restrict__ q,
                                                  T* __restrict__ out, const double* __restrict__ tab,
                                                  const double* __restrict__ scal, double theta) {
-  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  __shared__ T lds_rows[4][3 * TC_WAVE * R];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = blockIdx.x * 4 + wv;
   const int lane = lane_id();
   const int i = 1 + g / d.ny, j = 1 + g % d.ny;
   if (i > d.nx) return;
-  int base, n;
-  if (comp == 0) { base = i * d.sxu + j * d.syu; n = d.nz; }
-  else if (comp == 1) { base = i * d.sxv + j * d.syv; n = d.nz; }
-  else { base = i * d.sxw + j * d.syw; n = d.nz - 1; }
+  long long base;
+  int n;
+  if (comp == 0) { base = (long long)i * d.sxu + (long long)j * d.syu; n = d.nz; }
+  else if (comp == 1) { base = (long long)i * d.sxv + (long long)j * d.syv; n = d.nz; }
+  else { base = (long long)i * d.sxw + (long long)j * d.syw; n = d.nz - 1; }
+  const T* qc = q + base;
+  T* oc = out + base;
   const double dt = scal[TC_S_DT];
   T x[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    x[r] = m < n ? q[base + m + 1] : T(0);
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    x[r] = (int)k <= n ? qc[k] : T(0);
   }
+  T* rows = lds_rows[wv];
+  rows_zero_pads<R, T>(rows, lane);
   const T beta = (T)((1.0 - theta) * dt * mt.nu);
   if (theta < 1.0) {
-    if (comp < 2) cn_rhs_cell<R, T>(x, mt, n, d.top, beta, lane);
-    else cn_rhs_face<R, T>(x, mt, n, q[base], beta, lane);
+    if (comp < 2) cn_rhs_cell<R, T>(x, rows, mt, n, d.top, beta, lane);
+    else cn_rhs_face<R, T>(x, rows, mt, n, qc[0], beta, lane);
   }
-  pcr_shared_solve<R, T>(x, tab, n, lane);
+  pcr_shared_solve_lds<R, T>(x, rows, tab, n, lane);
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    if (m < n) out[base + m + 1] = x[r];
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    if ((int)k <= n) oc[k] = x[r];
   }
 }
 
 // ---------------------------------------------------------------------------------------------
 // K5: q = q* - dt*grad(p) (projection.py:154-163) written with every ghost set to its periodic
-// image / wall reflection (apply_bc_all, solver.py:43-79), plus the bulk-velocity partial sums
-// (utils.py:220-223).  The last block to finish reduces the partials in a fixed order and
-// updates the forcing controller (solver.py:701-708) and the simulated time.
+// image / wall reflection (apply_bc_all, solver.py:43-79), plus the per-block partial sums of the
+// bulk velocity (utils.py:220-223), folded by k_step_finalize.  16 wavefronts per block.
 // ---------------------------------------------------------------------------------------------
+#define TC_K5_WAVES 16
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_project_bc(
+__global__ __launch_bounds__(64 * TC_K5_WAVES) void k_project_bc(
     Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
     double* __restrict__ scal, double* __restrict__ partials, int flags) {
-  __shared__ double red[4];
+  __shared__ double red[TC_K5_WAVES];
   const int nb = gridDim.x;
   const int b = xcd_remap(blockIdx.x, nb);
-  const int wv = threadIdx.x >> 6;
-  const int g = b * 4 + wv;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int g = b * TC_K5_WAVES + wv;
   const int lane = lane_id();
   int comp, i, j;
   const bool in_grid = tiled_column(g, d.nx + 2, d.ny + 2, comp, i, j);
   const T dt = (T)scal[TC_S_DT];
   double part = 0.0;
   bool bad = false;
-  bool live = in_grid && !(comp == 0 && i > d.nx) && !(comp == 1 && j > d.ny);
+  const bool live = in_grid && !(comp == 0 && i > d.nx) && !(comp == 1 && j > d.ny);
   if (live) {
     const int si = (i == 0) ? d.nx : (i == d.nx + 1 ? 1 : i);
     const int sj = (j == 0) ? d.ny : (j == d.ny + 1 ? 1 : j);
-    const int pb = si * d.sxp + sj * d.syp;
+    const T* pc = p + (long long)si * d.sxp + (long long)sj * d.syp;
     if (comp < 2) {
-      const int dst = comp == 0 ? i * d.sxu + j * d.syu : i * d.sxv + j * d.syv;
-      const int src = comp == 0 ? si * d.sxu + sj * d.syu : si * d.sxv + sj * d.syv;
-      const T* s = comp == 0 ? su : sv;
-      T* o = comp == 0 ? u : v;
-      const int pstep = comp == 0 ? d.sxp : d.syp;
+      const T* s = comp == 0 ? su + (long long)si * d.sxu + (long long)sj * d.syu
+                             : sv + (long long)si * d.sxv + (long long)sj * d.syv;
+      T* o = comp == 0 ? u + (long long)i * d.sxu + (long long)j * d.syu
+                       : v + (long long)i * d.sxv + (long long)j * d.syv;
+      const T* pn = pc + (comp == 0 ? d.sxp : d.syp);
       const T rd = comp == 0 ? mt.rdx : mt.rdy;
       const bool bulk = comp == 0 && i >= 1 && j >= 1 && j <= d.ny;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int m = r * TC_WAVE + lane;
-        if (m < d.nz) {
-          const int k = m + 1;
-          const T val = s[src + k] - dt * ((p[pb + pstep + k] - p[pb + k]) * rd);
+        const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+        if ((int)k <= d.nz) {
+          const T val = s[k] - dt * ((pn[k] - pc[k]) * rd);
           bad |= !isfinite(val);
-          o[dst + k] = val;
-          if (m == 0) o[dst] = -val;
-          if (m == d.nz - 1) o[dst + k + 1] = d.top > 0 ? val : -val;
-          if (bulk) part += (double)val * mt.wbulk[m];
+          o[k] = val;
+          if (k == 1) o[0] = -val;
+          if ((int)k == d.nz) o[k + 1] = d.top > 0 ? val : -val;
+          if (bulk) part += (double)val * mt.wbulk[k - 1];
         }
       }
     } else {
-      const int dst = i * d.sxw + j * d.syw, src = si * d.sxw + sj * d.syw;
+      const T* s = sw + (long long)si * d.sxw + (long long)sj * d.syw;
+      T* o = w + (long long)i * d.sxw + (long long)j * d.syw;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int m = r * TC_WAVE + lane;
-        if (m < d.nz - 1) {
-          const int k = m + 1;
-          const T val = sw[src + k] - dt * ((p[pb + k + 1] - p[pb + k]) * mt.rdzc[k]);
+        const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+        if ((int)k < d.nz) {
+          const T val = s[k] - dt * ((pc[k + 1] - pc[k]) * mt.rdzc[k]);
           bad |= !isfinite(val);
-          w[dst + k] = val;
+          o[k] = val;
         }
-        if (m == 0) { w[dst] = T(0); w[dst + d.nz] = T(0); }
+        if (k == 1) { o[0] = T(0); o[d.nz] = T(0); }
       }
     }
   }
@@ -397,24 +749,35 @@ __global__ __launch_bounds__(256) void k_project_bc(
   part = wave_sum(part);
   if (lane == 0) red[wv] = part;
   __syncthreads();
-  if (threadIdx.x == 0) partials[b] = (red[0] + red[1]) + (red[2] + red[3]);
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+#pragma unroll
+    for (int q = 0; q < TC_K5_WAVES; ++q) tot += red[q];
+    partials[b] = tot;
+  }
 }
 
 // Folds the per-block bulk partials of k_project_bc in a fixed order (one block), then the
 // forcing controller F += 0.1*(U_b - u_b)/dt (solver.py:701-708), time and step count.
-__global__ __launch_bounds__(256) void k_step_finalize(const double* __restrict__ partials, int nb,
-                                                       double* __restrict__ scal, double inv_volume,
-                                                       int flags) {
-  __shared__ double acc[256];
+__global__ __launch_bounds__(1024) void k_step_finalize(const double* __restrict__ partials, int nb,
+                                                        double* __restrict__ scal, double inv_volume,
+                                                        int flags) {
+  __shared__ double acc[1024];
   double s = 0.0;
-  for (int t = threadIdx.x; t < nb; t += 256) s += partials[t];
-  acc[threadIdx.x] = s;
+  const int t = threadIdx.x;
+  int q = t;
+  for (; q + 3 * 1024 < nb; q += 4 * 1024) {
+    const double a0 = partials[q], a1 = partials[q + 1024], a2 = partials[q + 2048], a3 = partials[q + 3072];
+    s += (a0 + a1) + (a2 + a3);
+  }
+  for (; q < nb; q += 1024) s += partials[q];
+  acc[t] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) acc[threadIdx.x] += acc[threadIdx.x + o];
+  for (int o = 512; o > 0; o >>= 1) {
+    if (t < o) acc[t] += acc[t + o];
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     const double ub = acc[0] * inv_volume;
     const double dtd = scal[TC_S_DT];
     scal[TC_S_UBULK] = ub;
@@ -436,15 +799,15 @@ static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void*
                        const double* scal, const void* body, int mode, int branch, double theta,
                        hipStream_t st) {
   const Dims& d = gr->d;
-  const int waves = tiled_waves(d.nx + 2, d.ny + 2);
-  const int blocks = (waves + 3) / 4;
+  const int jb = gr->k1_jb > 0 ? gr->k1_jb : 8;
+  const int blocks = 3 * k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
 #define TC_K1(RR)                                                                                 \
-  hipLaunchKernelGGL((k_explicit_cn<T, RR>), dim3(blocks), dim3(256), 0, st, d, mt, (const T*)u,   \
-                     (const T*)v, (const T*)w, (const T*)pu, (const T*)pv, (const T*)pw, (T*)ru,   \
-                     (T*)rv, (T*)rw, (T*)su, (T*)sv, (T*)sw, tab_cell, tab_face, scal,            \
-                     (const T*)body, mode, branch, theta)
+  hipLaunchKernelGGL((k_explicit_cn<T, RR>), dim3(blocks), dim3(64 * TC_K1_WAVES), 0, st, d, mt,  \
+                     (const T*)u, (const T*)v, (const T*)w, (const T*)pu, (const T*)pv,            \
+                     (const T*)pw, (T*)ru, (T*)rv, (T*)rw, (T*)su, (T*)sv, (T*)sw, tab_cell,       \
+                     tab_face, scal, (const T*)body, mode, branch, theta, jb)
   switch (R) {
     case 1: TC_K1(1); break;
     case 2: TC_K1(2); break;
@@ -510,11 +873,11 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
                        double inv_volume, int flags, hipStream_t st) {
   const Dims& d = gr->d;
   const int waves = tiled_waves(d.nx + 2, d.ny + 2);
-  const int blocks = (waves + 3) / 4;
+  const int blocks = (waves + TC_K5_WAVES - 1) / TC_K5_WAVES;
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
 #define TC_K5(RR)                                                                                \
-  hipLaunchKernelGGL((k_project_bc<T, RR>), dim3(blocks), dim3(256), 0, st, d, mt, (const T*)su,  \
+  hipLaunchKernelGGL((k_project_bc<T, RR>), dim3(blocks), dim3(64 * TC_K5_WAVES), 0, st, d, mt, (const T*)su,  \
                      (const T*)sv, (const T*)sw, (const T*)p, (T*)u, (T*)v, (T*)w, scal, partials, \
                      flags)
   switch (R) {
@@ -527,13 +890,13 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
 #undef TC_K5
   TC_CHECK_LAUNCH();
   if (flags & TC_K5_BULK) {
-    hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(256), 0, st, partials, blocks, scal, inv_volume, flags);
+    hipLaunchKernelGGL(k_step_finalize, dim3(1), dim3(1024), 0, st, partials, blocks, scal, inv_volume, flags);
     TC_CHECK_LAUNCH();
   }
   return TC_OK;
 }
 
-int k5_blocks(const Grid* gr) { return (tiled_waves(gr->d.nx + 2, gr->d.ny + 2) + 3) / 4; }
+int k5_blocks(const Grid* gr) { return (tiled_waves(gr->d.nx + 2, gr->d.ny + 2) + TC_K5_WAVES - 1) / TC_K5_WAVES; }
 
 int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p, void* u,
               void* v, void* w, double* scal, double* partials, unsigned* counter, double inv_volume,
