@@ -59,6 +59,13 @@ int tc_grid_create(int nx, int ny, int nz, double dx, double dy, double nu, int 
                    const double* dz_f_host, const double* dz_c_host, tc_grid** out);
 int tc_grid_destroy(tc_grid* g);
 
+/* Launch-shape knobs of a grid's kernels (no counterpart in the reference: its kernels are
+ * TorchScript-fused).  Results do not depend on them; torchannel_amd/autotune.py picks them by
+ * timing.  TC_TUNE_K1_JB: j-columns per explicit/CN pencil, 1..ny+2 (0 restores the default). */
+#define TC_TUNE_K1_JB 1
+int tc_grid_set_tuning(tc_grid* g, int key, int value);
+int tc_grid_get_tuning(const tc_grid* g, int key, int* value);
+
 /* apply_bc_all (solver.py:23-79): periodic x/y images, then wall reflections, in place. */
 int tc_apply_bc(const tc_grid* g, void* u, void* v, void* w, void* stream);
 

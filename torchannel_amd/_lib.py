@@ -21,6 +21,7 @@ TC_F64, TC_F32 = 0, 1
 TC_NSCAL = 8
 # scalar-block slots (tc_internal.h)
 S_DT, S_F, S_UBULK, S_TIME, S_CFL, S_UTARGET, S_STEP, S_NONFINITE = 0, 1, 2, 3, 4, 5, 6, 7
+TUNE_K1_JB = 1
 
 _vp, _i, _d, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
 _pd = ctypes.POINTER(ctypes.c_double)
@@ -30,6 +31,8 @@ SIGNATURES = {
     "tc_version": [],
     "tc_grid_create": [_i, _i, _i, _d, _d, _d, _i, _i, _pd, _pd, ctypes.POINTER(_vp)],
     "tc_grid_destroy": [_vp],
+    "tc_grid_set_tuning": [_vp, _i, _i],
+    "tc_grid_get_tuning": [_vp, _i, ctypes.POINTER(_i)],
     "tc_apply_bc": [_vp, _vp, _vp, _vp, _vp],
     "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tc_cn_solve": [_vp, _i, _d, _vp, _d, _vp, _vp, _vp],
@@ -144,6 +147,15 @@ class Grid:
                                      f, c, ctypes.byref(h)))
         self.handle = h
         self._poisson = None
+
+    def set_tuning(self, key, value):
+        """Set a launch-shape knob (TUNE_*); results do not depend on it."""
+        check(load().tc_grid_set_tuning(self.handle, int(key), int(value)))
+
+    def get_tuning(self, key):
+        out = ctypes.c_int()
+        check(load().tc_grid_get_tuning(self.handle, int(key), ctypes.byref(out)))
+        return out.value
 
     def poisson(self):
         if self._poisson is None:

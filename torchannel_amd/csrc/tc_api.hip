@@ -147,6 +147,31 @@ int tc_grid_destroy(tc_grid* h) {
   return TC_OK;
 }
 
+int tc_grid_set_tuning(tc_grid* h, int key, int value) {
+  if (!h) return TC_ERR_ARG;
+  Grid& g = h->g;
+  switch (key) {
+    case TC_TUNE_K1_JB:
+      if (value < 0 || value > g.d.ny + 2) return TC_ERR_ARG;
+      g.k1_jb = value;
+      return TC_OK;
+    default:
+      return TC_ERR_ARG;
+  }
+}
+
+int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
+  if (!h || !value) return TC_ERR_ARG;
+  const Grid& g = h->g;
+  switch (key) {
+    case TC_TUNE_K1_JB:
+      *value = g.k1_jb > 0 ? g.k1_jb : 8;
+      return TC_OK;
+    default:
+      return TC_ERR_ARG;
+  }
+}
+
 int tc_apply_bc(const tc_grid* h, void* u, void* v, void* w, void* stream) {
   if (!h || !u || !v || !w) return TC_ERR_ARG;
   return launch_apply_bc(&h->g, u, v, w, (hipStream_t)stream);

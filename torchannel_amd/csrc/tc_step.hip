@@ -261,7 +261,7 @@ struct K1Ctx {
 
 // RHS column write (interior + the two wall-normal ghost rows), and AB2 + CN into `sout`.
 template <typename T, int R, bool FACE>
-__device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], const T (&q)[R], const T* rp,
+__device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], const T (&q)[R], const T (&rp)[R],
                                           T* rout, T* sout, const T* bcol, int n, int Kc, bool inner,
                                           const T (&rz0)[R], const T (&rz1)[R], const T (&rz2)[R],
                                           T wall) {
@@ -287,7 +287,7 @@ __device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], con
     const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
     T val = T(0);
     if ((int)k <= n) {
-      const T inc = first ? rhs[r] : (T(1.5) * rhs[r] - T(0.5) * rp[k]);
+      const T inc = first ? rhs[r] : (T(1.5) * rhs[r] - T(0.5) * rp[r]);
       val = q[r] + c.dt * inc;
     }
     x[r] = val;
@@ -343,6 +343,70 @@ __device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], con
   col_store<R, T>(sout, x, n, lane);
 }
 
+// Ghost column of an rhs array: the forcing (and body force) on every entry, no star column.
+template <typename T, int R>
+__device__ __forceinline__ void k1_ghost_col(const K1Ctx<T, R>& c, T* rout, const T* bcol, int n, int Kc) {
+  if (!(c.mode & TC_K1_RHS)) return;
+  T val[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + c.lane + 1);
+    val[r] = c.F;
+    if (bcol != nullptr && (int)k <= n) val[r] = val[r] + bcol[k];
+  }
+  col_store<R, T>(rout, val, n, c.lane);
+  if (c.lane == 0) rout[0] = c.F + (bcol != nullptr ? bcol[0] : T(0));
+  if (c.lane == 1) rout[Kc - 1] = c.F + (bcol != nullptr ? bcol[Kc - 1] : T(0));
+}
+
+// Wall-normal metric of the rows this lane holds.
+template <typename T, int R, bool FACE>
+__device__ __forceinline__ void k1_metric(const K1Ctx<T, R>& c, int n, T (&rz0)[R], T (&rz1)[R], T (&rz2)[R]) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + c.lane;
+    if (!FACE) {
+      rz0[r] = m < n ? c.mt.rdzc[m] : T(0);
+      rz1[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
+      rz2[r] = m < n ? c.mt.rdzf[m] : T(0);
+    } else {
+      rz0[r] = m < n ? c.mt.rdzf[m] : T(0);
+      rz1[r] = m < n ? c.mt.rdzf[m + 1] : T(0);
+      rz2[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
+    }
+  }
+}
+
+// Columns of a pencil that carry no stencil (ghost columns, and the GPU-branch u-face nx /
+// v-face ny whose explicit terms are zero): rhs = forcing, AB2 + CN on the inner ones.
+template <typename T, int R, bool FACE>
+__device__ __forceinline__ void k1_plain_cols(const K1Ctx<T, R>& c, const T* qi, const T* rpi, T* ri, T* si,
+                                              const T* bi, int sy, int ja, int jb, int jin_lo, int jin_hi, int n,
+                                              int Kc, const T (&rz0)[R], const T (&rz1)[R], const T (&rz2)[R]) {
+  for (int j = ja; j < jb; ++j) {
+    const long long cj = (long long)j * sy;
+    const T* bcol = bi != nullptr ? bi + cj : nullptr;
+    const bool inner = j >= jin_lo && j <= jin_hi;
+    if (!inner || !(c.mode & TC_K1_STEP)) {
+      k1_ghost_col<T, R>(c, ri + cj, bcol, n, Kc);
+      continue;
+    }
+    T rhs[R], q[R], rp[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rhs[r] = T(0);
+    col_load<R, T>(qi + cj, q, n, c.lane);
+    if (!(c.mode & TC_K1_FIRST)) col_load<R, T>(rpi + cj, rp, n, c.lane);
+    k1_finish<T, R, FACE>(c, rhs, q, rp, ri + cj, si + cj, bcol, n, Kc, true, rz0, rz1, rz2, T(0));
+  }
+}
+
+// Column inputs of one u stencil evaluation at (i, j), fetched one column ahead.
+template <typename T, int R>
+struct UIn {
+  T uN[R], uW[R], uE[R], v00[R], v10[R], w00[R], w10[R], rp[R];
+  T ub0, ubn, wb0, wb1;     // u(i,j) wall-normal ghosts, w(i,j)[0], w(i+1,j)[0]
+};
+
 // u pencil: columns (i, j0..j1-1) of u and rhs_u.
 template <typename T, int R>
 __device__ __forceinline__ void k1_pencil_u(const K1Ctx<T, R>& c, int i, int j0, int j1, const T* __restrict__ u,
@@ -354,80 +418,92 @@ __device__ __forceinline__ void k1_pencil_u(const K1Ctx<T, R>& c, int i, int j0,
   const bool i_in = i >= 1 && i <= d.nx;
   const bool adv = i_in && i <= d.nx - 1;
   const bool lap_ok = adv || (c.cpu && i_in);
+  const bool step = (c.mode & TC_K1_STEP) != 0, ab2 = step && !(c.mode & TC_K1_FIRST);
   T rz0[R], rz1[R], rz2[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    rz0[r] = m < n ? c.mt.rdzc[m] : T(0);
-    rz1[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
-    rz2[r] = m < n ? c.mt.rdzf[m] : T(0);
+  k1_metric<T, R, false>(c, n, rz0, rz1, rz2);
+  const long long oi = (long long)i * d.sxu;
+  const T* ui = u + oi;
+  const T* bi = c.body != nullptr ? c.body + oi : nullptr;
+  const int jlo = max(j0, 1), jhi = min(j1 - 1, d.ny);
+  if (!lap_ok || !i_in || jlo > jhi) {
+    k1_plain_cols<T, R, false>(c, ui, pu + oi, ru + oi, su + oi, bi, d.syu, j0, j1, i_in ? 1 : 1 << 30,
+                               d.ny, n, Kc, rz0, rz1, rz2);
+    return;
   }
-  const T* ui = u + (long long)i * d.sxu;
+  k1_plain_cols<T, R, false>(c, ui, pu + oi, ru + oi, su + oi, bi, d.syu, j0, jlo, 1, 0, n, Kc, rz0, rz1, rz2);
   const T* uw = ui - d.sxu;
   const T* ue = (i == d.nx) ? u + d.sxu : ui + d.sxu;        // periodic wrap of the CPU branch
   const T* vi = v + (long long)i * d.sxv;
   const T* wi = w + (long long)i * d.sxw;
-  T uS[R], uC[R], v0m[R], v1m[R];
-  const int jin0 = j0 > 1 ? j0 : 1;
-  if (lap_ok && jin0 <= d.ny && jin0 < j1) {
-    col_load<R, T>(ui + (long long)(jin0 - 1) * d.syu, uS, n, lane);
-    col_load<R, T>(ui + (long long)jin0 * d.syu, uC, n, lane);
+  auto fetch = [&](int j, UIn<T, R>& L) {
+    const long long cu = (long long)j * d.syu;
+    col_load<R, T>(ui + cu + d.syu, L.uN, n, lane);
+    col_load<R, T>(uw + cu, L.uW, n, lane);
+    col_load<R, T>(ue + cu, L.uE, n, lane);
+    if (ab2) col_load<R, T>(pu + oi + cu, L.rp, n, lane);
     if (adv) {
-      col_load<R, T>(vi + (long long)(jin0 - 1) * d.syv, v0m, n, lane);
-      col_load<R, T>(vi + d.sxv + (long long)(jin0 - 1) * d.syv, v1m, n, lane);
+      const T* w0 = wi + (long long)j * d.syw;
+      col_load<R, T>(vi + (long long)j * d.syv, L.v00, n, lane);
+      col_load<R, T>(vi + d.sxv + (long long)j * d.syv, L.v10, n, lane);
+      col_load<R, T>(w0, L.w00, n, lane);
+      col_load<R, T>(w0 + d.sxw, L.w10, n, lane);
+      L.ub0 = ui[cu];
+      L.ubn = ui[cu + n + 1];
+      L.wb0 = w0[0];
+      L.wb1 = w0[d.sxw];
     }
+  };
+  T uS[R], uC[R], v0m[R], v1m[R];
+  col_load<R, T>(ui + (long long)(jlo - 1) * d.syu, uS, n, lane);
+  col_load<R, T>(ui + (long long)jlo * d.syu, uC, n, lane);
+  if (adv) {
+    col_load<R, T>(vi + (long long)(jlo - 1) * d.syv, v0m, n, lane);
+    col_load<R, T>(vi + d.sxv + (long long)(jlo - 1) * d.syv, v1m, n, lane);
   }
-  for (int j = j0; j < j1; ++j) {
-    const bool inner = i_in && j >= 1 && j <= d.ny;
+  UIn<T, R> A;
+  fetch(jlo, A);
+  for (int j = jlo; j <= jhi; ++j) {
+    UIn<T, R> B;
+    if (j < jhi) fetch(j + 1, B);
     const long long cu = (long long)j * d.syu;
     T rhs[R], q[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) rhs[r] = T(0);
-    if (inner && lap_ok) {
-      T uN[R], uW[R], uE[R];
-      col_load<R, T>(ui + cu + d.syu, uN, n, lane);
-      col_load<R, T>(uw + cu, uW, n, lane);
-      col_load<R, T>(ue + cu, uE, n, lane);
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        rhs[r] = (uE[r] - T(2) * uC[r] + uW[r]) * c.mt.kxx + (uN[r] - T(2) * uC[r] + uS[r]) * c.mt.kyy;
-      if (adv) {
-        const T* w0 = wi + (long long)j * d.syw;
-        const T* w1 = w0 + d.sxw;
-        T v00[R], v10[R], w00[R], w10[R], ub[R], ut[R], w0m[R], w1m[R];
-        col_load<R, T>(vi + (long long)j * d.syv, v00, n, lane);
-        col_load<R, T>(vi + d.sxv + (long long)j * d.syv, v10, n, lane);
-        col_load<R, T>(w0, w00, n, lane);
-        col_load<R, T>(w1, w10, n, lane);
-        const T* uc = ui + cu;
-        rows_pm<R, T>(c.rows, uC, ub, ut, uc[0], uc[n + 1], n, lane);
-        rows_dn<R, T>(c.rows, w00, w0m, w0[0], lane);
-        rows_dn<R, T>(c.rows, w10, w1m, w1[0], lane);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const T ur = h * (uC[r] + uE[r]), ul = h * (uW[r] + uC[r]);
-          const T fx = (ur * ur - ul * ul) * c.mt.rdx;
-          const T fy = (h * (v00[r] + v10[r]) * (h * (uC[r] + uN[r])) - h * (v0m[r] + v1m[r]) * (h * (uS[r] + uC[r]))) * c.mt.rdy;
-          const T fz = (h * (w00[r] + w10[r]) * (h * (uC[r] + ut[r])) - h * (w0m[r] + w1m[r]) * (h * (ub[r] + uC[r]))) * rz2[r];
-          rhs[r] = rhs[r] - (fx + fy + fz);
-          v0m[r] = v00[r];
-          v1m[r] = v10[r];
-        }
-      }
+    for (int r = 0; r < R; ++r)
+      rhs[r] = (A.uE[r] - T(2) * uC[r] + A.uW[r]) * c.mt.kxx + (A.uN[r] - T(2) * uC[r] + uS[r]) * c.mt.kyy;
+    if (adv) {
+      T ub[R], ut[R], w0m[R], w1m[R];
+      rows_pm<R, T>(c.rows, uC, ub, ut, A.ub0, A.ubn, n, lane);
+      rows_dn<R, T>(c.rows, A.w00, w0m, A.wb0, lane);
+      rows_dn<R, T>(c.rows, A.w10, w1m, A.wb1, lane);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        q[r] = uC[r];
-        uS[r] = uC[r];
-        uC[r] = uN[r];
+        const T ur = h * (uC[r] + A.uE[r]), ul = h * (A.uW[r] + uC[r]);
+        const T fx = (ur * ur - ul * ul) * c.mt.rdx;
+        const T fy = (h * (A.v00[r] + A.v10[r]) * (h * (uC[r] + A.uN[r])) - h * (v0m[r] + v1m[r]) * (h * (uS[r] + uC[r]))) * c.mt.rdy;
+        const T fz = (h * (A.w00[r] + A.w10[r]) * (h * (uC[r] + ut[r])) - h * (w0m[r] + w1m[r]) * (h * (ub[r] + uC[r]))) * rz2[r];
+        rhs[r] = rhs[r] - (fx + fy + fz);
+        v0m[r] = A.v00[r];
+        v1m[r] = A.v10[r];
       }
-    } else if (inner && (c.mode & TC_K1_STEP)) {
-      col_load<R, T>(ui + cu, q, n, lane);
     }
-    const T* bcol = c.body != nullptr ? c.body + (long long)i * d.sxu + cu : nullptr;
-    k1_finish<T, R, false>(c, rhs, q, pu ? pu + (long long)i * d.sxu + cu : nullptr, ru + (long long)i * d.sxu + cu,
-                           su ? su + (long long)i * d.sxu + cu : nullptr, bcol, n, Kc, inner, rz0, rz1, rz2, T(0));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      q[r] = uC[r];
+      uS[r] = uC[r];
+      uC[r] = A.uN[r];
+    }
+    k1_finish<T, R, false>(c, rhs, q, A.rp, ru + oi + cu, su + oi + cu, bi != nullptr ? bi + cu : nullptr, n, Kc,
+                           true, rz0, rz1, rz2, T(0));
+    A = B;
   }
+  k1_plain_cols<T, R, false>(c, ui, pu + oi, ru + oi, su + oi, bi, d.syu, jhi + 1, j1, 1, 0, n, Kc, rz0, rz1, rz2);
 }
+
+template <typename T, int R>
+struct VIn {
+  T vN[R], vW[R], vE[R], u01[R], um1[R], w01[R], rp[R];
+  T vb0, vbn, wb1;          // v(i,j) wall-normal ghosts, w(i,j+1)[0]
+};
 
 // v pencil: columns (i, j0..j1-1) of v and rhs_v.
 template <typename T, int R>
@@ -438,80 +514,94 @@ __device__ __forceinline__ void k1_pencil_v(const K1Ctx<T, R>& c, int i, int j0,
   const int lane = c.lane, n = d.nz, Kc = d.nz + 2;
   const T h = T(0.5);
   const bool i_in = i >= 1 && i <= d.nx;
+  const bool step = (c.mode & TC_K1_STEP) != 0, ab2 = step && !(c.mode & TC_K1_FIRST);
   T rz0[R], rz1[R], rz2[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    rz0[r] = m < n ? c.mt.rdzc[m] : T(0);
-    rz1[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
-    rz2[r] = m < n ? c.mt.rdzf[m] : T(0);
+  k1_metric<T, R, false>(c, n, rz0, rz1, rz2);
+  const long long oi = (long long)i * d.sxv;
+  const T* vi = v + oi;
+  // stencil columns: j = 1..ny-1 (both branches), plus j = ny in the CPU branch
+  const int jst = c.cpu ? d.ny : d.ny - 1;
+  const int jlo = max(j0, 1), jhi = min(j1 - 1, jst);
+  if (!i_in || jlo > jhi) {
+    k1_plain_cols<T, R, false>(c, vi, pv + oi, rv + oi, sv + oi, nullptr, d.syv, j0, j1, i_in ? 1 : 1 << 30,
+                               d.ny, n, Kc, rz0, rz1, rz2);
+    return;
   }
-  const T* vi = v + (long long)i * d.sxv;
+  k1_plain_cols<T, R, false>(c, vi, pv + oi, rv + oi, sv + oi, nullptr, d.syv, j0, jlo, 1, 0, n, Kc, rz0, rz1, rz2);
   const T* ui = u + (long long)i * d.sxu;
   const T* wi = w + (long long)i * d.sxw;
+  auto fetch = [&](int j, VIn<T, R>& L) {
+    const long long cv = (long long)j * d.syv;
+    const T* vn = (j == d.ny) ? vi + d.syv : vi + cv + d.syv;    // periodic wrap of the CPU branch
+    col_load<R, T>(vn, L.vN, n, lane);
+    col_load<R, T>(vi - d.sxv + cv, L.vW, n, lane);
+    col_load<R, T>(vi + d.sxv + cv, L.vE, n, lane);
+    if (ab2) col_load<R, T>(pv + oi + cv, L.rp, n, lane);
+    if (j < d.ny) {
+      const T* w1 = wi + (long long)(j + 1) * d.syw;
+      col_load<R, T>(ui + (long long)(j + 1) * d.syu, L.u01, n, lane);
+      col_load<R, T>(ui - d.sxu + (long long)(j + 1) * d.syu, L.um1, n, lane);
+      col_load<R, T>(w1, L.w01, n, lane);
+      L.vb0 = vi[cv];
+      L.vbn = vi[cv + n + 1];
+      L.wb1 = w1[0];
+    }
+  };
   T vS[R], vC[R], u00[R], um0[R], w00[R], w0m[R];
-  const int jin0 = j0 > 1 ? j0 : 1;
-  if (i_in && jin0 <= d.ny && jin0 < j1) {
-    col_load<R, T>(vi + (long long)(jin0 - 1) * d.syv, vS, n, lane);
-    col_load<R, T>(vi + (long long)jin0 * d.syv, vC, n, lane);
-    col_load<R, T>(ui + (long long)jin0 * d.syu, u00, n, lane);
-    col_load<R, T>(ui - d.sxu + (long long)jin0 * d.syu, um0, n, lane);
-    const T* w0 = wi + (long long)jin0 * d.syw;
+  col_load<R, T>(vi + (long long)(jlo - 1) * d.syv, vS, n, lane);
+  col_load<R, T>(vi + (long long)jlo * d.syv, vC, n, lane);
+  col_load<R, T>(ui + (long long)jlo * d.syu, u00, n, lane);
+  col_load<R, T>(ui - d.sxu + (long long)jlo * d.syu, um0, n, lane);
+  {
+    const T* w0 = wi + (long long)jlo * d.syw;
     col_load<R, T>(w0, w00, n, lane);
     rows_dn<R, T>(c.rows, w00, w0m, w0[0], lane);
   }
-  for (int j = j0; j < j1; ++j) {
-    const bool inner = i_in && j >= 1 && j <= d.ny;
-    const bool adv = inner && j <= d.ny - 1;
-    const bool lap_ok = adv || (c.cpu && inner);
+  VIn<T, R> A;
+  fetch(jlo, A);
+  for (int j = jlo; j <= jhi; ++j) {
+    VIn<T, R> B;
+    if (j < jhi) fetch(j + 1, B);
     const long long cv = (long long)j * d.syv;
     T rhs[R], q[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) rhs[r] = T(0);
-    if (lap_ok) {
-      T vN[R], vW[R], vE[R];
-      const T* vn = (j == d.ny) ? vi + d.syv : vi + cv + d.syv;    // periodic wrap of the CPU branch
-      col_load<R, T>(vn, vN, n, lane);
-      col_load<R, T>(vi - d.sxv + cv, vW, n, lane);
-      col_load<R, T>(vi + d.sxv + cv, vE, n, lane);
-#pragma unroll
-      for (int r = 0; r < R; ++r)
-        rhs[r] = (vE[r] - T(2) * vC[r] + vW[r]) * c.mt.kxx + (vN[r] - T(2) * vC[r] + vS[r]) * c.mt.kyy;
-      if (adv) {
-        const T* w1 = wi + (long long)(j + 1) * d.syw;
-        T u01[R], um1[R], w01[R], w1m[R], vb[R], vt[R];
-        col_load<R, T>(ui + (long long)(j + 1) * d.syu, u01, n, lane);
-        col_load<R, T>(ui - d.sxu + (long long)(j + 1) * d.syu, um1, n, lane);
-        col_load<R, T>(w1, w01, n, lane);
-        const T* vc = vi + cv;
-        rows_pm<R, T>(c.rows, vC, vb, vt, vc[0], vc[n + 1], n, lane);
-        rows_dn<R, T>(c.rows, w01, w1m, w1[0], lane);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-          const T fx = (h * (u00[r] + u01[r]) * (h * (vC[r] + vE[r])) - h * (um0[r] + um1[r]) * (h * (vW[r] + vC[r]))) * c.mt.rdx;
-          const T vr = h * (vC[r] + vN[r]), vl = h * (vS[r] + vC[r]);
-          const T fy = (vr * vr - vl * vl) * c.mt.rdy;
-          const T fz = (h * (w00[r] + w01[r]) * (h * (vC[r] + vt[r])) - h * (w0m[r] + w1m[r]) * (h * (vb[r] + vC[r]))) * rz2[r];
-          rhs[r] = rhs[r] - (fx + fy + fz);
-          u00[r] = u01[r];
-          um0[r] = um1[r];
-          w00[r] = w01[r];
-          w0m[r] = w1m[r];
-        }
-      }
+    for (int r = 0; r < R; ++r)
+      rhs[r] = (A.vE[r] - T(2) * vC[r] + A.vW[r]) * c.mt.kxx + (A.vN[r] - T(2) * vC[r] + vS[r]) * c.mt.kyy;
+    if (j < d.ny) {
+      T vb[R], vt[R], w1m[R];
+      rows_pm<R, T>(c.rows, vC, vb, vt, A.vb0, A.vbn, n, lane);
+      rows_dn<R, T>(c.rows, A.w01, w1m, A.wb1, lane);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        q[r] = vC[r];
-        vS[r] = vC[r];
-        vC[r] = vN[r];
+        const T fx = (h * (u00[r] + A.u01[r]) * (h * (vC[r] + A.vE[r])) - h * (um0[r] + A.um1[r]) * (h * (A.vW[r] + vC[r]))) * c.mt.rdx;
+        const T vr = h * (vC[r] + A.vN[r]), vl = h * (vS[r] + vC[r]);
+        const T fy = (vr * vr - vl * vl) * c.mt.rdy;
+        const T fz = (h * (w00[r] + A.w01[r]) * (h * (vC[r] + vt[r])) - h * (w0m[r] + w1m[r]) * (h * (vb[r] + vC[r]))) * rz2[r];
+        rhs[r] = rhs[r] - (fx + fy + fz);
+        u00[r] = A.u01[r];
+        um0[r] = A.um1[r];
+        w00[r] = A.w01[r];
+        w0m[r] = w1m[r];
       }
-    } else if (inner && (c.mode & TC_K1_STEP)) {
-      col_load<R, T>(vi + cv, q, n, lane);
     }
-    k1_finish<T, R, false>(c, rhs, q, pv ? pv + (long long)i * d.sxv + cv : nullptr, rv + (long long)i * d.sxv + cv,
-                           sv ? sv + (long long)i * d.sxv + cv : nullptr, nullptr, n, Kc, inner, rz0, rz1, rz2, T(0));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      q[r] = vC[r];
+      vS[r] = vC[r];
+      vC[r] = A.vN[r];
+    }
+    k1_finish<T, R, false>(c, rhs, q, A.rp, rv + oi + cv, sv + oi + cv, nullptr, n, Kc, true, rz0, rz1, rz2, T(0));
+    A = B;
   }
+  k1_plain_cols<T, R, false>(c, vi, pv + oi, rv + oi, sv + oi, nullptr, d.syv, jhi + 1, j1, 1, d.ny, n, Kc, rz0,
+                             rz1, rz2);
 }
+
+template <typename T, int R>
+struct WIn {
+  T wN[R], wW[R], wE[R], u00[R], um0[R], v00[R], rp[R];
+  T wb0, wbn, ubn, umbn, vbn;   // w(i,j) walls, top rows of u(i,j), u(i-1,j), v(i,j)
+};
 
 // w pencil: columns (i, j0..j1-1) of w and rhs_w (rows k = 1..nz-1).
 template <typename T, int R>
@@ -522,66 +612,74 @@ __device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0,
   const int lane = c.lane, n = d.nz - 1, Kc = d.nz + 1;
   const T h = T(0.5);
   const bool i_in = i >= 1 && i <= d.nx;
+  const bool step = (c.mode & TC_K1_STEP) != 0, ab2 = step && !(c.mode & TC_K1_FIRST);
   T rz0[R], rz1[R], rz2[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) {
-    const int m = r * TC_WAVE + lane;
-    rz0[r] = m < n ? c.mt.rdzf[m] : T(0);
-    rz1[r] = m < n ? c.mt.rdzf[m + 1] : T(0);
-    rz2[r] = m < n ? c.mt.rdzc[m + 1] : T(0);
+  k1_metric<T, R, true>(c, n, rz0, rz1, rz2);
+  const long long oi = (long long)i * d.sxw;
+  const T* wi = w + oi;
+  const int jlo = max(j0, 1), jhi = min(j1 - 1, d.ny);
+  if (!i_in || jlo > jhi) {
+    k1_plain_cols<T, R, true>(c, wi, pw + oi, rw + oi, sw + oi, nullptr, d.syw, j0, j1, 1 << 30, 0, n, Kc, rz0,
+                              rz1, rz2);
+    return;
   }
-  const T* wi = w + (long long)i * d.sxw;
+  k1_plain_cols<T, R, true>(c, wi, pw + oi, rw + oi, sw + oi, nullptr, d.syw, j0, jlo, 1, 0, n, Kc, rz0, rz1, rz2);
   const T* ui = u + (long long)i * d.sxu;
   const T* vi = v + (long long)i * d.sxv;
+  auto fetch = [&](int j, WIn<T, R>& L) {
+    const long long cw = (long long)j * d.syw;
+    const T* u0 = ui + (long long)j * d.syu;
+    const T* v0 = vi + (long long)j * d.syv;
+    col_load<R, T>(wi + cw + d.syw, L.wN, n, lane);
+    col_load<R, T>(wi - d.sxw + cw, L.wW, n, lane);
+    col_load<R, T>(wi + d.sxw + cw, L.wE, n, lane);
+    col_load<R, T>(u0, L.u00, n, lane);
+    col_load<R, T>(u0 - d.sxu, L.um0, n, lane);
+    col_load<R, T>(v0, L.v00, n, lane);
+    if (ab2) col_load<R, T>(pw + oi + cw, L.rp, n, lane);
+    L.wb0 = wi[cw];
+    L.wbn = wi[cw + n + 1];
+    L.ubn = u0[n + 1];
+    L.umbn = u0[n + 1 - d.sxu];
+    L.vbn = v0[n + 1];
+  };
   T wS[R], wC[R], vm0[R], vm1[R];
-  const int jin0 = j0 > 1 ? j0 : 1;
-  if (i_in && jin0 <= d.ny && jin0 < j1) {
-    col_load<R, T>(wi + (long long)(jin0 - 1) * d.syw, wS, n, lane);
-    col_load<R, T>(wi + (long long)jin0 * d.syw, wC, n, lane);
-    const T* vp = vi + (long long)(jin0 - 1) * d.syv;
+  col_load<R, T>(wi + (long long)(jlo - 1) * d.syw, wS, n, lane);
+  col_load<R, T>(wi + (long long)jlo * d.syw, wC, n, lane);
+  {
+    const T* vp = vi + (long long)(jlo - 1) * d.syv;
     col_load<R, T>(vp, vm0, n, lane);
     rows_up<R, T>(c.rows, vm0, vm1, vp[n + 1], n, lane);
   }
-  for (int j = j0; j < j1; ++j) {
-    const bool inner = i_in && j >= 1 && j <= d.ny;
+  WIn<T, R> A;
+  fetch(jlo, A);
+  for (int j = jlo; j <= jhi; ++j) {
+    WIn<T, R> B;
+    if (j < jhi) fetch(j + 1, B);
     const long long cw = (long long)j * d.syw;
-    T rhs[R], q[R];
+    T rhs[R], q[R], wb[R], wt[R], u01[R], um1[R], v01[R];
+    rows_pm<R, T>(c.rows, wC, wb, wt, A.wb0, A.wbn, n, lane);
+    rows_up<R, T>(c.rows, A.u00, u01, A.ubn, n, lane);
+    rows_up<R, T>(c.rows, A.um0, um1, A.umbn, n, lane);
+    rows_up<R, T>(c.rows, A.v00, v01, A.vbn, n, lane);
 #pragma unroll
-    for (int r = 0; r < R; ++r) rhs[r] = T(0);
-    if (inner) {
-      T wN[R], wW[R], wE[R], u00[R], um0[R], v00[R], u01[R], um1[R], v01[R], wb[R], wt[R];
-      const T* u0 = ui + (long long)j * d.syu;
-      const T* um = u0 - d.sxu;
-      const T* v0 = vi + (long long)j * d.syv;
-      col_load<R, T>(wi + cw + d.syw, wN, n, lane);
-      col_load<R, T>(wi - d.sxw + cw, wW, n, lane);
-      col_load<R, T>(wi + d.sxw + cw, wE, n, lane);
-      col_load<R, T>(u0, u00, n, lane);
-      col_load<R, T>(um, um0, n, lane);
-      col_load<R, T>(v0, v00, n, lane);
-      const T* wc = wi + cw;
-      rows_pm<R, T>(c.rows, wC, wb, wt, wc[0], wc[n + 1], n, lane);
-      rows_up<R, T>(c.rows, u00, u01, u0[n + 1], n, lane);
-      rows_up<R, T>(c.rows, um0, um1, um[n + 1], n, lane);
-      rows_up<R, T>(c.rows, v00, v01, v0[n + 1], n, lane);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const T lap = (wE[r] - T(2) * wC[r] + wW[r]) * c.mt.kxx + (wN[r] - T(2) * wC[r] + wS[r]) * c.mt.kyy;
-        const T fx = (h * (u00[r] + u01[r]) * (h * (wC[r] + wE[r])) - h * (um0[r] + um1[r]) * (h * (wW[r] + wC[r]))) * c.mt.rdx;
-        const T fy = (h * (v00[r] + v01[r]) * (h * (wC[r] + wN[r])) - h * (vm0[r] + vm1[r]) * (h * (wS[r] + wC[r]))) * c.mt.rdy;
-        const T wr = h * (wC[r] + wt[r]), wl = h * (wb[r] + wC[r]);
-        const T fz = (wr * wr - wl * wl) * rz2[r];
-        rhs[r] = lap - (fx + fy + fz);
-        vm0[r] = v00[r];
-        vm1[r] = v01[r];
-        q[r] = wC[r];
-        wS[r] = wC[r];
-        wC[r] = wN[r];
-      }
+    for (int r = 0; r < R; ++r) {
+      const T lap = (A.wE[r] - T(2) * wC[r] + A.wW[r]) * c.mt.kxx + (A.wN[r] - T(2) * wC[r] + wS[r]) * c.mt.kyy;
+      const T fx = (h * (A.u00[r] + u01[r]) * (h * (wC[r] + A.wE[r])) - h * (A.um0[r] + um1[r]) * (h * (A.wW[r] + wC[r]))) * c.mt.rdx;
+      const T fy = (h * (A.v00[r] + v01[r]) * (h * (wC[r] + A.wN[r])) - h * (vm0[r] + vm1[r]) * (h * (wS[r] + wC[r]))) * c.mt.rdy;
+      const T wr = h * (wC[r] + wt[r]), wl = h * (wb[r] + wC[r]);
+      const T fz = (wr * wr - wl * wl) * rz2[r];
+      rhs[r] = lap - (fx + fy + fz);
+      vm0[r] = A.v00[r];
+      vm1[r] = v01[r];
+      q[r] = wC[r];
+      wS[r] = wC[r];
+      wC[r] = A.wN[r];
     }
-    k1_finish<T, R, true>(c, rhs, q, pw ? pw + (long long)i * d.sxw + cw : nullptr, rw + (long long)i * d.sxw + cw,
-                          sw ? sw + (long long)i * d.sxw + cw : nullptr, nullptr, n, Kc, inner, rz0, rz1, rz2, T(0));
+    k1_finish<T, R, true>(c, rhs, q, A.rp, rw + oi + cw, sw + oi + cw, nullptr, n, Kc, true, rz0, rz1, rz2, T(0));
+    A = B;
   }
+  k1_plain_cols<T, R, true>(c, wi, pw + oi, rw + oi, sw + oi, nullptr, d.syw, jhi + 1, j1, 1, 0, n, Kc, rz0, rz1, rz2);
 }
 
 // Pencil bands: a band is TC_K1_WAVES consecutive i (one per wavefront) by jb consecutive j, on
