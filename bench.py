@@ -22,8 +22,12 @@ sys.path.insert(0, ROOT)
 
 NX, NY, NZ = 192, 192, 128
 HBM_PEAK_GBS = 8000.0
-PHASES = ["cn_tables", "explicit_ab2_cn", "div_fft_x", "fft_y_fwd", "poisson_pcr", "fft_y_inv",
+PHASES = ["cn_tables", "explicit_ab2_cn", "div_fft_x", "fft_y_fwd", "poisson_thomas", "fft_y_inv",
           "ifft_x_p", "project_bc_bulk", "cfl_max"]
+KERNEL_OF = {"explicit_ab2_cn": "k_explicit_cn", "div_fft_x": "k_fwd_x", "fft_y_fwd": "k_fft_y",
+             "poisson_thomas": "k_poisson_thomas", "fft_y_inv": "k_fft_y", "ifft_x_p": "k_inv_x",
+             "project_bc_bulk": "k_project_bc", "cfl_max": "k_cfl"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "imex_step_192x192x128_hbm_traffic.json")
 
 
 def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
@@ -47,7 +51,7 @@ def algorithmic_words(nx, ny, nz):
         "explicit_ab2_cn": 12.0,              # read u, v, w, rhs_prev x3; write rhs x3, q* x3
         "div_fft_x": 3.0 + spec,              # read q* x3; write x half spectra
         "fft_y_fwd": 2 * spec,
-        "poisson_pcr": 2 * spec,
+        "poisson_thomas": 2 * spec,
         "fft_y_inv": 2 * spec,
         "ifft_x_p": spec + 1.0,               # read spectra; write p
         "project_bc_bulk": 7.0,               # read q* x3, p; write u, v, w
@@ -89,11 +93,20 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
     wall = time.perf_counter() - wall0
     ms = t0.elapsed_time(t1)
     n_cells = NX * NY * NZ
+    # fixed-dt kernel throughput: the same number of steps with no CFL evaluation in the chunks
+    ch.run_graph(0, chunk, dt_update_interval=0)
+    torch.cuda.synchronize()
+    t0.record()
+    ch.run_graph(n_chunks, chunk, dt_update_interval=0)
+    t1.record()
+    torch.cuda.synchronize()
+    ms_fixed = t0.elapsed_time(t1)
     out = {
         "mcell_steps_per_s": n_cells * done / (ms * 1e-3) / 1e6,
         "ms_per_step": ms / done,
         "wall_s": wall,
         "steps": done,
+        "fixed_dt_ms_per_step": ms_fixed / done,
         "dt_final": ch.device_dt(),
         "u_bulk": float(ch._scal[L.S_UBULK]),
         "stats_samples": ch.turbulence_stats.n_samples,
@@ -190,6 +203,12 @@ def main():
         print(f"[bench] cpu oracle: {cpu['value']:.2f} Mcell-steps/s", file=sys.stderr, flush=True)
     dom = main_run["dominant"]
     k = main_run["kernels"][dom]
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as fh:
+            t = json.load(fh).get(args.precision, {}).get(KERNEL_OF[dom])
+        if t is not None:
+            traffic = round(t["read_bytes"] + t["write_bytes"])
     line = {
         "metric": "Mcell-steps/s at 192x192x128 (fp32 vs fp64 plain); adjoint/forward step cost",
         "value": round(main_run["mcell_steps_per_s"], 2),
@@ -208,8 +227,12 @@ def main():
                                "turbstats every 200 steps",
                    "precision": args.precision, "graph_chunk_steps": 10},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-                     "avg_us": round(k["avg_us"], 2)},
+                     "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "alg_bytes": k["alg_bytes"], "avg_us": round(k["avg_us"], 2),
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "
+                                       "profiles/imex_step_192x192x128_hbm_traffic.md"},
+        "fixed_dt": {"value": round(NX * NY * NZ / (main_run["fixed_dt_ms_per_step"] * 1e-3) / 1e6, 2),
+                     "ms_per_step": round(main_run["fixed_dt_ms_per_step"], 5)},
         "step_roofline": {"alg_bytes_per_step": main_run["step_alg_bytes"],
                           "achieved_gbs": round(main_run["step_gbs"], 1),
                           "frac": round(main_run["step_gbs"] / HBM_PEAK_GBS, 4)},
