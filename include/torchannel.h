@@ -169,9 +169,32 @@ int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_targe
 int tc_solver_replay(tc_solver* s, void* stream);
 /* Launch one kernel of the step `reps` times back to back, for per-kernel timing (no reference
  * counterpart): 0 CN tables, 1 explicit+AB2+CN, 2 forward x FFT (with divergence), 3 forward y
- * FFT, 4 per-mode PCR, 5 inverse y FFT, 6 inverse x FFT (+ p ghosts), 7 projection + boundary +
+ * FFT, 4 per-mode tridiagonal solve, 5 inverse y FFT, 6 inverse x FFT (+ p ghosts), 7 projection + boundary +
  * bulk, 8 CFL reduction.  Overwrites the step's intermediates; needs a step taken first. */
 int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream);
+/* Set which rhs set holds the newest rhs (0 or 1), or -1 so the next step is a forward-Euler
+ * start: restoring a checkpointed state (rhs arrays included) into the solver's buffers.  The
+ * reference keeps this as the rhs_prev / rhs_curr swap of solver.py:649-651. */
+int tc_solver_set_newest(tc_solver* s, int newest);
+
+/* ---- discrete adjoint --------------------------------------------------------------------
+ * No counterpart in the reference.  Gradient of J = sum_n weight_n * tau_w(u_n) with respect to
+ * a body-force field f (shaped like u) added to rhs_u next to the forcing (solver.py:626), for a
+ * float64 GPU-branch solver stepped at fixed dt with tc_solver_step(.., body = f, ..).
+ * tau_w(u) = nu/z_c[1] * (mean u[:,:,1] + mean u[:,:,nz]) / 2 over whole planes, ghosts
+ * included (the linear core of compute_u_tau, utils.py:187-204); the free-slip top keeps the
+ * bottom term only, nu/z_c[1] * mean u[:,:,1].
+ * Reverse sweep: tc_adjoint_begin(dt, weight of u_N, grad) then, for n = N-1 .. 0,
+ * tc_adjoint_step(q_n, first = (n == 0), weight of u_n, grad) with q_n the full state arrays at
+ * the start of forward step n.  grad (shaped like u) receives dJ/df. */
+typedef struct tc_adjoint tc_adjoint;
+int tc_adjoint_create(tc_solver* s, double wall_dist, tc_adjoint** out);
+int tc_adjoint_destroy(tc_adjoint* a);
+/* j_dev[0] += weight * tau_w(u) (device float64 scalar). */
+int tc_adjoint_objective(tc_adjoint* a, const void* u, double weight, double* j_dev, void* stream);
+int tc_adjoint_begin(tc_adjoint* a, double dt, double obj_weight, void* grad, void* stream);
+int tc_adjoint_step(tc_adjoint* a, const void* u, const void* v, const void* w, int first, double obj_weight,
+                    void* grad, void* stream);
 
 #ifdef __cplusplus
 }

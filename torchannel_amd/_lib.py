@@ -32,6 +32,12 @@ SIGNATURES = {
     "tc_grid_create": [_i, _i, _i, _d, _d, _d, _i, _i, _pd, _pd, ctypes.POINTER(_vp)],
     "tc_grid_destroy": [_vp],
     "tc_grid_set_tuning": [_vp, _i, _i],
+    "tc_solver_set_newest": [_vp, _i],
+    "tc_adjoint_create": [_vp, _d, ctypes.POINTER(_vp)],
+    "tc_adjoint_destroy": [_vp],
+    "tc_adjoint_objective": [_vp, _vp, _d, _vp, _vp],
+    "tc_adjoint_begin": [_vp, _d, _d, _vp, _vp],
+    "tc_adjoint_step": [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp],
     "tc_grid_get_tuning": [_vp, _i, ctypes.POINTER(_i)],
     "tc_apply_bc": [_vp, _vp, _vp, _vp, _vp],
     "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -491,4 +491,124 @@ int tc_solver_replay(tc_solver* s, void* stream) {
   return TC_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Discrete adjoint of the fixed-dt step (float64, GPU-branch explicit terms)
+// ---------------------------------------------------------------------------------------------
+struct tc_adjoint {
+  tc_solver* s = nullptr;
+  double tau_coef = 0.0;     // tau_w = tau_coef * (mean u[:,:,1] [+ mean u[:,:,nz]])
+  int obj_top = 1;           // the top wall enters the objective (no-slip top)
+  double dt = 0.0;
+  void* lq[3] = {nullptr, nullptr, nullptr};    // cotangent of the state q
+  void* qs[3] = {nullptr, nullptr, nullptr};    // cotangent of the pre-CN velocities q*
+  void* lr[3] = {nullptr, nullptr, nullptr};    // cotangent of R_n
+  void* acc[3] = {nullptr, nullptr, nullptr};   // carried cotangent of R_{n-1}
+  void* e[3] = {nullptr, nullptr, nullptr};     // E'^T lR on full arrays
+  void* z[3] = {nullptr, nullptr, nullptr};     // P W^{-1} lq
+  double* ascal = nullptr;
+  double* one_scal = nullptr;                   // dt = 1 block for the projection
+  double* partials = nullptr;
+  double* tpart = nullptr;
+};
+
+int tc_adjoint_create(tc_solver* s, double wall_dist, tc_adjoint** out) {
+  if (!s || !out || !(wall_dist > 0.0)) return TC_ERR_ARG;
+  const Grid* g = s->g;
+  if (g->dtype != TC_F64) return TC_ERR_DTYPE;
+  if (s->branch != TC_BRANCH_GPU) return TC_ERR_UNSUPPORTED;
+  tc_adjoint* a = new (std::nothrow) tc_adjoint();
+  if (!a) return TC_ERR_NOMEM;
+  a->s = s;
+  const Dims& d = g->d;
+  a->obj_top = d.top < 0 ? 1 : 0;
+  a->tau_coef = (a->obj_top ? 0.5 : 1.0) * g->nu / wall_dist;
+  const size_t n3[3] = {(size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2), (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2),
+                        (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1)};
+  bool ok = true;
+  for (int c = 0; c < 3 && ok; ++c) {
+    void** bufs[6] = {&a->lq[c], &a->qs[c], &a->lr[c], &a->acc[c], &a->e[c], &a->z[c]};
+    for (void** b : bufs) ok = ok && hipMalloc(b, 8 * n3[c]) == hipSuccess && hipMemset(*b, 0, 8 * n3[c]) == hipSuccess;
+  }
+  ok = ok && hipMalloc(&a->ascal, sizeof(double) * TC_A_NSCAL) == hipSuccess &&
+       hipMalloc(&a->one_scal, sizeof(double) * TC_NSCAL) == hipSuccess &&
+       hipMalloc(&a->partials, sizeof(double) * adj_cn_blocks(g)) == hipSuccess &&
+       hipMalloc(&a->tpart, sizeof(double) * adj_tauw_blocks(g)) == hipSuccess;
+  if (ok) {
+    double one[TC_NSCAL] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    ok = hipMemcpy(a->one_scal, one, sizeof(one), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(a->ascal, 0, sizeof(double) * TC_A_NSCAL) == hipSuccess;
+  }
+  if (!ok) {
+    tc_adjoint_destroy(a);
+    return TC_ERR_NOMEM;
+  }
+  *out = a;
+  return TC_OK;
+}
+
+int tc_adjoint_destroy(tc_adjoint* a) {
+  if (!a) return TC_OK;
+  for (int c = 0; c < 3; ++c) {
+    hipFree(a->lq[c]); hipFree(a->qs[c]); hipFree(a->lr[c]); hipFree(a->acc[c]); hipFree(a->e[c]); hipFree(a->z[c]);
+  }
+  hipFree(a->ascal); hipFree(a->one_scal); hipFree(a->partials); hipFree(a->tpart);
+  delete a;
+  return TC_OK;
+}
+
+int tc_adjoint_objective(tc_adjoint* a, const void* u, double weight, double* j_dev, void* stream) {
+  if (!a || !u || !j_dev) return TC_ERR_ARG;
+  const Grid* g = a->s->g;
+  const double scale = weight * a->tau_coef / ((double)(g->d.nx + 1) * (g->d.ny + 2));
+  return adj_launch_tauw(g, u, a->obj_top, scale, a->tpart, j_dev, (hipStream_t)stream);
+}
+
+int tc_adjoint_begin(tc_adjoint* a, double dt, double obj_weight, void* grad, void* stream) {
+  if (!a || !grad || !(dt > 0.0)) return TC_ERR_ARG;
+  const Grid* g = a->s->g;
+  const Dims& d = g->d;
+  hipStream_t st = (hipStream_t)stream;
+  a->dt = dt;
+  const size_t n3[3] = {(size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2), (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2),
+                        (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1)};
+  for (int c = 0; c < 3; ++c) {
+    TC_HIP(hipMemsetAsync(a->acc[c], 0, 8 * n3[c], st));
+    TC_HIP(hipMemsetAsync(a->qs[c], 0, 8 * n3[c], st));
+    TC_HIP(hipMemsetAsync(a->e[c], 0, 8 * n3[c], st));
+  }
+  TC_HIP(hipMemsetAsync(grad, 0, 8 * n3[0], st));
+  TC_HIP(hipMemsetAsync(a->ascal, 0, sizeof(double) * TC_A_NSCAL, st));
+  const double coef = obj_weight * a->tau_coef / ((double)(d.nx + 1) * (d.ny + 2));
+  return adj_launch_fold(g, a->qs, a->e, coef, a->obj_top, a->lq, st);
+}
+
+int tc_adjoint_step(tc_adjoint* a, const void* u, const void* v, const void* w, int first, double obj_weight,
+                    void* grad, void* stream) {
+  if (!a || !u || !v || !w || !grad || !(a->dt > 0.0)) return TC_ERR_ARG;
+  tc_solver* s = a->s;
+  const Grid* g = s->g;
+  const Dims& d = g->d;
+  hipStream_t st = (hipStream_t)stream;
+  const double dt = a->dt;
+  void* ys[3] = {s->su, s->sv, s->sw};
+  TC_RC(launch_tables(g, s->scal, dt, 0.5, s->tabs, s->tabs + TC_TAB_STRIDE, st));
+  TC_RC(adj_launch_pre(g, 1.0 / dt, s->inv_volume, a->ascal, a->lq, ys, st));
+  TC_RC(poisson_solve(s->pl, true, s->su, s->sv, s->sw, nullptr, a->one_scal, s->p, st));
+  TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, a->z[0], a->z[1], a->z[2], a->one_scal, s->partials,
+                  s->counters, s->inv_volume, 0, st));
+  const double c1 = first ? 1.0 : 1.5, c0 = first ? 0.0 : 0.5;
+  TC_RC(adj_launch_cn(g, a->z, s->tabs, s->tabs + TC_TAB_STRIDE, dt, c1, c0, a->qs, a->lr, a->acc, grad,
+                      a->partials, st));
+  TC_RC(adj_launch_lamf(a->partials, adj_cn_blocks(g), a->ascal, st));
+  TC_RC(adj_launch_rhs(g, u, v, w, a->lr, a->e, st));
+  const double coef = obj_weight * a->tau_coef / ((double)(d.nx + 1) * (d.ny + 2));
+  return adj_launch_fold(g, a->qs, a->e, coef, a->obj_top, a->lq, st);
+}
+
+int tc_solver_set_newest(tc_solver* s, int newest) {
+  if (!s || newest < -1 || newest > 1) return TC_ERR_ARG;
+  s->newest = newest;
+  return TC_OK;
+}
+
 }  // extern "C"

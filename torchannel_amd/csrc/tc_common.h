@@ -212,6 +212,63 @@ __device__ __forceinline__ void pcr_shared_solve_lds(T (&dv)[R], T* buf, const d
   }
 }
 
+// Right-hand side d = q + beta*D2(q) of the CN system for one u/v column (operators.py:627-644):
+// rows m = 0..n-1 hold cell k = m+1; the wall closures are the interior stencil with the
+// reflected neighbours X[-1] = -X[0] and X[n] = top*X[n-1].
+template <int R, typename T>
+__device__ __forceinline__ void cn_rhs_cell(T (&x)[R], T* rows, const Metric<T>& mt, int n, int top, T beta,
+                                            int lane) {
+  T dn[R], up[R];
+  rows_put<R, T>(rows, x, lane);
+  rows_get<-1, R, T>(rows, dn, lane);
+  rows_get<1, R, T>(rows, up, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m == 0) dn[r] = -x[r];
+    if (m == n - 1) up[r] = top > 0 ? x[r] : -x[r];
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m < n) {
+      const T lap = ((up[r] - x[r]) * mt.rdzc[m + 1] - (x[r] - dn[r]) * mt.rdzc[m]) * mt.rdzf[m];
+      x[r] = x[r] + beta * lap;
+    } else {
+      x[r] = T(0);
+    }
+  }
+}
+
+// Same for a w column (operators.py:788-803): row m holds face m+1 but its explicit half is
+// centred on face m, with the wall value W[0] (`wall`) below row 0, zero below that, and the
+// top row's upper difference taken against zero.
+template <int R, typename T>
+__device__ __forceinline__ void cn_rhs_face(T (&x)[R], T* rows, const Metric<T>& mt, int n, T wall, T beta,
+                                            int lane) {
+  T d1[R], d2[R];
+  rows_put<R, T>(rows, x, lane);
+  rows_get<-1, R, T>(rows, d1, lane);
+  rows_get<-2, R, T>(rows, d2, lane);
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m == 0) d1[r] = wall;
+    if (m == 1) d2[r] = wall;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int m = r * TC_WAVE + lane;
+    if (m < n) {
+      const T xp = (m == n - 1) ? T(0) : x[r];
+      const T lap = ((xp - d1[r]) * mt.rdzf[m + 1] - (d1[r] - d2[r]) * mt.rdzf[m]) * mt.rdzc[m + 1];
+      x[r] = x[r] + beta * lap;
+    } else {
+      x[r] = T(0);
+    }
+  }
+}
+
 // Parallel cyclic reduction of one column whose matrix is shared by every column (the CN
 // systems), so the whole a/b/c reduction is a precomputed table: per stage st, alpha[m] at
 // tab[st*2*MAXNZ + m], gamma[m] at tab[(st*2+1)*MAXNZ + m]; final diagonal b at

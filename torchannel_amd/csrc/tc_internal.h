@@ -12,6 +12,9 @@
 #define TC_S_UTARGET 5   // target bulk velocity U_bulk
 #define TC_S_STEP 6      // steps taken (as a double)
 #define TC_S_NONFINITE 7   // set to 1 by the projection kernel when a non-finite velocity appears
+// adjoint scalar block
+#define TC_A_LF 0        // cotangent of the forcing F
+#define TC_A_NSCAL 4
 
 // k_explicit_cn modes
 #define TC_K1_RHS 1
@@ -98,5 +101,21 @@ int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, doub
                double* partials, unsigned* counter, int update_dt, double cfl_target, double dt_min,
                double dt_max, int nblocks_max, hipStream_t st);
 int cfl_blocks(const Grid* gr);
+
+// tc_adjoint.hip (float64)
+int adj_launch_pre(const Grid* g, double rdt, double inv_volume, const double* ascal, void* const* lq,
+                   void* const* ys, hipStream_t st);
+int adj_cn_blocks(const Grid* g);
+int adj_launch_cn(const Grid* g, void* const* z, const double* tab_cell, const double* tab_face, double dt,
+                  double c1, double c0, void* const* qs, void* const* lr, void* const* acc, void* grad,
+                  double* partials, hipStream_t st);
+int adj_launch_lamf(const double* partials, int nb, double* ascal, hipStream_t st);
+int adj_launch_rhs(const Grid* g, const void* u, const void* v, const void* w, void* const* lr, void* const* e,
+                   hipStream_t st);
+int adj_launch_fold(const Grid* g, void* const* qs, void* const* e, double obj_coef, int obj_top, void* const* lq,
+                    hipStream_t st);
+int adj_tauw_blocks(const Grid* g);
+int adj_launch_tauw(const Grid* g, const void* u, int top, double scale, double* partials, double* out,
+                    hipStream_t st);
 
 }  // namespace tc
