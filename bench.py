@@ -140,6 +140,44 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
     return out
 
 
+def run_adjoint(n_steps=100, ckpt=10, reps=3):
+    """Adjoint/forward cost at 96x96x64: 100 fixed-dt float64 steps, checkpoints every 10, a
+    seeded Gaussian body force (sigma 1e-3).  forward = objective(f) (the steps, the per-step
+    objective and the checkpoints); adjoint = gradient(f) (recompute from the checkpoints + the
+    reverse sweep).  Medians over `reps` runs."""
+    import torch
+    import torchannel_amd as tc
+
+    c = config("fp64", 96, 96, 64, steps=n_steps)
+    c["compute"]["rhs_branch"] = "gpu"
+    ch = tc.ChannelFlow(config=c)
+    gen = torch.Generator().manual_seed(0)
+    f = (1e-3 * torch.randn(tuple(ch.u.shape), generator=gen, dtype=torch.float64)).cuda()
+    run = tc.WallShearAdjoint(ch, n_steps, 0.005, checkpoint_every=ckpt)
+    run.gradient(f)
+    torch.cuda.synchronize()
+    tf, tg = [], []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e2 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run.objective(f)
+        e1.record()
+        run.gradient(f)
+        e2.record()
+        torch.cuda.synchronize()
+        tf.append(e0.elapsed_time(e1))
+        tg.append(e1.elapsed_time(e2))
+    tf.sort()
+    tg.sort()
+    fw, ad = tf[len(tf) // 2], tg[len(tg) // 2]
+    return {"workload": "96x96x64, 100 fixed-dt (0.005) float64 steps, checkpoints every 10",
+            "forward_ms": round(fw, 3), "adjoint_ms": round(ad, 3),
+            "forward_step_us": round(fw * 1e3 / n_steps, 2), "adjoint_step_us": round(ad * 1e3 / n_steps, 2),
+            "adjoint_over_forward": round(ad / fw, 3)}
+
+
 def cpu_baseline(budget_s=20.0):
     """The float64 CPU oracle (a port of the reference op sequence, GPU-branch RHS) on the
     host cores, on a bounded sample of the same 192x192x128 workload at fixed dt."""
@@ -168,6 +206,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
@@ -197,6 +236,10 @@ def main():
     if not args.no_fp32 and args.precision == "fp64":
         other = run_gpu("fp32", steps, warmup)
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
+    adj = None
+    if not args.no_adjoint and args.precision == "fp64":
+        adj = run_adjoint()
+        print(f"[bench] adjoint/forward: {adj['adjoint_over_forward']:.2f}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline()
@@ -238,6 +281,7 @@ def main():
                           "frac": round(main_run["step_gbs"] / HBM_PEAK_GBS, 4)},
         "kernels_us": {n: round(v["avg_us"], 2) for n, v in main_run["kernels"].items()},
         "cpu_baseline": cpu,
+        "adjoint": adj,
     }
     if other is not None:
         line["fp32"] = {"value": round(other["mcell_steps_per_s"], 2), "ms_per_step": round(other["ms_per_step"], 5),
