@@ -1,0 +1,72 @@
+"""Launch-shape knobs and the autotuner: every setting gives bit-identical steps."""
+import math
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg(nx, ny, nz, precision="fp64", autotune=False):
+    return {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": 4 * math.pi, "Ly": 4 * math.pi / 3, "Lz": 2.0},
+        "flow": {"Re": 2870.0, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": 2.6},
+        "time": {"dt": 0.005, "n_steps": 10},
+        "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": 0},
+        "compute": {"device": "cuda", "precision": precision, "autotune": autotune},
+    }
+
+
+@pytest.fixture(scope="module")
+def tc():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import torchannel_amd
+    torchannel_amd._lib.load()
+    return torchannel_amd
+
+
+def run(tc, knobs, precision, steps=4, shape=(32, 24, 48)):
+    ch = tc.ChannelFlow(config=cfg(*shape, precision=precision))
+    for k, v in knobs.items():
+        ch._grid.set_tuning(k, v)
+    for _ in range(steps):
+        ch.step_imex(0.005)
+    return [q.clone() for q in (ch.u, ch.v, ch.w, ch.p, ch._scal)]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_knobs_do_not_change_results(tc, precision):
+    from torchannel_amd import _lib as L
+    ref = run(tc, {}, precision)
+    for knobs in ({L.TUNE_K1_JB: 1}, {L.TUNE_K1_JB: 3}, {L.TUNE_K1_JB: 26},
+                  {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 17}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 9}):
+        print(f"[tuning] {precision} {knobs}", flush=True)
+        got = run(tc, knobs, precision)
+        for a, b in zip(got, ref):
+            assert torch.equal(a, b), knobs
+
+
+def test_autotune_keeps_state_and_results(tc, tmp_path):
+    from torchannel_amd import _lib as L
+    ref = run(tc, {}, "fp64", steps=3)
+    ch = tc.ChannelFlow(config=cfg(32, 24, 48, autotune=True))
+    for _ in range(3):
+        ch.step_imex(0.005)
+    assert ch.tuning is not None and L.TUNE_K1_JB in ch.tuning
+    for a, b in zip([ch.u, ch.v, ch.w, ch.p, ch._scal], ref):
+        assert torch.equal(a, b)
+    path = str(tmp_path / "tune.json")
+    chosen = tc.tune(ch, path=path, force=True)
+    again = tc.tune(ch, path=path)
+    assert chosen == again
+
+
+def test_bad_knob_rejected(tc):
+    from torchannel_amd import _lib as L
+    ch = tc.ChannelFlow(config=cfg(16, 12, 16))
+    with pytest.raises(RuntimeError):
+        ch._grid.set_tuning(L.TUNE_K1_JB, 10 ** 6)
+    with pytest.raises(RuntimeError):
+        ch._grid.set_tuning(99, 1)

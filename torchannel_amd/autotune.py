@@ -1,0 +1,99 @@
+"""Launch-shape autotuner for the step kernels (no counterpart in the reference, whose kernels
+are TorchScript-fused).
+
+Each knob changes only how work is dealt to workgroups, never the arithmetic, so every setting
+gives bit-identical results; the tuner times the kernel a knob drives with HIP events on the
+solver's stream and keeps the fastest setting on the grid.  Knobs:
+
+* ``TUNE_K1_JB``     j-columns per explicit/AB2/CN pencil (k_explicit_cn)
+* ``TUNE_THOMAS_SB`` (kx, ky) systems per block of the wall-normal Poisson solve
+
+Chosen settings are cached per (nx, ny, nz, dtype, top wall, device name) in memory and, when
+``path`` (or the ``TORCHANNEL_TUNE_CACHE`` environment variable) names a JSON file, on disk.
+"""
+import json
+import os
+
+import torch
+
+from . import _lib as L
+
+# phase indices of tc_solver_phase
+_PHASE_K1 = 1
+_PHASE_THOMAS = 4
+
+_memory = {}
+
+
+def _key(ch):
+    name = torch.cuda.get_device_name(ch.u.device)
+    return f"{ch.nx}x{ch.ny}x{ch.nz}:{str(ch.wdtype).split('.')[-1]}:{ch.top_wall_bc_type}:{name}"
+
+
+def _candidates(ch):
+    jb = sorted({j for j in (2, 4, 6, 8, 12, 16, 24, 32) if j <= ch.ny + 2})
+    sb = [4, 8, 16, 24, 32, 48, 64]
+    return {L.TUNE_K1_JB: (jb, _PHASE_K1), L.TUNE_THOMAS_SB: (sb, _PHASE_THOMAS)}
+
+
+def _time_phase(ch, phase, reps):
+    st = ch._stream()
+    L.call("tc_solver_phase", ch._solver, phase, 2, st)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    L.call("tc_solver_phase", ch._solver, phase, reps, st)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def _load(path):
+    if path and os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return {}
+
+
+def tune(channel, reps=10, path=None, force=False):
+    """Pick the knobs for `channel`'s grid; returns {knob: value}.  The solver must have taken
+    one step (the phases time the step's own intermediates); the state is not advanced by the
+    timing launches except for scratch buffers the next step rewrites."""
+    ch = channel
+    if ch._newest < 0:
+        raise RuntimeError("autotune needs one step taken first")
+    path = path or os.environ.get("TORCHANNEL_TUNE_CACHE")
+    key = _key(ch)
+    if not force:
+        hit = _memory.get(key) or _load(path).get(key)
+        if hit:
+            chosen = {int(k): int(v) for k, v in hit.items()}
+            for k, v in chosen.items():
+                ch._grid.set_tuning(k, v)
+            _memory[key] = chosen
+            return chosen
+    # the timing launches overwrite the step's intermediates and the newest rhs set: keep the state
+    saved = [q.clone() for q in (ch.u, ch.v, ch.w, ch.p, ch._scal)] + [r.clone() for s in ch._rhs for r in s]
+    chosen = {}
+    for knob, (values, phase) in _candidates(ch).items():
+        best, best_t = None, float("inf")
+        for v in values:
+            try:
+                ch._grid.set_tuning(knob, v)
+                t = _time_phase(ch, phase, reps)
+            except RuntimeError:
+                continue                     # setting outside the kernel's limits on this grid
+            if t < best_t:
+                best, best_t = v, t
+        ch._grid.set_tuning(knob, best if best is not None else 0)
+        chosen[knob] = best if best is not None else 0
+    dst = [ch.u, ch.v, ch.w, ch.p, ch._scal] + [r for s in ch._rhs for r in s]
+    for d, s in zip(dst, saved):
+        d.copy_(s)
+    _memory[key] = chosen
+    if path:
+        data = _load(path)
+        data[key] = {str(k): v for k, v in chosen.items()}
+        with open(path, "w") as fh:
+            json.dump(data, fh, indent=1)
+    return chosen

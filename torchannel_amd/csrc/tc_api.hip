@@ -155,6 +155,10 @@ int tc_grid_set_tuning(tc_grid* h, int key, int value) {
       if (value < 0 || value > g.d.ny + 2) return TC_ERR_ARG;
       g.k1_jb = value;
       return TC_OK;
+    case TC_TUNE_THOMAS_SB:
+      if (value < 0 || value > 128) return TC_ERR_ARG;
+      g.thomas_sb = value;
+      return TC_OK;
     default:
       return TC_ERR_ARG;
   }
@@ -166,6 +170,9 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
   switch (key) {
     case TC_TUNE_K1_JB:
       *value = g.k1_jb > 0 ? g.k1_jb : 8;
+      return TC_OK;
+    case TC_TUNE_THOMAS_SB:
+      *value = g.thomas_sb;
       return TC_OK;
     default:
       return TC_ERR_ARG;

@@ -49,6 +49,7 @@ struct Grid {
   double* tabs = nullptr;         // CN tables: cell, face
   double* scal = nullptr;         // scalar block for stand-alone ops
   int k1_jb = 0;                  // j-columns per K1 pencil (0: default 8; set by the tuner)
+  int thomas_sb = 0;              // systems per wall-normal Thomas block (0: plan default)
 
   template <typename T>
   Metric<T> metric() const;

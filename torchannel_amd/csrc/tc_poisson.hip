@@ -584,9 +584,11 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 1: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, -1, st);
     case 2: {
       const int nsys = pl->nkx * d.ny;
-      const int blocks = (nsys + pl->sb - 1) / pl->sb;
-      const size_t lds = sizeof(cplx<T>) * (size_t)pl->sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
-      hipLaunchKernelGGL((k_poisson_thomas<T>), dim3(blocks), dim3(256), lds, st, nsys, d.nz, pl->sb, pl->cl,
+      const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : pl->sb;
+      const int blocks = (nsys + sb - 1) / sb;
+      const size_t lds = sizeof(cplx<T>) * (size_t)sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
+      if (lds > 160 * 1024) return TC_ERR_ARG;
+      hipLaunchKernelGGL((k_poisson_thomas<T>), dim3(blocks), dim3(256), lds, st, nsys, d.nz, sb, pl->cl,
                          (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,
                          (cplx<T>*)pl->spec);
       TC_CHECK_LAUNCH();

@@ -43,7 +43,8 @@ class ChannelFlow:
     output, statistics, boundary_conditions) plus ``compute.precision`` ('fp64' | 'fp32') and
     ``initialization.seed`` (torch.manual_seed on the CPU generator before the noise draws) and
     ``compute.rhs_branch`` ('gpu' = the fused RHS the reference runs on any HIP/CUDA device,
-    'cpu' = the separate kernels of its CPU branch, solver.py:593-622).
+    'cpu' = the separate kernels of its CPU branch, solver.py:593-622) and ``compute.autotune``
+    (pick the kernels' launch shapes by timing after the first step; results do not change).
     ``config`` may be given as a dict instead of a file path.
     """
 
@@ -61,6 +62,7 @@ class ChannelFlow:
         if prec not in ("fp64", "fp32"):
             raise ValueError(f"Invalid precision: {prec}")
         self.precision = prec
+        self.autotune = bool(config.get("compute", {}).get("autotune", False))
         branch = config.get("compute", {}).get("rhs_branch", "gpu")
         if branch not in ("gpu", "cpu"):
             raise ValueError(f"Invalid rhs_branch: {branch}")
@@ -149,6 +151,7 @@ class ChannelFlow:
         self._scal[L.S_UTARGET] = self.U_bulk
         self._newest = -1
         self.body_force = None
+        self.tuning = None
         ra = (ctypes.c_void_p * 3)(*[q.data_ptr() for q in self._rhs[0]])
         rb = (ctypes.c_void_p * 3)(*[q.data_ptr() for q in self._rhs[1]])
         h = ctypes.c_void_p()
@@ -222,7 +225,11 @@ class ChannelFlow:
         self._newest = newest.value
         if dt is not None:
             self.dt = float(dt)
-        return self._scal[L.S_UBULK].clone(), self._scal[L.S_F].clone()
+        out = self._scal[L.S_UBULK].clone(), self._scal[L.S_F].clone()
+        if self.autotune and self.tuning is None:
+            from .autotune import tune
+            self.tuning = tune(self)
+        return out
 
     def compute_cfl_dt(self):
         """solver.py:465-496 (host value; one device sync)."""
