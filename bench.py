@@ -178,6 +178,61 @@ def run_adjoint(n_steps=100, ckpt=10, reps=3):
             "adjoint_over_forward": round(ad / fw, 3)}
 
 
+def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
+    """Batched ensemble: 48 channels, nz from {32, 48, 64, 96}, nx = ny from {48, 64, 96},
+    Re_bulk in [2000, 4000], gamma in [2.0, 2.8] (numpy default_rng(2026)), instance i seeded
+    with i, float32, fixed dt 0.005; 300 steps as graph chunks of 10 ensemble steps.  Baselines:
+    the same channels stepped one after another, eagerly (tc_solver_step per channel) and by
+    replaying each channel's own 10-step graph."""
+    import torch
+    import torchannel_amd as tc
+
+    ens = tc.make_ensemble(tc.ensemble_configs(n))
+    ens.step(0.005)
+    for _ in range(warmup - 1):
+        ens.step()
+    ens.run_graph(0, chunk)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ens.run_graph(steps // chunk, chunk)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    cells = ens.n_cells
+    done = (steps // chunk) * chunk
+    # looping the single-channel path
+    chans = ens.channels
+    loop_steps = 20
+    e0.record()
+    for _ in range(loop_steps):
+        for ch in chans:
+            ch.step_imex(None)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_loop = e0.elapsed_time(e1) / loop_steps
+    for ch in chans:
+        ch.run_graph(0, chunk)
+    torch.cuda.synchronize()
+    loop_chunks = 3
+    e0.record()
+    for _ in range(loop_chunks):
+        for ch in chans:
+            ch.run_graph(1, chunk)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_loop_graph = e0.elapsed_time(e1) / (loop_chunks * chunk)
+    ms_step = ms / done
+    return {"workload": f"{n} channels ({cells} cells), float32, fixed dt 0.005, {done} steps in graph chunks of {chunk}",
+            "cells": cells, "value": round(cells / (ms_step * 1e-3) / 1e6, 2), "unit": "Mcell-steps/s",
+            "ms_per_step": round(ms_step, 4),
+            "loop_eager_ms_per_step": round(ms_loop, 4),
+            "loop_graph_ms_per_step": round(ms_loop_graph, 4),
+            "speedup_vs_loop_eager": round(ms_loop / ms_step, 2),
+            "speedup_vs_loop_graph": round(ms_loop_graph / ms_step, 2)}
+
+
 def cpu_baseline(budget_s=20.0):
     """The float64 CPU oracle (a port of the reference op sequence, GPU-branch RHS) on the
     host cores, on a bounded sample of the same 192x192x128 workload at fixed dt."""
@@ -207,6 +262,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
@@ -236,6 +292,11 @@ def main():
     if not args.no_fp32 and args.precision == "fp64":
         other = run_gpu("fp32", steps, warmup)
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
+    ens = None
+    if not args.no_ensemble:
+        ens = run_ensemble()
+        print(f"[bench] ensemble: {ens['value']:.1f} Mcell-steps/s, x{ens['speedup_vs_loop_graph']} vs looped graphs",
+              file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint and args.precision == "fp64":
         adj = run_adjoint()
@@ -282,6 +343,7 @@ def main():
         "kernels_us": {n: round(v["avg_us"], 2) for n, v in main_run["kernels"].items()},
         "cpu_baseline": cpu,
         "adjoint": adj,
+        "ensemble": ens,
     }
     if other is not None:
         line["fp32"] = {"value": round(other["mcell_steps_per_s"], 2), "ms_per_step": round(other["ms_per_step"], 5),

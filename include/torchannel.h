@@ -199,6 +199,22 @@ int tc_adjoint_begin(tc_adjoint* a, double dt, double obj_weight, void* grad, vo
 int tc_adjoint_step(tc_adjoint* a, const void* u, const void* v, const void* w, int first, double obj_weight,
                     void* grad, void* stream);
 
+/* ---- ensembles ---------------------------------------------------------------------------
+ * No counterpart in the reference (one channel per process there).  Independent channels of
+ * any sizes (each a tc_solver with its own grid and Poisson plan; one dtype and one rhs branch)
+ * advance together: every phase of the step is one launch over all channels, blocks mapped to
+ * (channel, block) by prefix sums, so small channels share the GPU instead of each launching
+ * its own under-filled kernels.  Results are bit-identical to tc_solver_step on each channel.
+ * The channels' launch-shape knobs are read at creation.  dt > 0 sets every channel's dt, else
+ * each keeps its device dt; `first` takes the forward-Euler start (histories must be in step). */
+typedef struct tc_ensemble tc_ensemble;
+int tc_ensemble_create(tc_solver* const* solvers, int n, tc_ensemble** out);
+int tc_ensemble_destroy(tc_ensemble* e);
+int tc_ensemble_step(tc_ensemble* e, double dt, int first, void* stream);
+/* `steps` (even) ensemble steps captured into one HIP graph; replay advances every channel. */
+int tc_ensemble_capture(tc_ensemble* e, int steps, void* stream);
+int tc_ensemble_replay(tc_ensemble* e, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

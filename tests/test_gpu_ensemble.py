@@ -1,0 +1,78 @@
+"""Ensembles: channels of uneven sizes stepped together are bit-identical to stepping each alone."""
+import copy
+import math
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def cfg(nx, ny, nz, re=2870.0, gamma=2.6, seed=0, precision="fp64", private=True, top="dirichlet"):
+    return {
+        "grid": {"nx": nx, "ny": ny, "nz": nz},
+        "domain": {"Lx": 4 * math.pi, "Ly": 4 * math.pi / 3, "Lz": 2.0},
+        "flow": {"Re": re, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": gamma},
+        "time": {"dt": 0.005, "n_steps": 10},
+        "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": seed},
+        "compute": {"device": "cuda", "precision": precision, "private_plan": private},
+        "boundary_conditions": {"top_wall": {"type": top}},
+    }
+
+
+@pytest.fixture(scope="module")
+def tc():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import torchannel_amd
+    torchannel_amd._lib.load()
+    return torchannel_amd
+
+
+def state(ch):
+    return [ch.u, ch.v, ch.w, ch.p, ch._scal] + [r for s in ch._rhs for r in s]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ensemble_matches_single_channels(tc, precision):
+    cfgs = [cfg(16, 12, 16, seed=1, precision=precision),
+            cfg(24, 18, 48, re=2100.0, gamma=2.1, seed=2, precision=precision),
+            cfg(32, 24, 96, re=3900.0, gamma=2.75, seed=3, precision=precision),
+            cfg(16, 12, 16, seed=1, precision=precision),                   # same grid, own plan
+            cfg(20, 16, 130, re=3000.0, seed=4, precision=precision, top="neumann")]
+    print(f"[ensemble] {precision} {len(cfgs)} channels", flush=True)
+    ens = tc.make_ensemble(cfgs)
+    singles = [tc.ChannelFlow(config=copy.deepcopy(c)) for c in cfgs]
+    ens.step(0.005)
+    for s in singles:
+        s.step_imex(0.005)
+    for _ in range(3):
+        ens.step()
+        for s in singles:
+            s.step_imex(None)
+    ens.run_graph(2, 4)
+    for _ in range(8):
+        for s in singles:
+            s.step_imex(None)
+    torch.cuda.synchronize()
+    for q, (a, b) in enumerate(zip(ens.channels, singles)):
+        for x, y in zip(state(a), state(b)):
+            assert torch.equal(x, y), q
+
+
+def test_ensemble_rejects_repeated_channel(tc):
+    a = tc.ChannelFlow(config=cfg(16, 12, 16, private=False))
+    with pytest.raises(RuntimeError):
+        tc.ChannelEnsemble([a, a])
+    b = tc.ChannelFlow(config=cfg(16, 12, 16, precision="fp32"))
+    with pytest.raises(ValueError):
+        tc.ChannelEnsemble([a, b])
+
+
+def test_ensemble_configs_are_seeded():
+    import torchannel_amd as t
+    a = t.ensemble_configs(8)
+    b = t.ensemble_configs(8)
+    assert a == b
+    assert {c["grid"]["nz"] for c in t.ensemble_configs(48)} <= {32, 48, 64, 96}
+    assert all(2000 <= c["flow"]["Re"] <= 4000 and 2.0 <= c["flow"]["gamma"] <= 2.8 for c in a)

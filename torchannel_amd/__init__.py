@@ -8,6 +8,7 @@ hot operation is a hand-written HIP kernel in libtorchannel_hip.so behind a C AB
 from . import _lib
 from .adjoint import WallShearAdjoint, wall_shear_objective
 from .autotune import tune
+from .ensemble import ChannelEnsemble, ensemble_configs, make_ensemble
 from .initflow import initialize_flow
 from .operators import (compute_cfl_fused, compute_momentum_rhs_fused_imex, compute_momentum_rhs_separate,
                         solve_implicit_diffusion_u, solve_implicit_diffusion_v, solve_implicit_diffusion_w,
@@ -24,5 +25,6 @@ __all__ = [
     "compute_momentum_rhs_fused_imex", "compute_momentum_rhs_separate", "compute_u_tau", "generate_grid",
     "initialize_fft_solver", "initialize_flow", "pcr_solve", "project_velocity", "solve_implicit_diffusion_u",
     "solve_implicit_diffusion_v", "solve_implicit_diffusion_w", "solve_poisson_fft", "solve_tridiagonal",
-    "solve_tridiagonal_batch", "TurbulenceStats", "WallShearAdjoint", "wall_shear_objective", "tune",
+    "solve_tridiagonal_batch", "TurbulenceStats", "WallShearAdjoint", "wall_shear_objective", "tune", "ChannelEnsemble",
+    "ensemble_configs", "make_ensemble",
 ]

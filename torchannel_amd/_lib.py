@@ -39,6 +39,11 @@ SIGNATURES = {
     "tc_adjoint_objective": [_vp, _vp, _d, _vp, _vp],
     "tc_adjoint_begin": [_vp, _d, _d, _vp, _vp],
     "tc_adjoint_step": [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp],
+    "tc_ensemble_create": [ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp)],
+    "tc_ensemble_destroy": [_vp],
+    "tc_ensemble_step": [_vp, _d, _i, _vp],
+    "tc_ensemble_capture": [_vp, _i, _vp],
+    "tc_ensemble_replay": [_vp, _vp],
     "tc_grid_get_tuning": [_vp, _i, ctypes.POINTER(_i)],
     "tc_apply_bc": [_vp, _vp, _vp, _vp, _vp],
     "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -171,6 +176,18 @@ class Grid:
                 check(load().tc_poisson_create(self.handle, ctypes.byref(h)))
             self._poisson = h
         return self._poisson
+
+    def new_poisson(self):
+        """A Poisson plan of this grid with its own workspace, owned by the caller
+        (destroy_poisson); channels stepped together each need one."""
+        h = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            check(load().tc_poisson_create(self.handle, ctypes.byref(h)))
+        return h
+
+    @staticmethod
+    def destroy_poisson(h):
+        load().tc_poisson_destroy(h)
 
     def __del__(self):
         try:

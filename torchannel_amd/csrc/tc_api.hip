@@ -2,6 +2,7 @@
 // points and the fused IMEX step with optional HIP-graph capture.
 #include <new>
 
+#include "tc_ensemble.h"
 #include "tc_internal.h"
 #include "tc_poisson.h"
 
@@ -615,6 +616,222 @@ int tc_adjoint_step(tc_adjoint* a, const void* u, const void* v, const void* w, 
 int tc_solver_set_newest(tc_solver* s, int newest) {
   if (!s || newest < -1 || newest > 1) return TC_ERR_ARG;
   s->newest = newest;
+  return TC_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Ensembles: independent channels stepped together, one launch per phase
+// ---------------------------------------------------------------------------------------------
+struct tc_ensemble {
+  int n = 0;
+  int dtype = TC_F64;
+  int branch = TC_BRANCH_GPU;
+  std::vector<tc_solver*> solvers;
+  void* dch = nullptr;                     // EnsChan<T>[n]
+  struct Group {
+    int R = 1, n = 0, blocks_k1 = 0, blocks_k5 = 0;
+    int *chans = nullptr, *start_k1 = nullptr, *start_k5 = nullptr;
+  };
+  std::vector<Group> groups;                // channels grouped by rows per lane
+  int* pstart[4] = {nullptr, nullptr, nullptr, nullptr};
+  int pblocks[4] = {0, 0, 0, 0};
+  size_t plds[4] = {0, 0, 0, 0};
+  int newest = -1;
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+}  // extern "C"
+
+template <typename T>
+static void fill_chan(EnsChan<T>& c, tc_solver* s) {
+  const Grid* g = s->g;
+  const Poisson* pl = s->pl;
+  c.d = g->d;
+  c.mt = g->metric<T>();
+  c.u = (T*)s->u; c.v = (T*)s->v; c.w = (T*)s->w; c.p = (T*)s->p;
+  c.su = (T*)s->su; c.sv = (T*)s->sv; c.sw = (T*)s->sw;
+  for (int a = 0; a < 2; ++a)
+    for (int q = 0; q < 3; ++q) c.rs[a][q] = (T*)s->rs[a][q];
+  c.scal = s->scal;
+  c.tabs = s->tabs;
+  c.partials = s->partials;
+  c.body = nullptr;
+  c.fx = pl->fx; c.fy = pl->fy; c.twx = pl->twx; c.twy = pl->twy;
+  c.spec = (cplx<T>*)pl->spec;
+  c.cl = pl->cl;
+  c.th_cp = (const T*)pl->th_cp; c.th_iden = (const T*)pl->th_iden;
+  c.th_cp0 = pl->th_cp0; c.th_iden0 = pl->th_iden0;
+  c.pairs = pl->pairs; c.kzy = pl->kzy;
+  c.sb = g->thomas_sb > 0 ? g->thomas_sb : pl->sb;
+  c.nkx = pl->nkx;
+  c.jb = g->k1_jb > 0 ? g->k1_jb : 8;
+  c.k5_blocks = k5_blocks(g);
+  c.inv_volume = s->inv_volume;
+  c.nu = g->nu;
+  c.theta = 0.5;
+  c.top = g->d.top;
+}
+
+static int* dev_ints(const std::vector<int>& h) {
+  int* d = nullptr;
+  if (hipMalloc(&d, sizeof(int) * std::max<size_t>(1, h.size())) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(d);
+    return nullptr;
+  }
+  return d;
+}
+
+extern "C" {
+
+int tc_ensemble_create(tc_solver* const* solvers, int n, tc_ensemble** out) {
+  if (!solvers || n < 1 || !out) return TC_ERR_ARG;
+  for (int c = 0; c < n; ++c) {
+    if (!solvers[c]) return TC_ERR_ARG;
+    if (solvers[c]->g->dtype != solvers[0]->g->dtype || solvers[c]->branch != solvers[0]->branch)
+      return TC_ERR_ARG;
+    for (int q = 0; q < c; ++q)   // channels must not share a solver or a Poisson workspace
+      if (solvers[q] == solvers[c] || solvers[q]->pl == solvers[c]->pl) return TC_ERR_ARG;
+  }
+  tc_ensemble* e = new (std::nothrow) tc_ensemble();
+  if (!e) return TC_ERR_NOMEM;
+  e->n = n;
+  e->dtype = solvers[0]->g->dtype;
+  e->branch = solvers[0]->branch;
+  e->solvers.assign(solvers, solvers + n);
+  bool ok = true;
+  // channel descriptors
+  if (e->dtype == TC_F64) {
+    std::vector<EnsChan<double>> h(n);
+    for (int c = 0; c < n; ++c) fill_chan(h[c], solvers[c]);
+    ok = hipMalloc(&e->dch, sizeof(EnsChan<double>) * n) == hipSuccess &&
+         hipMemcpy(e->dch, h.data(), sizeof(EnsChan<double>) * n, hipMemcpyHostToDevice) == hipSuccess;
+  } else {
+    std::vector<EnsChan<float>> h(n);
+    for (int c = 0; c < n; ++c) fill_chan(h[c], solvers[c]);
+    ok = hipMalloc(&e->dch, sizeof(EnsChan<float>) * n) == hipSuccess &&
+         hipMemcpy(e->dch, h.data(), sizeof(EnsChan<float>) * n, hipMemcpyHostToDevice) == hipSuccess;
+  }
+  // stencil / projection kernels: one launch per rows-per-lane group
+  for (int R = 1; R <= 4 && ok; ++R) {
+    std::vector<int> ch, s1{0}, s5{0};
+    for (int c = 0; c < n; ++c) {
+      const Grid* g = solvers[c]->g;
+      if ((g->d.nz + TC_WAVE - 1) / TC_WAVE != R) continue;
+      ch.push_back(c);
+      s1.push_back(s1.back() + k1_blocks(g));
+      s5.push_back(s5.back() + k5_blocks(g));
+    }
+    if (ch.empty()) continue;
+    tc_ensemble::Group gr;
+    gr.R = R;
+    gr.n = (int)ch.size();
+    gr.blocks_k1 = s1.back();
+    gr.blocks_k5 = s5.back();
+    gr.chans = dev_ints(ch);
+    gr.start_k1 = dev_ints(s1);
+    gr.start_k5 = dev_ints(s5);
+    ok = gr.chans && gr.start_k1 && gr.start_k5;
+    e->groups.push_back(gr);
+  }
+  // Poisson phases
+  const int tsz = e->dtype == TC_F64 ? 8 : 4;
+  std::vector<int> st[4];
+  for (int q = 0; q < 4; ++q) st[q].push_back(0);
+  for (int c = 0; c < n && ok; ++c) {
+    int b[4];
+    size_t l[4];
+    poisson_geometry(solvers[c]->pl, tsz, b, l);
+    for (int q = 0; q < 4; ++q) {
+      st[q].push_back(st[q].back() + b[q]);
+      e->plds[q] = std::max(e->plds[q], l[q]);
+    }
+  }
+  for (int q = 0; q < 4 && ok; ++q) {
+    e->pblocks[q] = st[q].back();
+    e->pstart[q] = dev_ints(st[q]);
+    ok = e->pstart[q] != nullptr;
+  }
+  ok = ok && hipStreamCreateWithFlags(&e->cap_stream, hipStreamNonBlocking) == hipSuccess;
+  if (!ok) {
+    tc_ensemble_destroy(e);
+    return TC_ERR_NOMEM;
+  }
+  e->newest = solvers[0]->newest;
+  for (int c = 1; c < n; ++c)
+    if (solvers[c]->newest != e->newest) e->newest = -2;   // histories out of step
+  if (e->newest == -2) {
+    tc_ensemble_destroy(e);
+    return TC_ERR_ARG;
+  }
+  *out = e;
+  return TC_OK;
+}
+
+int tc_ensemble_destroy(tc_ensemble* e) {
+  if (!e) return TC_OK;
+  if (e->gexec) hipGraphExecDestroy(e->gexec);
+  if (e->graph) hipGraphDestroy(e->graph);
+  if (e->cap_stream) hipStreamDestroy(e->cap_stream);
+  for (auto& g : e->groups) { hipFree(g.chans); hipFree(g.start_k1); hipFree(g.start_k5); }
+  for (int q = 0; q < 4; ++q) hipFree(e->pstart[q]);
+  hipFree(e->dch);
+  delete e;
+  return TC_OK;
+}
+
+static int ensemble_step_impl(tc_ensemble* e, double dt, int first, hipStream_t st) {
+  if (!first && e->newest < 0) return TC_ERR_ARG;
+  const int cur = first ? 0 : 1 - e->newest;
+  const int prev = first ? -1 : e->newest;
+  TC_RC(ens_launch_tables(e->dtype, e->dch, e->n, dt, st));
+  const int mode = TC_K1_RHS | TC_K1_STEP | TC_K1_FORCE | (first ? TC_K1_FIRST : 0);
+  for (auto& g : e->groups)
+    TC_RC(ens_launch_k1(e->dtype, g.R, e->dch, g.chans, g.start_k1, g.n, g.blocks_k1, cur, prev, mode, e->branch, st));
+  const int* starts[4] = {e->pstart[0], e->pstart[1], e->pstart[2], e->pstart[3]};
+  TC_RC(ens_launch_poisson(e->dtype, e->dch, e->n, starts, e->pblocks, e->plds, st));
+  for (auto& g : e->groups)
+    TC_RC(ens_launch_k5(e->dtype, g.R, e->dch, g.chans, g.start_k5, g.n, g.blocks_k5, TC_K5_BULK | TC_K5_FORCING, st));
+  TC_RC(ens_launch_finalize(e->dtype, e->dch, e->n, TC_K5_BULK | TC_K5_FORCING, st));
+  e->newest = cur;
+  for (tc_solver* s : e->solvers) s->newest = cur;
+  return TC_OK;
+}
+
+int tc_ensemble_step(tc_ensemble* e, double dt, int first, void* stream) {
+  if (!e) return TC_ERR_ARG;
+  return ensemble_step_impl(e, dt, first, (hipStream_t)stream);
+}
+
+int tc_ensemble_capture(tc_ensemble* e, int steps, void* stream) {
+  if (!e || steps < 2 || (steps & 1) || e->newest < 0) return TC_ERR_ARG;
+  if (e->gexec) { hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+  if (e->graph) { hipGraphDestroy(e->graph); e->graph = nullptr; }
+  hipStream_t cs = e->cap_stream;
+  TC_HIP(hipStreamSynchronize((hipStream_t)stream));
+  const int newest0 = e->newest;
+  TC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+  int rc = TC_OK;
+  for (int i = 0; i < steps && rc == TC_OK; ++i) rc = ensemble_step_impl(e, -1.0, 0, cs);
+  hipGraph_t graph = nullptr;
+  hipError_t err = hipStreamEndCapture(cs, &graph);
+  e->newest = newest0;
+  for (tc_solver* s : e->solvers) s->newest = newest0;
+  if (rc != TC_OK) {
+    if (graph) hipGraphDestroy(graph);
+    return rc;
+  }
+  if (err != hipSuccess) return -(int)err;
+  e->graph = graph;
+  TC_HIP(hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0));
+  return TC_OK;
+}
+
+int tc_ensemble_replay(tc_ensemble* e, void* stream) {
+  if (!e || !e->gexec) return TC_ERR_ARG;
+  TC_HIP(hipGraphLaunch(e->gexec, (hipStream_t)stream));
   return TC_OK;
 }
 

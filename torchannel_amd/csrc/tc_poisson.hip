@@ -8,6 +8,7 @@
 //   P3 k_poisson_pcr : one wavefront per (kx, ky) system, PCR along z, (0,0) gauge-fixed in f64
 //   P4 k_fft_y : inverse complex FFT along y
 //   P5 k_inv_x : half spectra -> real rows, 1/(nx*ny), p written with every ghost image
+#include "tc_ensemble.h"
 #include "tc_fft.h"
 #include "tc_internal.h"
 #include "tc_poisson.h"
@@ -21,18 +22,18 @@ namespace tc {
 // w walls are zero).  Otherwise rhs is read from a (nx, ny, nz_in) array (planes of nz_in).
 // ---------------------------------------------------------------------------------------------
 template <typename T, bool FROM_VEL>
-__global__ __launch_bounds__(256) void k_fwd_x(Dims d, Metric<T> mt, const T* __restrict__ u,
+__device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __restrict__ u,
                                                const T* __restrict__ v, const T* __restrict__ w,
                                                const T* __restrict__ rin, int nz_in,
                                                const double* __restrict__ scal, FftDesc fx,
                                                const double2* __restrict__ twx, cplx<T>* __restrict__ S,
-                                               int pairs) {
+                                               int pairs,
+    int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
   const int nx = d.nx, ny = d.ny;
   const int KZ = 2 * pairs;
   const int nch = (nz_in + KZ - 1) / KZ;
-  const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int j = blk / nch, k0 = (blk - j * nch) * KZ;
   const int ldb = nx + 1;
   T rdt = T(1);
@@ -76,15 +77,25 @@ __global__ __launch_bounds__(256) void k_fwd_x(Dims d, Metric<T> mt, const T* __
   }
 }
 
+template <typename T, bool FROM_VEL>
+__global__ __launch_bounds__(256) void k_fwd_x(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                               const T* __restrict__ v, const T* __restrict__ w,
+                                               const T* __restrict__ rin, int nz_in,
+                                               const double* __restrict__ scal, FftDesc fx,
+                                               const double2* __restrict__ twx, cplx<T>* __restrict__ S,
+                                               int pairs) {
+  fwd_x_body<T, FROM_VEL>(d, mt, u, v, w, rin, nz_in, scal, fx, twx, S, pairs, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // P2 / P4: complex FFT along y for every (kx, z) column.  Block = (kx, chunk of KZ z-rows).
 template <typename T>
-__global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDesc fy,
+__device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, FftDesc fy,
                                                const double2* __restrict__ twy,
-                                               cplx<T>* __restrict__ S, int KZ, int sign) {
+                                               cplx<T>* __restrict__ S, int KZ, int sign,
+    int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem_raw);
   const int nch = (nz_in + KZ - 1) / KZ;
-  const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int m = blk / nch, k0 = (blk - m * nch) * KZ;
   const int ldb = ny + 1;
   for (int t = threadIdx.x; t < ny * KZ; t += 256) {
@@ -101,19 +112,26 @@ __global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDe
   }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDesc fy,
+                                               const double2* __restrict__ twy,
+                                               cplx<T>* __restrict__ S, int KZ, int sign) {
+  fft_y_body<T>(nkx, ny, nz_in, fy, twy, S, KZ, sign, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // P5: half spectra -> real rows (irfft semantics: the imaginary parts of the kx = 0 and Nyquist
 // bins are dropped), scaled by 1/(nx*ny), written to p with the ghost images of
 // projection_fft.py:150-163 (x copies, y copies, Neumann z walls).
 template <typename T>
-__global__ __launch_bounds__(256) void k_inv_x(Dims d, FftDesc fx, const double2* __restrict__ twx,
+__device__ __forceinline__ void inv_x_body(Dims d, FftDesc fx, const double2* __restrict__ twx,
                                                const cplx<T>* __restrict__ S, T* __restrict__ p,
-                                               T scale, int pairs) {
+                                               T scale, int pairs,
+    int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
   const int nx = d.nx, ny = d.ny, nz = d.nz;
   const int KZ = 2 * pairs;
   const int nch = (nz + KZ - 1) / KZ;
-  const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int j = blk / nch, k0 = (blk - j * nch) * KZ;
   const int ldb = nx + 1;
   const int h = nx / 2;
@@ -150,6 +168,13 @@ __global__ __launch_bounds__(256) void k_inv_x(Dims d, FftDesc fx, const double2
       for (int bq = 0; bq < nj; ++bq)
         for (int c = 0; c < nk; ++c) p[xs[a] * d.sxp + ys[bq] * d.syp + zs[c]] = val;
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_x(Dims d, FftDesc fx, const double2* __restrict__ twx,
+                                               const cplx<T>* __restrict__ S, T* __restrict__ p,
+                                               T scale, int pairs) {
+  inv_x_body<T>(d, fx, twx, S, p, scale, pairs, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -213,14 +238,15 @@ __device__ __forceinline__ void pcr_general(U (&a)[R], U (&b)[R], U (&c)[R], U (
 // (one contiguous chunk of S) in LDS; lane l of wave 0 sweeps system s0 + l.  The singular
 // (0,0) system is gauge-fixed (first unknown pinned to zero) and swept in float64.
 template <typename T>
-__global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
+__device__ __forceinline__ void thomas_body(int nsys, int nz, int sb, const double* __restrict__ cl,
                                                         const T* __restrict__ cp, const T* __restrict__ iden,
                                                         const double* __restrict__ cp0,
                                                         const double* __restrict__ iden0,
-                                                        cplx<T>* __restrict__ S) {
+                                                        cplx<T>* __restrict__ S,
+    int blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cplx<T>* col = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int s0 = xcd_remap(blockIdx.x, gridDim.x) * sb;
+  const int s0 = blk * sb;
   const int ns = min(sb, nsys - s0);
   const int ld = nz + 1;
   const cplx<T>* src = S + (size_t)s0 * nz;
@@ -313,6 +339,56 @@ __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb
     dst[e] = col[sl * ld + k];
   }
 }
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
+                                                        const T* __restrict__ cp, const T* __restrict__ iden,
+                                                        const double* __restrict__ cp0,
+                                                        const double* __restrict__ iden0,
+                                                        cplx<T>* __restrict__ S) {
+  thomas_body<T>(nsys, nz, sb, cl, cp, iden, cp0, iden0, S, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Ensemble forms of P1-P5: global block -> (channel, block of that channel) via prefix sums.
+template <typename T>
+__global__ __launch_bounds__(256) void k_fwd_x_ens(const EnsChan<T>* __restrict__ ch, const int* __restrict__ start,
+                                                   int n) {
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = ens_chan(start, n, gb);
+  const EnsChan<T>& c = ch[q];
+  fwd_x_body<T, true>(c.d, c.mt, c.su, c.sv, c.sw, nullptr, c.d.nz, c.scal, c.fx, c.twx, c.spec, c.pairs,
+                      gb - start[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fft_y_ens(const EnsChan<T>* __restrict__ ch, const int* __restrict__ start,
+                                                   int n, int sign) {
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = ens_chan(start, n, gb);
+  const EnsChan<T>& c = ch[q];
+  fft_y_body<T>(c.nkx, c.d.ny, c.d.nz, c.fy, c.twy, c.spec, c.kzy, sign, gb - start[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_poisson_thomas_ens(const EnsChan<T>* __restrict__ ch,
+                                                            const int* __restrict__ start, int n) {
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = ens_chan(start, n, gb);
+  const EnsChan<T>& c = ch[q];
+  thomas_body<T>(c.nkx * c.d.ny, c.d.nz, c.sb, c.cl, c.th_cp, c.th_iden, c.th_cp0, c.th_iden0, c.spec,
+                 gb - start[q]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_x_ens(const EnsChan<T>* __restrict__ ch, const int* __restrict__ start,
+                                                   int n) {
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = ens_chan(start, n, gb);
+  const EnsChan<T>& c = ch[q];
+  const T scale = (T)(1.0 / ((double)c.d.nx * (double)c.d.ny));
+  inv_x_body<T>(c.d, c.fx, c.twx, c.spec, c.p, scale, c.pairs, gb - start[q]);
+}
+
 
 // Generic batched PCR for the stand-alone tridiagonal entry points.
 template <typename T, int R, bool CPLX>
@@ -666,6 +742,52 @@ int pcr_batched(int n, int batch, int dtype, int cplx_d, const void* a, const vo
                   : pcr_dispatch_r<double, false>(n, batch, a, b, c, s, d, x, st);
   return cplx_d ? pcr_dispatch_r<float, true>(n, batch, a, b, c, s, d, x, st)
                 : pcr_dispatch_r<float, false>(n, batch, a, b, c, s, d, x, st);
+}
+
+}  // namespace tc
+
+namespace tc {
+
+// Geometry of one channel's Poisson launches (block counts and dynamic LDS), shared by the
+// single-channel launchers above and the ensemble.
+void poisson_geometry(const Poisson* pl, int tsz, int* blocks, size_t* lds) {
+  const Dims& d = pl->g->d;
+  const size_t csz = 2 * (size_t)tsz;
+  const int KZ = 2 * pl->pairs;
+  const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : pl->sb;
+  blocks[0] = d.ny * ((d.nz + KZ - 1) / KZ);
+  lds[0] = csz * (size_t)pl->pairs * (d.nx + 1);
+  blocks[1] = pl->nkx * ((d.nz + pl->kzy - 1) / pl->kzy);
+  lds[1] = csz * (size_t)pl->kzy * (d.ny + 1);
+  blocks[2] = (pl->nkx * d.ny + sb - 1) / sb;
+  lds[2] = csz * (size_t)sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
+  blocks[3] = blocks[0];
+  lds[3] = lds[0];
+}
+
+template <typename T>
+static int ens_poisson_t(const void* dch, int n, const int* const* start, const int* blocks, const size_t* lds,
+                         hipStream_t st) {
+  const EnsChan<T>* c = (const EnsChan<T>*)dch;
+  hipLaunchKernelGGL((k_fwd_x_ens<T>), dim3(blocks[0]), dim3(256), lds[0], st, c, start[0], n);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_fft_y_ens<T>), dim3(blocks[1]), dim3(256), lds[1], st, c, start[1], n, -1);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_poisson_thomas_ens<T>), dim3(blocks[2]), dim3(256), lds[2], st, c, start[2], n);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_fft_y_ens<T>), dim3(blocks[1]), dim3(256), lds[1], st, c, start[1], n, +1);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_inv_x_ens<T>), dim3(blocks[3]), dim3(256), lds[3], st, c, start[3], n);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int ens_launch_poisson(int dtype, const void* dch, int n, const int* const* start, const int* blocks,
+                       const size_t* lds, hipStream_t st) {
+  for (int q = 0; q < 4; ++q)
+    if (lds[q] > 160 * 1024) return TC_ERR_ARG;
+  if (dtype == TC_F64) return ens_poisson_t<double>(dch, n, start, blocks, lds, st);
+  return ens_poisson_t<float>(dch, n, start, blocks, lds, st);
 }
 
 }  // namespace tc

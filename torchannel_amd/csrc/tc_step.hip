@@ -5,6 +5,7 @@
 // u, v and w columns at the same (i, j) -- which read each other's neighbours -- run together on
 // the same XCD and share its L2.  Rows of a column sit in registers (row m = r*64 + lane) and the
 // wall-normal tridiagonal solve is a wavefront-level parallel cyclic reduction with lane shuffles.
+#include "tc_ensemble.h"
 #include "tc_internal.h"
 
 namespace tc {
@@ -14,15 +15,13 @@ namespace tc {
 // the PCR recurrence of tridiag.py:40-56 once per dt.  Block 0: cell rows (u, v; n = nz);
 // block 1: face rows (w; n = nz-1).  One thread per row, neighbours through LDS.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(TC_MAX_NZ) void k_cn_tables(const double* __restrict__ dzf,
-                                                          const double* __restrict__ dzc, int nz,
-                                                          int top, double nu, double theta,
-                                                          double* __restrict__ scal, double dt_set,
-                                                          double* __restrict__ tab_cell,
-                                                          double* __restrict__ tab_face) {
+__device__ __forceinline__ void cn_tables_body(const double* __restrict__ dzf, const double* __restrict__ dzc,
+                                               int nz, int top, double nu, double theta,
+                                               double* __restrict__ scal, double dt_set,
+                                               double* __restrict__ tab_cell, double* __restrict__ tab_face,
+                                               bool face) {
   __shared__ double sa[TC_MAX_NZ], sb[TC_MAX_NZ], sc[TC_MAX_NZ];
   const int m = threadIdx.x;
-  const bool face = blockIdx.x == 1;
   const int n = face ? nz - 1 : nz;
   double* tab = face ? tab_face : tab_cell;
   const double dt = dt_set > 0.0 ? dt_set : scal[TC_S_DT];
@@ -66,7 +65,24 @@ __global__ __launch_bounds__(TC_MAX_NZ) void k_cn_tables(const double* __restric
     tab[(2 * st + 1) * TC_MAX_NZ + m] = ga;
   }
   tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m] = m < n ? b : 1.0;
-  if (dt_set > 0.0 && blockIdx.x == 0 && m == 0) scal[TC_S_DT] = dt_set;
+  if (dt_set > 0.0 && !face && m == 0) scal[TC_S_DT] = dt_set;
+}
+
+__global__ __launch_bounds__(TC_MAX_NZ) void k_cn_tables(const double* __restrict__ dzf,
+                                                          const double* __restrict__ dzc, int nz,
+                                                          int top, double nu, double theta,
+                                                          double* __restrict__ scal, double dt_set,
+                                                          double* __restrict__ tab_cell,
+                                                          double* __restrict__ tab_face) {
+  cn_tables_body(dzf, dzc, nz, top, nu, theta, scal, dt_set, tab_cell, tab_face, blockIdx.x == 1);
+}
+
+// Ensemble form: blocks 2c, 2c+1 build channel c's cell and face tables.
+template <typename T>
+__global__ __launch_bounds__(TC_MAX_NZ) void k_cn_tables_ens(const EnsChan<T>* __restrict__ ch, double dt_set) {
+  const EnsChan<T>& c = ch[blockIdx.x >> 1];
+  cn_tables_body(c.mt.dzf, c.mt.dzc, c.d.nz, c.d.top, c.nu, c.theta, c.scal, dt_set, c.tabs,
+                 c.tabs + TC_TAB_STRIDE, (blockIdx.x & 1) != 0);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -632,15 +648,15 @@ __host__ __device__ __forceinline__ int k1_bands_i(int nx) { return (nx + 2 + TC
 __host__ __device__ __forceinline__ int k1_bands_j(int ny, int jb) { return (ny + 2 + jb - 1) / jb; }
 
 template <typename T, int R>
-__global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
-    Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+__device__ __forceinline__ void explicit_cn_body(
+    const Dims& d, const Metric<T>& mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
     T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
-    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
+    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb,
+    int b) {
   __shared__ T lds_tab[(2 * TC_MAX_STAGES + 1) * TC_WAVE * R];
   __shared__ T lds_rows[TC_K1_WAVES][3 * TC_WAVE * R];
-  const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int comp = b % 3, band = b / 3;
   const int nbj = k1_bands_j(d.ny, jb);
   const int bi = band / nbj, bj = band - bi * nbj;
@@ -677,6 +693,34 @@ __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
   if (comp == 0) k1_pencil_u<T, R>(c, i, j0, j1, u, v, w, pu, ru, su);
   else if (comp == 1) k1_pencil_v<T, R>(c, i, j0, j1, u, v, w, pv, rv, sv);
   else k1_pencil_w<T, R>(c, i, j0, j1, u, v, w, pw, rw, sw);
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
+    Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+    const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
+    T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
+    const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
+    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
+  explicit_cn_body<T, R>(d, mt, u, v, w, pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body, mode,
+                         branch, theta, jb, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Ensemble form over the channels chans[0..n) (all of row count R); start = prefix sums of
+// their block counts.  rhs set `cur` receives the new rhs, `prev` holds the previous one.
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn_ens(const EnsChan<T>* __restrict__ ch,
+                                                                      const int* __restrict__ chans,
+                                                                      const int* __restrict__ start, int n,
+                                                                      int cur, int prev, int mode, int branch) {
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = ens_chan(start, n, gb);
+  const EnsChan<T>& c = ch[chans[q]];
+  T* const* rc = c.rs[cur];
+  T* const* rp = c.rs[prev < 0 ? cur : prev];
+  explicit_cn_body<T, R>(c.d, c.mt, c.u, c.v, c.w, rp[0], rp[1], rp[2], rc[0], rc[1], rc[2], c.su, c.sv, c.sw,
+                         c.tabs, c.tabs + TC_TAB_STRIDE, c.scal, c.body, mode, branch, c.theta, c.jb,
+                         gb - start[q]);
 }
 
 // Stand-alone CN solve of one component (solve_implicit_diffusion_u/v/w, operators.py:563-811):
@@ -728,13 +772,11 @@ __global__ __launch_bounds__(256) void k_cn_only(Dims d, Metric<T> mt, int comp,
 // ---------------------------------------------------------------------------------------------
 #define TC_K5_WAVES 16
 template <typename T, int R>
-__global__ __launch_bounds__(64 * TC_K5_WAVES) void k_project_bc(
-    Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
+__device__ __forceinline__ void project_bc_body(
+    const Dims& d, const Metric<T>& mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
-    double* __restrict__ scal, double* __restrict__ partials, int flags) {
+    double* __restrict__ scal, double* __restrict__ partials, int flags, int b) {
   __shared__ double red[TC_K5_WAVES];
-  const int nb = gridDim.x;
-  const int b = xcd_remap(blockIdx.x, nb);
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int g = b * TC_K5_WAVES + wv;
   const int lane = lane_id();
@@ -798,11 +840,29 @@ __global__ __launch_bounds__(64 * TC_K5_WAVES) void k_project_bc(
   }
 }
 
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_K5_WAVES) void k_project_bc(
+    Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
+    const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+    double* __restrict__ scal, double* __restrict__ partials, int flags) {
+  project_bc_body<T, R>(d, mt, su, sv, sw, p, u, v, w, scal, partials, flags, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_K5_WAVES) void k_project_bc_ens(const EnsChan<T>* __restrict__ ch,
+                                                                     const int* __restrict__ chans,
+                                                                     const int* __restrict__ start, int n,
+                                                                     int flags) {
+  const int gb = xcd_remap(blockIdx.x, gridDim.x);
+  const int q = ens_chan(start, n, gb);
+  const EnsChan<T>& c = ch[chans[q]];
+  project_bc_body<T, R>(c.d, c.mt, c.su, c.sv, c.sw, c.p, c.u, c.v, c.w, c.scal, c.partials, flags, gb - start[q]);
+}
+
 // Folds the per-block bulk partials of k_project_bc in a fixed order (one block), then the
 // forcing controller F += 0.1*(U_b - u_b)/dt (solver.py:701-708), time and step count.
-__global__ __launch_bounds__(1024) void k_step_finalize(const double* __restrict__ partials, int nb,
-                                                        double* __restrict__ scal, double inv_volume,
-                                                        int flags) {
+__device__ __forceinline__ void step_finalize_body(const double* __restrict__ partials, int nb,
+                                                   double* __restrict__ scal, double inv_volume, int flags) {
   __shared__ double acc[1024];
   double s = 0.0;
   const int t = threadIdx.x;
@@ -828,6 +888,19 @@ __global__ __launch_bounds__(1024) void k_step_finalize(const double* __restrict
       scal[TC_S_STEP] = scal[TC_S_STEP] + 1.0;
     }
   }
+}
+
+__global__ __launch_bounds__(1024) void k_step_finalize(const double* __restrict__ partials, int nb,
+                                                        double* __restrict__ scal, double inv_volume,
+                                                        int flags) {
+  step_finalize_body(partials, nb, scal, inv_volume, flags);
+}
+
+// Ensemble form: block c folds channel c.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_step_finalize_ens(const EnsChan<T>* __restrict__ ch, int flags) {
+  const EnsChan<T>& c = ch[blockIdx.x];
+  step_finalize_body(c.partials, c.k5_blocks, c.scal, c.inv_volume, flags);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -945,6 +1018,90 @@ int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, co
   if (gr->dtype == TC_F64)
     return launch_k5_t<double>(gr, su, sv, sw, p, u, v, w, scal, partials, counter, inv_volume, flags, st);
   return launch_k5_t<float>(gr, su, sv, sw, p, u, v, w, scal, partials, counter, inv_volume, flags, st);
+}
+
+}  // namespace tc
+
+// ---------------------------------------------------------------------------------------------
+// Ensemble launchers (one launch per phase over every channel of the batch)
+// ---------------------------------------------------------------------------------------------
+namespace tc {
+
+int ens_launch_tables(int dtype, const void* dch, int n, double dt_set, hipStream_t st) {
+  if (dtype == TC_F64)
+    hipLaunchKernelGGL((k_cn_tables_ens<double>), dim3(2 * n), dim3(TC_MAX_NZ), 0, st,
+                       (const EnsChan<double>*)dch, dt_set);
+  else
+    hipLaunchKernelGGL((k_cn_tables_ens<float>), dim3(2 * n), dim3(TC_MAX_NZ), 0, st,
+                       (const EnsChan<float>*)dch, dt_set);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+template <typename T>
+static int ens_k1_t(int R, const void* dch, const int* chans, const int* start, int ng, int blocks, int cur,
+                    int prev, int mode, int branch, hipStream_t st) {
+  const EnsChan<T>* c = (const EnsChan<T>*)dch;
+#define TC_EK1(RR)                                                                                         \
+  hipLaunchKernelGGL((k_explicit_cn_ens<T, RR>), dim3(blocks), dim3(64 * TC_K1_WAVES), 0, st, c, chans,   \
+                     start, ng, cur, prev, mode, branch)
+  switch (R) {
+    case 1: TC_EK1(1); break;
+    case 2: TC_EK1(2); break;
+    case 3: TC_EK1(3); break;
+    case 4: TC_EK1(4); break;
+    default: return TC_ERR_UNSUPPORTED;
+  }
+#undef TC_EK1
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int ens_launch_k1(int dtype, int R, const void* dch, const int* chans, const int* start, int ng, int blocks,
+                  int cur, int prev, int mode, int branch, hipStream_t st) {
+  if (dtype == TC_F64) return ens_k1_t<double>(R, dch, chans, start, ng, blocks, cur, prev, mode, branch, st);
+  return ens_k1_t<float>(R, dch, chans, start, ng, blocks, cur, prev, mode, branch, st);
+}
+
+template <typename T>
+static int ens_k5_t(int R, const void* dch, const int* chans, const int* start, int ng, int blocks, int flags,
+                    hipStream_t st) {
+  const EnsChan<T>* c = (const EnsChan<T>*)dch;
+#define TC_EK5(RR)                                                                                         \
+  hipLaunchKernelGGL((k_project_bc_ens<T, RR>), dim3(blocks), dim3(64 * TC_K5_WAVES), 0, st, c, chans,    \
+                     start, ng, flags)
+  switch (R) {
+    case 1: TC_EK5(1); break;
+    case 2: TC_EK5(2); break;
+    case 3: TC_EK5(3); break;
+    case 4: TC_EK5(4); break;
+    default: return TC_ERR_UNSUPPORTED;
+  }
+#undef TC_EK5
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int ens_launch_k5(int dtype, int R, const void* dch, const int* chans, const int* start, int ng, int blocks,
+                  int flags, hipStream_t st) {
+  if (dtype == TC_F64) return ens_k5_t<double>(R, dch, chans, start, ng, blocks, flags, st);
+  return ens_k5_t<float>(R, dch, chans, start, ng, blocks, flags, st);
+}
+
+int ens_launch_finalize(int dtype, const void* dch, int n, int flags, hipStream_t st) {
+  if (dtype == TC_F64)
+    hipLaunchKernelGGL((k_step_finalize_ens<double>), dim3(n), dim3(1024), 0, st, (const EnsChan<double>*)dch,
+                       flags);
+  else
+    hipLaunchKernelGGL((k_step_finalize_ens<float>), dim3(n), dim3(1024), 0, st, (const EnsChan<float>*)dch,
+                       flags);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int k1_blocks(const Grid* g) {
+  const int jb = g->k1_jb > 0 ? g->k1_jb : 8;
+  return 3 * k1_bands_i(g->d.nx) * k1_bands_j(g->d.ny, jb);
 }
 
 }  // namespace tc

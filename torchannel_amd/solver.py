@@ -43,8 +43,9 @@ class ChannelFlow:
     output, statistics, boundary_conditions) plus ``compute.precision`` ('fp64' | 'fp32') and
     ``initialization.seed`` (torch.manual_seed on the CPU generator before the noise draws) and
     ``compute.rhs_branch`` ('gpu' = the fused RHS the reference runs on any HIP/CUDA device,
-    'cpu' = the separate kernels of its CPU branch, solver.py:593-622) and ``compute.autotune``
-    (pick the kernels' launch shapes by timing after the first step; results do not change).
+    'cpu' = the separate kernels of its CPU branch, solver.py:593-622), ``compute.autotune``
+    (pick the kernels' launch shapes by timing after the first step; results do not change) and
+    ``compute.private_plan`` (a Poisson workspace of its own, for channels stepped together).
     ``config`` may be given as a dict instead of a file path.
     """
 
@@ -128,7 +129,9 @@ class ChannelFlow:
         self._grid = L.get_grid(self.nx, self.ny, self.nz, self.dx, self.dy, self.nu,
                                 L.top_code(self.top_wall_bc_type), self.wdtype, self.device,
                                 dz_f=self.dz_f, dz_c=self.dz_c)
-        self._plan = self._grid.poisson()
+        # a plan of its own when the channel is stepped together with others (ensembles)
+        self._own_plan = bool(config.get("compute", {}).get("private_plan", False))
+        self._plan = self._grid.new_poisson() if self._own_plan else self._grid.poisson()
 
         st = config.get("statistics", {})
         self.n_stats = st.get("n_stats", 0)
@@ -176,6 +179,8 @@ class ChannelFlow:
         try:
             if getattr(self, "_solver", None) is not None:
                 L.load().tc_solver_destroy(self._solver)
+            if getattr(self, "_own_plan", False) and getattr(self, "_plan", None) is not None:
+                L.Grid.destroy_poisson(self._plan)
         except Exception:
             pass
 
