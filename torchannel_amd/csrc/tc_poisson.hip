@@ -355,7 +355,7 @@ __global__ __launch_bounds__(256) void k_fwd_x_ens(const EnsChan<T>* __restrict_
                                                    int n) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
-  const EnsChan<T>& c = ch[q];
+  const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
   fwd_x_body<T, true>(c.d, c.mt, c.su, c.sv, c.sw, nullptr, c.d.nz, c.scal, c.fx, c.twx, c.spec, c.pairs,
                       gb - start[q]);
 }
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(256) void k_fft_y_ens(const EnsChan<T>* __restrict_
                                                    int n, int sign) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
-  const EnsChan<T>& c = ch[q];
+  const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
   fft_y_body<T>(c.nkx, c.d.ny, c.d.nz, c.fy, c.twy, c.spec, c.kzy, sign, gb - start[q]);
 }
 
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(256) void k_poisson_thomas_ens(const EnsChan<T>* __
                                                             const int* __restrict__ start, int n) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
-  const EnsChan<T>& c = ch[q];
+  const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
   thomas_body<T>(c.nkx * c.d.ny, c.d.nz, c.sb, c.cl, c.th_cp, c.th_iden, c.th_cp0, c.th_iden0, c.spec,
                  gb - start[q]);
 }
@@ -384,7 +384,7 @@ __global__ __launch_bounds__(256) void k_inv_x_ens(const EnsChan<T>* __restrict_
                                                    int n) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
-  const EnsChan<T>& c = ch[q];
+  const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
   const T scale = (T)(1.0 / ((double)c.d.nx * (double)c.d.ny));
   inv_x_body<T>(c.d, c.fx, c.twx, c.spec, c.p, scale, c.pairs, gb - start[q]);
 }

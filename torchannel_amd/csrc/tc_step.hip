@@ -649,7 +649,7 @@ __host__ __device__ __forceinline__ int k1_bands_j(int ny, int jb) { return (ny 
 
 template <typename T, int R>
 __device__ __forceinline__ void explicit_cn_body(
-    const Dims& d, const Metric<T>& mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+    Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
     T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn_ens(const EnsC
                                                                       int cur, int prev, int mode, int branch) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
-  const EnsChan<T>& c = ch[chans[q]];
+  const EnsChan<T>& c = ch[chans[q]];   // fields are passed to the body by value
   T* const* rc = c.rs[cur];
   T* const* rp = c.rs[prev < 0 ? cur : prev];
   explicit_cn_body<T, R>(c.d, c.mt, c.u, c.v, c.w, rp[0], rp[1], rp[2], rc[0], rc[1], rc[2], c.su, c.sv, c.sw,
@@ -773,7 +773,7 @@ __global__ __launch_bounds__(256) void k_cn_only(Dims d, Metric<T> mt, int comp,
 #define TC_K5_WAVES 16
 template <typename T, int R>
 __device__ __forceinline__ void project_bc_body(
-    const Dims& d, const Metric<T>& mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
+    Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
     const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
     double* __restrict__ scal, double* __restrict__ partials, int flags, int b) {
   __shared__ double red[TC_K5_WAVES];
@@ -855,7 +855,7 @@ __global__ __launch_bounds__(64 * TC_K5_WAVES) void k_project_bc_ens(const EnsCh
                                                                      int flags) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
-  const EnsChan<T>& c = ch[chans[q]];
+  const EnsChan<T>& c = ch[chans[q]];   // fields are passed to the body by value
   project_bc_body<T, R>(c.d, c.mt, c.su, c.sv, c.sw, c.p, c.u, c.v, c.w, c.scal, c.partials, flags, gb - start[q]);
 }
 
