@@ -768,41 +768,68 @@ __global__ __launch_bounds__(256) void k_cn_only(Dims d, Metric<T> mt, int comp,
 // ---------------------------------------------------------------------------------------------
 // K5: q = q* - dt*grad(p) (projection.py:154-163) written with every ghost set to its periodic
 // image / wall reflection (apply_bc_all, solver.py:43-79), plus the per-block partial sums of the
-// bulk velocity (utils.py:220-223), folded by k_step_finalize.  16 wavefronts per block.
+// bulk velocity (utils.py:220-223), folded by k_step_finalize.
+// Work decomposition as in K1: a block of TC_K5_WAVES wavefronts owns one component and a band of
+// TC_K5_WAVES consecutive i; each wavefront walks TC_K5_JB consecutive j columns (ghost columns
+// included, each reading its periodic source) with the next column's loads issued before the
+// current one is written, so a wavefront keeps two columns of loads in flight.  The walk is
+// specialised per component (its pointers are computed on straight-line paths).
 // ---------------------------------------------------------------------------------------------
-#define TC_K5_WAVES 16
+#define TC_K5_WAVES 4
+#define TC_K5_JB 8
+__host__ __device__ __forceinline__ int k5_bands_i(int nx) { return (nx + 2 + TC_K5_WAVES - 1) / TC_K5_WAVES; }
+__host__ __device__ __forceinline__ int k5_bands_j(int ny) { return (ny + 2 + TC_K5_JB - 1) / TC_K5_JB; }
+
 template <typename T, int R>
-__device__ __forceinline__ void project_bc_body(
-    Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
-    const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
-    double* __restrict__ scal, double* __restrict__ partials, int flags, int b) {
-  __shared__ double red[TC_K5_WAVES];
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int g = b * TC_K5_WAVES + wv;
-  const int lane = lane_id();
-  int comp, i, j;
-  const bool in_grid = tiled_column(g, d.nx + 2, d.ny + 2, comp, i, j);
-  const T dt = (T)scal[TC_S_DT];
-  double part = 0.0;
-  bool bad = false;
-  const bool live = in_grid && !(comp == 0 && i > d.nx) && !(comp == 1 && j > d.ny);
-  if (live) {
-    const int si = (i == 0) ? d.nx : (i == d.nx + 1 ? 1 : i);
-    const int sj = (j == 0) ? d.ny : (j == d.ny + 1 ? 1 : j);
-    const T* pc = p + (long long)si * d.sxp + (long long)sj * d.syp;
-    if (comp < 2) {
-      const T* s = comp == 0 ? su + (long long)si * d.sxu + (long long)sj * d.syu
-                             : sv + (long long)si * d.sxv + (long long)sj * d.syv;
-      T* o = comp == 0 ? u + (long long)i * d.sxu + (long long)j * d.syu
-                       : v + (long long)i * d.sxv + (long long)j * d.syv;
-      const T* pn = pc + (comp == 0 ? d.sxp : d.syp);
-      const T rd = comp == 0 ? mt.rdx : mt.rdy;
-      const bool bulk = comp == 0 && i >= 1 && j >= 1 && j <= d.ny;
+struct K5In {
+  T s[R], a[R], b[R];      // star column; u, v: p and its +x / +y neighbour; w: p[k] and p[k+1]
+};
+
+// Loads of column (i, j) of component COMP: the star column and the two pressure columns the
+// gradient needs, from the periodic source (si, sj) of a ghost column.
+template <typename T, int R, int COMP>
+__device__ __forceinline__ void k5_fetch(const Dims& d, int i, int j, const T* __restrict__ s,
+                                         const T* __restrict__ p, int lane, K5In<T, R>& L) {
+  const int si = (i == 0) ? d.nx : (i == d.nx + 1 ? 1 : i);
+  const int sj = (j == 0) ? d.ny : (j == d.ny + 1 ? 1 : j);
+  const T* pc = p + (long long)si * d.sxp + (long long)sj * d.syp;
+  constexpr bool W = COMP == 2;
+  const int n = W ? d.nz - 1 : d.nz;
+  const long long sx = COMP == 0 ? d.sxu : (COMP == 1 ? d.sxv : d.sxw);
+  const long long sy = COMP == 0 ? d.syu : (COMP == 1 ? d.syv : d.syw);
+  const long long po = COMP == 0 ? d.sxp : (COMP == 1 ? d.syp : 1);
+  const T* sc = s + si * sx + sj * sy;
+  const T* pb = pc + po;
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    const bool ok = (int)k <= n;
+    L.s[r] = ok ? sc[k] : T(0);
+    L.a[r] = ok ? pc[k] : T(0);
+    L.b[r] = ok ? pb[k] : T(0);
+  }
+}
+
+template <typename T, int R, int COMP>
+__device__ __forceinline__ void k5_pencil(const Dims& d, const Metric<T>& mt, int i, int j0, int j1,
+                                          const T* __restrict__ s, const T* __restrict__ p, T* __restrict__ q,
+                                          T dt, int lane, double& part, bool& bad) {
+  const T rd = COMP == 0 ? mt.rdx : mt.rdy;
+  const long long sx = COMP == 0 ? d.sxu : (COMP == 1 ? d.sxv : d.sxw);
+  const long long sy = COMP == 0 ? d.syu : (COMP == 1 ? d.syv : d.syw);
+  K5In<T, R> A, B;
+  k5_fetch<T, R, COMP>(d, i, j0, s, p, lane, A);
+  for (int j = j0; j < j1; ++j) {
+    const bool more = j + 1 < j1;
+    if (more) k5_fetch<T, R, COMP>(d, i, j + 1, s, p, lane, B);
+    T* o = q + i * sx + j * sy;
+    if constexpr (COMP < 2) {
+      const bool bulk = COMP == 0 && i >= 1 && j >= 1 && j <= d.ny;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
         if ((int)k <= d.nz) {
-          const T val = s[k] - dt * ((pn[k] - pc[k]) * rd);
+          const T val = A.s[r] - dt * ((A.b[r] - A.a[r]) * rd);
           bad |= !isfinite(val);
           o[k] = val;
           if (k == 1) o[0] = -val;
@@ -811,19 +838,44 @@ __device__ __forceinline__ void project_bc_body(
         }
       }
     } else {
-      const T* s = sw + (long long)si * d.sxw + (long long)sj * d.syw;
-      T* o = w + (long long)i * d.sxw + (long long)j * d.syw;
 #pragma unroll
       for (int r = 0; r < R; ++r) {
         const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
         if ((int)k < d.nz) {
-          const T val = s[k] - dt * ((pc[k + 1] - pc[k]) * mt.rdzc[k]);
+          const T val = A.s[r] - dt * ((A.b[r] - A.a[r]) * mt.rdzc[k]);
           bad |= !isfinite(val);
           o[k] = val;
         }
         if (k == 1) { o[0] = T(0); o[d.nz] = T(0); }
       }
     }
+    if (more) A = B;
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void project_bc_body(
+    Dims d, Metric<T> mt, const T* __restrict__ su, const T* __restrict__ sv, const T* __restrict__ sw,
+    const T* __restrict__ p, T* __restrict__ u, T* __restrict__ v, T* __restrict__ w,
+    double* __restrict__ scal, double* __restrict__ partials, int flags, int b) {
+  __shared__ double red[TC_K5_WAVES];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int comp = b % 3, band = b / 3;
+  const int nbj = k5_bands_j(d.ny);
+  const int bi = band / nbj, bj = band - bi * nbj;
+  const int i = bi * TC_K5_WAVES + wv;
+  const int imax = comp == 0 ? d.nx : d.nx + 1;
+  const int jmax = comp == 1 ? d.ny : d.ny + 1;
+  const int j0 = bj * TC_K5_JB;
+  const int j1 = min(j0 + TC_K5_JB, jmax + 1);
+  const T dt = (T)scal[TC_S_DT];
+  double part = 0.0;
+  bool bad = false;
+  if (i <= imax && j0 <= jmax) {
+    if (comp == 0) k5_pencil<T, R, 0>(d, mt, i, j0, j1, su, p, u, dt, lane, part, bad);
+    else if (comp == 1) k5_pencil<T, R, 1>(d, mt, i, j0, j1, sv, p, v, dt, lane, part, bad);
+    else k5_pencil<T, R, 2>(d, mt, i, j0, j1, sw, p, w, dt, lane, part, bad);
   }
   // device-side non-finite flag (the reference scans u, v, w, p on the host at save points,
   // solver.py:862-867); every writer stores the same value, so no atomic is needed
@@ -986,8 +1038,7 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
                        void* u, void* v, void* w, double* scal, double* partials, unsigned* counter,
                        double inv_volume, int flags, hipStream_t st) {
   const Dims& d = gr->d;
-  const int waves = tiled_waves(d.nx + 2, d.ny + 2);
-  const int blocks = (waves + TC_K5_WAVES - 1) / TC_K5_WAVES;
+  const int blocks = k5_blocks(gr);
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
 #define TC_K5(RR)                                                                                \
@@ -1010,7 +1061,7 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
   return TC_OK;
 }
 
-int k5_blocks(const Grid* gr) { return (tiled_waves(gr->d.nx + 2, gr->d.ny + 2) + TC_K5_WAVES - 1) / TC_K5_WAVES; }
+int k5_blocks(const Grid* gr) { return 3 * k5_bands_i(gr->d.nx) * k5_bands_j(gr->d.ny); }
 
 int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p, void* u,
               void* v, void* w, double* scal, double* partials, unsigned* counter, double inv_volume,
