@@ -178,6 +178,60 @@ def run_adjoint(n_steps=100, ckpt=10, reps=3):
             "adjoint_over_forward": round(ad / fw, 3)}
 
 
+def run_c3(steps=500, warmup=20, phase_reps=10):
+    """Re_tau~550 single channel: 384x384x256, Lx=2pi, Ly=pi, gamma=2.6, Re_bulk=10000, adaptive
+    dt at CFL 0.25 (every 10 steps), float32 (float64 gauge mode), seeded perturbed parabola;
+    500 steps as graph chunks of 10, then per-kernel average durations and algorithmic GB/s
+    (the wall-normal Poisson solve is one Thomas sweep per (kx, ky) system: 193 x 384 systems of
+    length 256 per launch)."""
+    import torch
+    import torchannel_amd as tc
+    from torchannel_amd import _lib as L
+
+    nx, ny, nz = 384, 384, 256
+    c = config("fp32", nx, ny, nz, steps=steps)
+    c["domain"] = {"Lx": 2 * math.pi, "Ly": math.pi, "Lz": 2.0}
+    c["flow"]["Re"] = 10000.0
+    c["flow"]["Re_tau"] = 550.0
+    c["statistics"] = {"n_stats": 0}
+    ch = tc.ChannelFlow(config=c)
+    chunk = 10
+    # the first dt from the CFL rule (a white-noise start at this resolution is far above CFL 0.25
+    # at the nominal dt = 0.005), then the regular cadence
+    ch.update_dt_device()
+    for s_ in range(warmup):
+        if (s_ + 1) % 10 == 0:
+            ch.update_dt_device()
+        ch.step_imex(None)
+    ch.run_graph(0, chunk, dt_update_interval=10)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    ch.run_graph(steps // chunk, chunk, dt_update_interval=10)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / ((steps // chunk) * chunk)
+    n_cells = nx * ny * nz
+    words = algorithmic_words(nx, ny, nz)
+    per = {}
+    for i, name in enumerate(PHASES):
+        L.call("tc_solver_phase", ch._solver, i, 2, ch._stream())
+        torch.cuda.synchronize()
+        e0.record()
+        L.call("tc_solver_phase", ch._solver, i, phase_reps, ch._stream())
+        e1.record()
+        torch.cuda.synchronize()
+        avg = e0.elapsed_time(e1) / phase_reps
+        byts = words[name] * 4 * n_cells
+        per[name] = {"avg_us": round(avg * 1e3, 2), "gbs": round(byts / (avg * 1e-3) / 1e9, 1) if byts else 0.0}
+    finite = float(ch._scal[L.S_NONFINITE]) == 0.0
+    return {"workload": "384x384x256, Lx=2pi, Ly=pi, Re_bulk=10000, float32, adaptive dt CFL 0.25 (first dt "
+                        f"from the CFL rule), {(steps // chunk) * chunk} steps", "finite": finite,
+            "value": round(n_cells / (ms * 1e-3) / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(ms, 4),
+            "dt_final": ch.device_dt(), "u_bulk": float(ch._scal[L.S_UBULK]), "kernels": per}
+
+
 def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
     """Batched ensemble: 48 channels, nz from {32, 48, 64, 96}, nx = ny from {48, 64, 96},
     Re_bulk in [2000, 4000], gamma in [2.0, 2.8] (numpy default_rng(2026)), instance i seeded
@@ -263,6 +317,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
@@ -292,6 +347,10 @@ def main():
     if not args.no_fp32 and args.precision == "fp64":
         other = run_gpu("fp32", steps, warmup)
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
+    c3 = None
+    if not args.no_c3:
+        c3 = run_c3()
+        print(f"[bench] 384x384x256 fp32: {c3['value']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
     ens = None
     if not args.no_ensemble:
         ens = run_ensemble()
@@ -344,6 +403,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adj,
         "ensemble": ens,
+        "re_tau_550": c3,
     }
     if other is not None:
         line["fp32"] = {"value": round(other["mcell_steps_per_s"], 2), "ms_per_step": round(other["ms_per_step"], 5),
