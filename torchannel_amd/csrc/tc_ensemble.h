@@ -41,7 +41,9 @@ __device__ __forceinline__ int ens_chan(const int* __restrict__ start, int nchan
     if (start[mid] <= gb) lo = mid;
     else hi = mid - 1;
   }
-  return lo;
+  // gb is the same for the whole block, so lo is too; saying so lets the descriptor loads that
+  // follow go through scalar registers
+  return __builtin_amdgcn_readfirstlane(lo);
 }
 
 }  // namespace tc

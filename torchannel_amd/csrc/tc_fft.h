@@ -14,7 +14,20 @@ struct FftDesc {
   int n;
   int npass;
   int radix[TC_FFT_MAXPASS];
+  // per pass: ceil(2^32 / d) for d = n / radix and for d = p (the product of the radices already
+  // applied), so that x / d = mulhi(x, magic) for every x < 2^16 the pass divides
+  unsigned mag_m[TC_FFT_MAXPASS];
+  unsigned mag_p[TC_FFT_MAXPASS];
 };
+
+// x / d for x < 2^16, 2 <= d < 2^16, with magic = ceil(2^32 / d): the rounding error of magic is
+// below d / 2^32 per unit of x, so x * magic / 2^32 stays below the next integer.  d = 1 (whose
+// magic does not fit 32 bits) is encoded as magic 0.
+__device__ __forceinline__ int fdiv(int x, unsigned magic) {
+  return magic == 0u ? x : (int)__umulhi((unsigned)x, magic);
+}
+
+static inline unsigned fft_magic(unsigned d) { return d <= 1u ? 0u : (unsigned)(((1ull << 32) + d - 1) / d); }
 
 template <typename T, int R>
 __device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
@@ -64,8 +77,8 @@ __device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
 }
 
 template <typename T, int R>
-__device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N, int p,
-                                         const double2* __restrict__ tw, int sign) {
+__device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N, int p, unsigned mag_m,
+                                         unsigned mag_p, const cplx<T>* __restrict__ twl, int sign) {
   constexpr int MB = (TC_FFT_ELEMS / R + 255) / 256;
   const int M = N / R;
   const int total = batch * M;
@@ -75,7 +88,7 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
   for (int q = 0; q < MB; ++q) {
     const int t = threadIdx.x + q * 256;
     if (t < total) {
-      const int bb = t / M, idx = t - bb * M;
+      const int bb = fdiv(t, mag_m), idx = t - bb * M;
       const cplx<T>* base = buf + bb * ldb;
 #pragma unroll
       for (int mm = 0; mm < R; ++mm) x[q][mm] = base[idx + mm * M];
@@ -86,12 +99,12 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
   for (int q = 0; q < MB; ++q) {
     const int t = threadIdx.x + q * 256;
     if (t < total) {
-      const int bb = t / M, idx = t - bb * M;
-      const int k = idx % p;
+      const int bb = fdiv(t, mag_m), idx = t - bb * M;
+      const int k = idx - fdiv(idx, mag_p) * p;
 #pragma unroll
       for (int mm = 1; mm < R; ++mm) {
-        const double2 wv = tw[mm * k * tstride];
-        const cplx<T> wc = {(T)wv.x, sign < 0 ? (T)wv.y : (T)(-wv.y)};
+        const cplx<T> wv = twl[mm * k * tstride];
+        const cplx<T> wc = {wv.re, sign < 0 ? wv.im : -wv.im};
         x[q][mm] = cmul(x[q][mm], wc);
       }
       fft_butterfly<T, R>(x[q], sign);
@@ -104,17 +117,26 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
   __syncthreads();
 }
 
-// tw[t] = exp(-2*pi*i*t/N), t in [0, N), float64.
+// Twiddles exp(-2*pi*i*t/N), t in [0, N), copied from the float64 table into LDS in the
+// working type (every thread of the block takes part; the caller's next barrier publishes them).
 template <typename T>
-__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const double2* __restrict__ tw,
+__device__ __forceinline__ void fft_stage_twiddles(cplx<T>* twl, const double2* __restrict__ tw, int N) {
+  for (int t = threadIdx.x; t < N; t += blockDim.x) {
+    const double2 v = tw[t];
+    twl[t] = {(T)v.x, (T)v.y};
+  }
+}
+
+template <typename T>
+__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const cplx<T>* __restrict__ twl,
                         int sign) {
   int p = 1;
   for (int ps = 0; ps < f.npass; ++ps) {
     const int R = f.radix[ps];
-    if (R == 4) fft_pass<T, 4>(buf, ldb, batch, f.n, p, tw, sign);
-    else if (R == 2) fft_pass<T, 2>(buf, ldb, batch, f.n, p, tw, sign);
-    else if (R == 3) fft_pass<T, 3>(buf, ldb, batch, f.n, p, tw, sign);
-    else fft_pass<T, 5>(buf, ldb, batch, f.n, p, tw, sign);
+    if (R == 4) fft_pass<T, 4>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 2) fft_pass<T, 2>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 3) fft_pass<T, 3>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else fft_pass<T, 5>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     p *= R;
   }
 }

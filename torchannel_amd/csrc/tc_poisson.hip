@@ -25,7 +25,7 @@ template <typename T, bool FROM_VEL>
 __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __restrict__ u,
                                                const T* __restrict__ v, const T* __restrict__ w,
                                                const T* __restrict__ rin, int nz_in,
-                                               const double* __restrict__ scal, FftDesc fx,
+                                               const double* __restrict__ scal, const FftDesc& fx,
                                                const double2* __restrict__ twx, cplx<T>* __restrict__ S,
                                                int pairs,
     int blk) {
@@ -36,10 +36,13 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
   const int nch = (nz_in + KZ - 1) / KZ;
   const int j = blk / nch, k0 = (blk - j * nch) * KZ;
   const int ldb = nx + 1;
+  const int lkz = 31 - __clz(KZ);   // KZ is a power of two (checked by the launchers)
+  cplx<T>* twl = Z + pairs * ldb;
+  fft_stage_twiddles<T>(twl, twx, nx);
   T rdt = T(1);
   if (FROM_VEL) rdt = (T)(1.0 / scal[TC_S_DT]);
   for (int t = threadIdx.x; t < nx * KZ; t += 256) {
-    const int kk = t % KZ, i = t / KZ;
+    const int kk = t & (KZ - 1), i = t >> lkz;
     const int k = k0 + kk;
     T val = T(0);
     if (k < nz_in) {
@@ -62,10 +65,10 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
     zf[kk & 1] = val;
   }
   __syncthreads();
-  lds_fft<T>(Z, ldb, pairs, fx, twx, -1);
+  lds_fft<T>(Z, ldb, pairs, fx, twl, -1);
   const int nkx = nx / 2 + 1;
   for (int t = threadIdx.x; t < nkx * KZ; t += 256) {
-    const int kk = t % KZ, m = t / KZ;
+    const int kk = t & (KZ - 1), m = t >> lkz;
     const int k = k0 + kk;
     if (k >= nz_in) continue;
     const cplx<T> z1 = Z[(kk >> 1) * ldb + m];
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(256) void k_fwd_x(Dims d, Metric<T> mt, const T* __
 
 // P2 / P4: complex FFT along y for every (kx, z) column.  Block = (kx, chunk of KZ z-rows).
 template <typename T>
-__device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, FftDesc fy,
+__device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const FftDesc& fy,
                                                const double2* __restrict__ twy,
                                                cplx<T>* __restrict__ S, int KZ, int sign,
     int blk) {
@@ -98,15 +101,18 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, FftDesc f
   const int nch = (nz_in + KZ - 1) / KZ;
   const int m = blk / nch, k0 = (blk - m * nch) * KZ;
   const int ldb = ny + 1;
+  const int lkz = 31 - __clz(KZ);   // KZ is a power of two (checked by the launchers)
+  cplx<T>* twl = Y + KZ * ldb;
+  fft_stage_twiddles<T>(twl, twy, ny);
   for (int t = threadIdx.x; t < ny * KZ; t += 256) {
-    const int kk = t % KZ, j = t / KZ;
+    const int kk = t & (KZ - 1), j = t >> lkz;
     const int k = k0 + kk;
     Y[kk * ldb + j] = k < nz_in ? S[((size_t)m * ny + j) * nz_in + k] : cplx<T>{T(0), T(0)};
   }
   __syncthreads();
-  lds_fft<T>(Y, ldb, KZ, fy, twy, sign);
+  lds_fft<T>(Y, ldb, KZ, fy, twl, sign);
   for (int t = threadIdx.x; t < ny * KZ; t += 256) {
-    const int kk = t % KZ, j = t / KZ;
+    const int kk = t & (KZ - 1), j = t >> lkz;
     const int k = k0 + kk;
     if (k < nz_in) S[((size_t)m * ny + j) * nz_in + k] = Y[kk * ldb + j];
   }
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDe
 // bins are dropped), scaled by 1/(nx*ny), written to p with the ghost images of
 // projection_fft.py:150-163 (x copies, y copies, Neumann z walls).
 template <typename T>
-__device__ __forceinline__ void inv_x_body(Dims d, FftDesc fx, const double2* __restrict__ twx,
+__device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const double2* __restrict__ twx,
                                                const cplx<T>* __restrict__ S, T* __restrict__ p,
                                                T scale, int pairs,
     int blk) {
@@ -135,8 +141,11 @@ __device__ __forceinline__ void inv_x_body(Dims d, FftDesc fx, const double2* __
   const int j = blk / nch, k0 = (blk - j * nch) * KZ;
   const int ldb = nx + 1;
   const int h = nx / 2;
+  const int lpr = 31 - __clz(pairs);   // pairs is a power of two (checked by the launchers)
+  cplx<T>* twl = Z + pairs * ldb;
+  fft_stage_twiddles<T>(twl, twx, nx);
   for (int t = threadIdx.x; t < (h + 1) * pairs; t += 256) {
-    const int q = t % pairs, m = t / pairs;
+    const int q = t & (pairs - 1), m = t >> lpr;
     const int ka = k0 + 2 * q, kb = ka + 1;
     cplx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
     if (ka < nz) A = S[((size_t)m * ny + j) * nz + ka];
@@ -146,27 +155,34 @@ __device__ __forceinline__ void inv_x_body(Dims d, FftDesc fx, const double2* __
     if (m > 0 && m < h) Z[q * ldb + nx - m] = {A.re + B.im, B.re - A.im};
   }
   __syncthreads();
-  lds_fft<T>(Z, ldb, pairs, fx, twx, +1);
+  lds_fft<T>(Z, ldb, pairs, fx, twl, +1);
   for (int t = threadIdx.x; t < nx * KZ; t += 256) {
-    const int kk = t % KZ, i = t / KZ;
+    const int kk = t & (KZ - 1), i = t >> (lpr + 1);
     const int k = k0 + kk;
     if (k >= nz) continue;
     const cplx<T> z = Z[(kk >> 1) * ldb + i];
     const T val = ((kk & 1) ? z.im : z.re) * scale;
     const int I = i + 1, J = j + 1, K = k + 1;
-    int xs[3], ys[3], zs[3], ni = 0, nj = 0, nk = 0;
-    xs[ni++] = I;
-    if (I == nx) xs[ni++] = 0;
-    if (I == 1) xs[ni++] = nx + 1;
-    ys[nj++] = J;
-    if (J == ny) ys[nj++] = 0;
-    if (J == 1) ys[nj++] = ny + 1;
-    zs[nk++] = K;
-    if (K == 1) zs[nk++] = 0;
-    if (K == nz) zs[nk++] = nz + 1;
-    for (int a = 0; a < ni; ++a)
-      for (int bq = 0; bq < nj; ++bq)
-        for (int c = 0; c < nk; ++c) p[xs[a] * d.sxp + ys[bq] * d.syp + zs[c]] = val;
+    p[I * d.sxp + J * d.syp + K] = val;
+    if (I == 1 || I == nx || J == 1 || J == ny || K == 1 || K == nz) {
+      // every combination of the cell and its images along each axis, the cell itself excepted
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int X = a == 0 ? I : (a == 1 ? (I == nx ? 0 : -1) : (I == 1 ? nx + 1 : -1));
+        if (X < 0) continue;
+#pragma unroll
+        for (int bq = 0; bq < 3; ++bq) {
+          const int Y = bq == 0 ? J : (bq == 1 ? (J == ny ? 0 : -1) : (J == 1 ? ny + 1 : -1));
+          if (Y < 0) continue;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int Zc = c == 0 ? K : (c == 1 ? (K == 1 ? 0 : -1) : (K == nz ? nz + 1 : -1));
+            if (Zc < 0 || (a | bq | c) == 0) continue;
+            p[X * d.sxp + Y * d.syp + Zc] = val;
+          }
+        }
+      }
+    }
   }
 }
 
@@ -502,6 +518,12 @@ static bool factor_fft(int n, FftDesc& f) {
   while (m % 2 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 2; m /= 2; }
   while (m % 3 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 3; m /= 3; }
   while (m % 5 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 5; m /= 5; }
+  int pp = 1;
+  for (int q = 0; q < f.npass; ++q) {
+    f.mag_m[q] = fft_magic((unsigned)(n / f.radix[q]));
+    f.mag_p[q] = fft_magic((unsigned)pp);
+    pp *= f.radix[q];
+  }
   return m == 1;
 }
 
@@ -622,8 +644,9 @@ int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* 
   const Grid* g = pl->g;
   const Dims& d = g->d;
   const int KZ = 2 * pairs;
+  if (pairs < 1 || (pairs & (pairs - 1)) != 0) return TC_ERR_ARG;
   const int blocks = d.ny * ((nz_in + KZ - 1) / KZ);
-  const size_t lds = sizeof(cplx<T>) * (size_t)pairs * (d.nx + 1);
+  const size_t lds = sizeof(cplx<T>) * ((size_t)pairs * (d.nx + 1) + d.nx);
   const Metric<T> mt = g->metric<T>();
   if (from_vel)
     hipLaunchKernelGGL((k_fwd_x<T, true>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,
@@ -640,8 +663,9 @@ int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* 
 template <typename T>
 int launch_fft_y_t(const Poisson* pl, void* spec, int nz_in, int kz, int sign, hipStream_t st) {
   const Dims& d = pl->g->d;
+  if (kz < 1 || (kz & (kz - 1)) != 0) return TC_ERR_ARG;
   const int blocks = pl->nkx * ((nz_in + kz - 1) / kz);
-  const size_t lds = sizeof(cplx<T>) * (size_t)kz * (d.ny + 1);
+  const size_t lds = sizeof(cplx<T>) * ((size_t)kz * (d.ny + 1) + d.ny);
   hipLaunchKernelGGL((k_fft_y<T>), dim3(blocks), dim3(256), lds, st, pl->nkx, d.ny, nz_in, pl->fy,
                      pl->twy, (cplx<T>*)spec, kz, sign);
   TC_CHECK_LAUNCH();
@@ -674,7 +698,7 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 4: {
       const int KZ = 2 * pl->pairs;
       const int blocks5 = d.ny * ((d.nz + KZ - 1) / KZ);
-      const size_t lds = sizeof(cplx<T>) * (size_t)pl->pairs * (d.nx + 1);
+      const size_t lds = sizeof(cplx<T>) * ((size_t)pl->pairs * (d.nx + 1) + d.nx);
       const T scale = (T)(1.0 / ((double)d.nx * (double)d.ny));
       hipLaunchKernelGGL((k_inv_x<T>), dim3(blocks5), dim3(256), lds, st, d, pl->fx, pl->twx,
                          (const cplx<T>*)pl->spec, (T*)p, scale, pl->pairs);
@@ -703,7 +727,7 @@ int poisson_solve(const Poisson* pl, bool from_vel, const void* u, const void* v
 // Plane FFTs for the statistics: planes (nx, ny, 6) float64 -> folded spectra inputs.
 int plane_fft(const Poisson* pl, hipStream_t st) {
   int rc;
-  const int pairs = std::min(3, pow2_floor(std::max(1, TC_FFT_ELEMS / pl->g->d.nx)));
+  const int pairs = std::min(2, pow2_floor(std::max(1, TC_FFT_ELEMS / pl->g->d.nx)));
   if ((rc = launch_fwd_x_t<double>(pl, false, nullptr, nullptr, nullptr, pl->planes, 6, nullptr,
                                    pl->plane_spec, pairs, st)) != TC_OK)
     return rc;
@@ -756,9 +780,9 @@ void poisson_geometry(const Poisson* pl, int tsz, int* blocks, size_t* lds) {
   const int KZ = 2 * pl->pairs;
   const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : pl->sb;
   blocks[0] = d.ny * ((d.nz + KZ - 1) / KZ);
-  lds[0] = csz * (size_t)pl->pairs * (d.nx + 1);
+  lds[0] = csz * ((size_t)pl->pairs * (d.nx + 1) + d.nx);
   blocks[1] = pl->nkx * ((d.nz + pl->kzy - 1) / pl->kzy);
-  lds[1] = csz * (size_t)pl->kzy * (d.ny + 1);
+  lds[1] = csz * ((size_t)pl->kzy * (d.ny + 1) + d.ny);
   blocks[2] = (pl->nkx * d.ny + sb - 1) / sb;
   lds[2] = csz * (size_t)sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
   blocks[3] = blocks[0];
