@@ -41,7 +41,7 @@ def test_knobs_do_not_change_results(tc, precision):
     from torchannel_amd import _lib as L
     ref = run(tc, {}, precision)
     for knobs in ({L.TUNE_K1_JB: 1}, {L.TUNE_K1_JB: 3}, {L.TUNE_K1_JB: 26},
-                  {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 17}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 9}):
+                  {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 8}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 8}):
         print(f"[tuning] {precision} {knobs}", flush=True)
         got = run(tc, knobs, precision)
         for a, b in zip(got, ref):
@@ -68,5 +68,7 @@ def test_bad_knob_rejected(tc):
     ch = tc.ChannelFlow(config=cfg(16, 12, 16))
     with pytest.raises(RuntimeError):
         ch._grid.set_tuning(L.TUNE_K1_JB, 10 ** 6)
+    with pytest.raises(RuntimeError):
+        ch._grid.set_tuning(L.TUNE_THOMAS_SB, 17)
     with pytest.raises(RuntimeError):
         ch._grid.set_tuning(99, 1)

@@ -32,7 +32,7 @@ def _key(ch):
 
 def _candidates(ch):
     jb = sorted({j for j in (2, 4, 6, 8, 12, 16, 24, 32) if j <= ch.ny + 2})
-    sb = [4, 8, 16, 24, 32, 48, 64]
+    sb = [4, 8, 16]
     return {L.TUNE_K1_JB: (jb, _PHASE_K1), L.TUNE_THOMAS_SB: (sb, _PHASE_THOMAS)}
 
 

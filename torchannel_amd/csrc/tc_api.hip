@@ -157,7 +157,7 @@ int tc_grid_set_tuning(tc_grid* h, int key, int value) {
       g.k1_jb = value;
       return TC_OK;
     case TC_TUNE_THOMAS_SB:
-      if (value < 0 || value > 128) return TC_ERR_ARG;
+      if (value != 0 && value != 4 && value != 8 && value != 16) return TC_ERR_ARG;
       g.thomas_sb = value;
       return TC_OK;
     default:
@@ -173,7 +173,7 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
       *value = g.k1_jb > 0 ? g.k1_jb : 8;
       return TC_OK;
     case TC_TUNE_THOMAS_SB:
-      *value = g.thomas_sb;
+      *value = g.thomas_sb > 0 ? g.thomas_sb : 16;
       return TC_OK;
     default:
       return TC_ERR_ARG;
@@ -636,6 +636,7 @@ struct tc_ensemble {
   int* pstart[4] = {nullptr, nullptr, nullptr, nullptr};
   int pblocks[4] = {0, 0, 0, 0};
   size_t plds[4] = {0, 0, 0, 0};
+  int th_rm = 1;                            // rows per lane of the widest wall-normal Poisson solve
   int newest = -1;
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -664,7 +665,6 @@ static void fill_chan(EnsChan<T>& c, tc_solver* s) {
   c.th_cp = (const T*)pl->th_cp; c.th_iden = (const T*)pl->th_iden;
   c.th_cp0 = pl->th_cp0; c.th_iden0 = pl->th_iden0;
   c.pairs = pl->pairs; c.kzy = pl->kzy;
-  c.sb = g->thomas_sb > 0 ? g->thomas_sb : pl->sb;
   c.nkx = pl->nkx;
   c.jb = g->k1_jb > 0 ? g->k1_jb : 8;
   c.k5_blocks = k5_blocks(g);
@@ -744,6 +744,7 @@ int tc_ensemble_create(tc_solver* const* solvers, int n, tc_ensemble** out) {
     int b[4];
     size_t l[4];
     poisson_geometry(solvers[c]->pl, tsz, b, l);
+    e->th_rm = std::max(e->th_rm, thomas_rows(solvers[c]->g->d.nz));
     for (int q = 0; q < 4; ++q) {
       st[q].push_back(st[q].back() + b[q]);
       e->plds[q] = std::max(e->plds[q], l[q]);
@@ -791,7 +792,7 @@ static int ensemble_step_impl(tc_ensemble* e, double dt, int first, hipStream_t 
   for (auto& g : e->groups)
     TC_RC(ens_launch_k1(e->dtype, g.R, e->dch, g.chans, g.start_k1, g.n, g.blocks_k1, cur, prev, mode, e->branch, st));
   const int* starts[4] = {e->pstart[0], e->pstart[1], e->pstart[2], e->pstart[3]};
-  TC_RC(ens_launch_poisson(e->dtype, e->dch, e->n, starts, e->pblocks, e->plds, st));
+  TC_RC(ens_launch_poisson(e->dtype, e->dch, e->n, starts, e->pblocks, e->plds, e->th_rm, st));
   for (auto& g : e->groups)
     TC_RC(ens_launch_k5(e->dtype, g.R, e->dch, g.chans, g.start_k5, g.n, g.blocks_k5, TC_K5_BULK | TC_K5_FORCING, st));
   TC_RC(ens_launch_finalize(e->dtype, e->dch, e->n, TC_K5_BULK | TC_K5_FORCING, st));

@@ -83,6 +83,24 @@ __device__ __forceinline__ bool tiled_column(int g, int nxp, int nyp, int& comp,
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (TC_WAVE - 1); }
 
+// DPP lane moves within rows of 16 lanes (GFX9 encodings): row_shr:n takes lane l - n, row_shl:n
+// lane l + n; lanes whose source falls outside their row keep `old`.
+#define TC_DPP_ROW_SHL(n) (0x100 + (n))
+#define TC_DPP_ROW_SHR(n) (0x110 + (n))
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_row(float old, float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_row(double old, double x) {
+  const long long o = __double_as_longlong(old), v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll

@@ -25,7 +25,7 @@ struct EnsChan {
   const double* cl;
   const T *th_cp, *th_iden;
   const double *th_cp0, *th_iden0;
-  int pairs, kzy, sb, nkx;
+  int pairs, kzy, nkx;
   // launch shapes and scalars
   int jb, k5_blocks;
   double inv_volume;
@@ -60,5 +60,6 @@ int k1_blocks(const Grid* g);
 // blocks[4] / lds[4]: forward x FFT, y FFT (both directions), Thomas, inverse x FFT
 void poisson_geometry(const Poisson* pl, int tsz, int* blocks, size_t* lds);
 int ens_launch_poisson(int dtype, const void* dch, int n, const int* const* start, const int* blocks,
-                       const size_t* lds, hipStream_t st);
+                       const size_t* lds, int th_rm, hipStream_t st);
+int thomas_rows(int nz);   // rows per lane of the wall-normal Poisson solve
 }  // namespace tc

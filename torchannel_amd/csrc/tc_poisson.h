@@ -25,7 +25,6 @@ struct Poisson {
   void* th_iden = nullptr;     // Thomas 1/pivot_k per system, [k][system]
   double* th_cp0 = nullptr;    // float64 rows of the gauge-fixed (0,0) system
   double* th_iden0 = nullptr;
-  int sb = 32;                 // systems per Thomas block (LDS-staged)
   double* planes = nullptr;    // statistics: (nx, ny, 6) float64
   void* plane_spec = nullptr;  // statistics: (nkx, ny, 6) complex float64
 };

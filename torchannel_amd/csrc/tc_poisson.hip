@@ -16,6 +16,15 @@
 namespace tc {
 
 // ---------------------------------------------------------------------------------------------
+// elements each thread loads before storing any, in the staging loops of P1, P2/P4 and P5
+#define FFT_U 4
+
+// LDS of the x and y FFT blocks: `rows` rows of n + 1 entries, the n twiddles, one spare entry.
+template <typename T>
+__host__ __device__ __forceinline__ size_t fft_lds(int rows, int n) {
+  return sizeof(cplx<T>) * ((size_t)rows * (n + 1) + n + 1);
+}
+
 // P1: real rows along x -> half spectra.  Block = (y row j, chunk of KZ = 2*pairs z-rows).
 // from_vel: rhs = div(q)/dt with div of utils.py:235-240 on the post-CN velocities (whose
 // ghost columns are not stored: x-face 0 is the image of face nx, y-face 0 of face ny, and the
@@ -41,11 +50,19 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
   fft_stage_twiddles<T>(twl, twx, nx);
   T rdt = T(1);
   if (FROM_VEL) rdt = (T)(1.0 / scal[TC_S_DT]);
-  for (int t = threadIdx.x; t < nx * KZ; t += 256) {
-    const int kk = t & (KZ - 1), i = t >> lkz;
-    const int k = k0 + kk;
-    T val = T(0);
-    if (k < nz_in) {
+  // FFT_U elements per thread are loaded before any is stored: indices are clamped into range
+  // (rows past nz_in read row nz_in - 1 and are zeroed, slots past the end store to the spare
+  // entry) so that no load sits behind a branch
+  const int tot = nx * KZ;
+  T* zr = reinterpret_cast<T*>(Z);
+  const int spare = 2 * (pairs * ldb + nx);
+  for (int t0 = threadIdx.x; t0 < tot; t0 += 256 * FFT_U) {
+    T val[FFT_U];
+#pragma unroll
+    for (int q = 0; q < FFT_U; ++q) {
+      const int t = min(t0 + 256 * q, tot - 1);
+      const int kk = t & (KZ - 1), i = t >> lkz;
+      const int k = min(k0 + kk, nz_in - 1);
       if constexpr (FROM_VEL) {
         const int iul = (i == 0) ? nx : i;
         const int jvl = (j == 0) ? ny : j;
@@ -54,15 +71,20 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
         const T vr = v[(i + 1) * d.sxv + (j + 1) * d.syv + k + 1];
         const T vl = v[(i + 1) * d.sxv + jvl * d.syv + k + 1];
         const int cw = (i + 1) * d.sxw + (j + 1) * d.syw;
-        const T wt = (k + 1 < d.nz) ? w[cw + k + 1] : T(0);
-        const T wb = (k > 0) ? w[cw + k] : T(0);
-        val = ((ur - ul) * mt.rdx + (vr - vl) * mt.rdy + (wt - wb) * mt.rdzf[k]) * rdt;
+        const T wt = w[cw + k + 1], wb = w[cw + k];
+        const T wtv = (k + 1 < d.nz) ? wt : T(0), wbv = (k > 0) ? wb : T(0);
+        val[q] = ((ur - ul) * mt.rdx + (vr - vl) * mt.rdy + (wtv - wbv) * mt.rdzf[k]) * rdt;
       } else {
-        val = rin[((size_t)i * ny + j) * nz_in + k];
+        val[q] = rin[((size_t)i * ny + j) * nz_in + k];
       }
     }
-    T* zf = reinterpret_cast<T*>(&Z[(kk >> 1) * ldb + i]);
-    zf[kk & 1] = val;
+#pragma unroll
+    for (int q = 0; q < FFT_U; ++q) {
+      const int t = t0 + 256 * q;
+      const int kk = t & (KZ - 1), i = t >> lkz;
+      const T x = (k0 + kk < nz_in) ? val[q] : T(0);
+      zr[t < tot ? 2 * ((kk >> 1) * ldb + i) + (kk & 1) : spare] = x;
+    }
   }
   __syncthreads();
   lds_fft<T>(Z, ldb, pairs, fx, twl, -1);
@@ -104,10 +126,23 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const Fft
   const int lkz = 31 - __clz(KZ);   // KZ is a power of two (checked by the launchers)
   cplx<T>* twl = Y + KZ * ldb;
   fft_stage_twiddles<T>(twl, twy, ny);
-  for (int t = threadIdx.x; t < ny * KZ; t += 256) {
-    const int kk = t & (KZ - 1), j = t >> lkz;
-    const int k = k0 + kk;
-    Y[kk * ldb + j] = k < nz_in ? S[((size_t)m * ny + j) * nz_in + k] : cplx<T>{T(0), T(0)};
+  // loads ahead of stores, as in fwd_x_body
+  const int tot = ny * KZ;
+  const int spare = KZ * ldb + ny;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += 256 * FFT_U) {
+    cplx<T> val[FFT_U];
+#pragma unroll
+    for (int q = 0; q < FFT_U; ++q) {
+      const int t = min(t0 + 256 * q, tot - 1);
+      const int kk = t & (KZ - 1), jj = t >> lkz;
+      val[q] = S[((size_t)m * ny + jj) * nz_in + min(k0 + kk, nz_in - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < FFT_U; ++q) {
+      const int t = t0 + 256 * q;
+      const int kk = t & (KZ - 1), jj = t >> lkz;
+      Y[t < tot ? kk * ldb + jj : spare] = (k0 + kk < nz_in) ? val[q] : cplx<T>{T(0), T(0)};
+    }
   }
   __syncthreads();
   lds_fft<T>(Y, ldb, KZ, fy, twl, sign);
@@ -144,15 +179,32 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
   const int lpr = 31 - __clz(pairs);   // pairs is a power of two (checked by the launchers)
   cplx<T>* twl = Z + pairs * ldb;
   fft_stage_twiddles<T>(twl, twx, nx);
-  for (int t = threadIdx.x; t < (h + 1) * pairs; t += 256) {
-    const int q = t & (pairs - 1), m = t >> lpr;
-    const int ka = k0 + 2 * q, kb = ka + 1;
-    cplx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
-    if (ka < nz) A = S[((size_t)m * ny + j) * nz + ka];
-    if (kb < nz) B = S[((size_t)m * ny + j) * nz + kb];
-    if (m == 0 || m == h) { A.im = T(0); B.im = T(0); }
-    Z[q * ldb + m] = {A.re - B.im, A.im + B.re};
-    if (m > 0 && m < h) Z[q * ldb + nx - m] = {A.re + B.im, B.re - A.im};
+  // loads ahead of stores, as in fwd_x_body
+  const int tot = (h + 1) * pairs;
+  const int spare = pairs * ldb + nx;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += 256 * FFT_U) {
+    cplx<T> va[FFT_U], vb[FFT_U];
+#pragma unroll
+    for (int q = 0; q < FFT_U; ++q) {
+      const int t = min(t0 + 256 * q, tot - 1);
+      const int qq = t & (pairs - 1), m = t >> lpr;
+      const int ka = k0 + 2 * qq;
+      const cplx<T>* col = S + ((size_t)m * ny + j) * nz;
+      va[q] = col[min(ka, nz - 1)];
+      vb[q] = col[min(ka + 1, nz - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < FFT_U; ++q) {
+      const int t = t0 + 256 * q;
+      const int qq = t & (pairs - 1), m = t >> lpr;
+      const int ka = k0 + 2 * qq, kb = ka + 1;
+      cplx<T> A = ka < nz ? va[q] : cplx<T>{T(0), T(0)};
+      cplx<T> B = kb < nz ? vb[q] : cplx<T>{T(0), T(0)};
+      if (m == 0 || m == h) { A.im = T(0); B.im = T(0); }
+      const bool in = t < tot;
+      Z[in ? qq * ldb + m : spare] = {A.re - B.im, A.im + B.re};
+      if (m > 0 && m < h) Z[in ? qq * ldb + nx - m : spare] = {A.re + B.im, B.re - A.im};
+    }
   }
   __syncthreads();
   lds_fft<T>(Z, ldb, pairs, fx, twl, +1);
@@ -253,116 +305,139 @@ __device__ __forceinline__ void pcr_general(U (&a)[R], U (&b)[R], U (&c)[R], U (
 // each system is one forward and one backward sweep.  A block stages `sb` consecutive systems
 // (one contiguous chunk of S) in LDS; lane l of wave 0 sweeps system s0 + l.  The singular
 // (0,0) system is gauge-fixed (first unknown pinned to zero) and swept in float64.
-template <typename T>
-__device__ __forceinline__ void thomas_body(int nsys, int nz, int sb, const double* __restrict__ cl,
-                                                        const T* __restrict__ cp, const T* __restrict__ iden,
-                                                        const double* __restrict__ cp0,
-                                                        const double* __restrict__ iden0,
-                                                        cplx<T>* __restrict__ S,
-    int blk) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T>* col = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int s0 = blk * sb;
-  const int ns = min(sb, nsys - s0);
-  const int ld = nz + 1;
-  const cplx<T>* src = S + (size_t)s0 * nz;
-  for (int e = threadIdx.x; e < ns * nz; e += 256) {
-    const int sl = e / nz, k = e - sl * nz;
-    col[sl * ld + k] = src[e];
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < ns) {
-    const int sl = threadIdx.x, s = s0 + sl;
-    cplx<T>* c = col + sl * ld;
-    if (s == 0) {
-      // gauge-fixed (0,0) mode in float64 (forward values kept in a float64 LDS row): row 0 is
-      // x0 = 0, row 1 is decoupled from it
-      double* fw = reinterpret_cast<double*>(col + sb * ld);
-      double pr = 0.0, pi = 0.0;
-      fw[0] = 0.0; fw[nz] = 0.0;
-      for (int k = 1; k < nz; ++k) {
-        const double a = (k == 1) ? 0.0 : cl[k];
-        const double id = iden0[k];
-        const cplx<T> d = c[k];
-        pr = ((double)d.re - a * pr) * id;
-        pi = ((double)d.im - a * pi) * id;
-        fw[k] = pr; fw[nz + k] = pi;
-      }
-      double xr = pr, xi = pi;
-      c[nz - 1] = {(T)xr, (T)xi};
-      for (int k = nz - 2; k >= 1; --k) {
-        const double q = cp0[k];
-        xr = fw[k] - q * xr;
-        xi = fw[nz + k] - q * xi;
-        c[k] = {(T)xr, (T)xi};
-      }
-      c[0] = {T(0), T(0)};
-    } else {
-      // the sweeps walk k with the table loads issued a chunk of 8 rows ahead of their use
-      constexpr int CH = 8;
-      cplx<T> dp = {T(0), T(0)};
-      T idb[CH], idn[CH];
+// P3: one wall-normal solve per (kx, ky) system (projection_fft.py:184-215) with the Thomas
+// factorisation computed once per plan (extended precision on the host, stored [system][k]):
+// forward y_k = (d_k - a_k y_{k-1}) / pivot_k, backward x_k = y_k - c'_k x_{k+1}.  Both are
+// first-order linear recurrences, evaluated here as scans of affine maps y -> A y + B, whose
+// composition is associative: a system is one DPP row of 16 lanes, lane g holds rows
+// g*R .. g*R+R-1, composes its R maps in registers, and the 16 lane maps are combined by a
+// Hillis-Steele scan over row_shr (forward) / row_shl (backward) moves.  The products of
+// |a_k / pivot_k| <= 1 and |c'_k| <= 1 (diagonally dominant rows) keep every partial map
+// bounded, so the scan carries the sequential sweep's round-off.  The singular (0,0) system is
+// gauge-fixed (row 0 pinned to zero, row 1 decoupled from it) and solved in float64.
+#define TH_L 16
+#define TH_SYS_PER_BLOCK (256 / TH_L)
+
+template <int CTRL, typename U>
+__device__ __forceinline__ void th_combine(U& a, cplx<U>& b) {
+  const U a2 = dpp_row<CTRL>(U(1), a);
+  const U br = dpp_row<CTRL>(U(0), b.re), bi = dpp_row<CTRL>(U(0), b.im);
+  b = {a * br + b.re, a * bi + b.im};
+  a = a * a2;
+}
+
+template <typename U, typename T, typename TT, int RM>
+__device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT* __restrict__ cps,
+                                            const TT* __restrict__ ids, const double* __restrict__ cl, int nz,
+                                            int R, int g, bool gauge, bool store) {
+  const int m0 = g * R;
+  U A[RM], C[RM];
+  cplx<U> B[RM];
+  // every load is issued up front and unconditionally (rows past nz read row nz - 1 and act as
+  // identity maps)
 #pragma unroll
-      for (int q = 0; q < CH; ++q) idb[q] = q < nz ? iden[(size_t)q * nsys + s] : T(0);
-      for (int k0 = 0; k0 < nz; k0 += CH) {
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-          const int kn = k0 + CH + q;
-          idn[q] = kn < nz ? iden[(size_t)kn * nsys + s] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-          const int k = k0 + q;
-          if (k < nz) {
-            const T a = k > 0 ? (T)cl[k] : T(0);
-            const cplx<T> d = c[k];
-            dp = {(d.re - a * dp.re) * idb[q], (d.im - a * dp.im) * idb[q]};
-            c[k] = dp;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < CH; ++q) idb[q] = idn[q];
-      }
-      cplx<T> x = dp;
-      T cpb[CH], cpn[CH];
-      const int top = nz - 2;
-#pragma unroll
-      for (int q = 0; q < CH; ++q) cpb[q] = top - q >= 0 ? cp[(size_t)(top - q) * nsys + s] : T(0);
-      for (int k0 = top; k0 >= 0; k0 -= CH) {
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-          const int kn = k0 - CH - q;
-          cpn[q] = kn >= 0 ? cp[(size_t)kn * nsys + s] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < CH; ++q) {
-          const int k = k0 - q;
-          if (k >= 0) {
-            const cplx<T> d = c[k];
-            x = {d.re - cpb[q] * x.re, d.im - cpb[q] * x.im};
-            c[k] = x;
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < CH; ++q) cpb[q] = cpn[q];
-      }
+  for (int r = 0; r < RM; ++r) {
+    if (r < R) {
+      const int m = m0 + r, mm = min(m, nz - 1);
+      const cplx<T> dv = col[mm];
+      const U idv = (U)ids[mm], clv = (U)cl[mm];
+      C[r] = (U)cps[mm];
+      const U be = (gauge && mm == 0) ? U(0) : idv;
+      const U a = (mm == 0 || (gauge && mm == 1)) ? U(0) : clv;
+      const bool in = m < nz;
+      A[r] = in ? -a * be : U(1);
+      B[r] = in ? cplx<U>{be * (U)dv.re, be * (U)dv.im} : cplx<U>{U(0), U(0)};
     }
   }
-  __syncthreads();
-  cplx<T>* dst = S + (size_t)s0 * nz;
-  for (int e = threadIdx.x; e < ns * nz; e += 256) {
-    const int sl = e / nz, k = e - sl * nz;
-    dst[e] = col[sl * ld + k];
+  // forward: lane-local prefix maps, then the scan of the 16 lane maps
+  U pa = U(1);
+  cplx<U> pb = {U(0), U(0)};
+#pragma unroll
+  for (int r = 0; r < RM; ++r) {
+    if (r < R) {
+      pb = {A[r] * pb.re + B[r].re, A[r] * pb.im + B[r].im};
+      pa = A[r] * pa;
+      A[r] = pa;
+      B[r] = pb;
+    }
+  }
+  th_combine<TC_DPP_ROW_SHR(1)>(pa, pb);
+  th_combine<TC_DPP_ROW_SHR(2)>(pa, pb);
+  th_combine<TC_DPP_ROW_SHR(4)>(pa, pb);
+  th_combine<TC_DPP_ROW_SHR(8)>(pa, pb);
+  const U ein_r = dpp_row<TC_DPP_ROW_SHR(1)>(U(0), pb.re), ein_i = dpp_row<TC_DPP_ROW_SHR(1)>(U(0), pb.im);
+  // y rows; backward maps x -> -c'_k x + y_k (rows past nz: x = 0)
+#pragma unroll
+  for (int r = 0; r < RM; ++r) {
+    if (r < R) {
+      const int m = m0 + r, mm = min(m, nz - 1);
+      const bool in = m < nz;
+      const cplx<U> y = {A[r] * ein_r + B[r].re, A[r] * ein_i + B[r].im};
+      A[r] = in ? -C[r] : U(0);
+      B[r] = in ? y : cplx<U>{U(0), U(0)};
+    }
+  }
+  U qa = U(1);
+  cplx<U> qb = {U(0), U(0)};
+#pragma unroll
+  for (int r = RM - 1; r >= 0; --r) {
+    if (r < R) {
+      qb = {A[r] * qb.re + B[r].re, A[r] * qb.im + B[r].im};
+      qa = A[r] * qa;
+      A[r] = qa;
+      B[r] = qb;
+    }
+  }
+  th_combine<TC_DPP_ROW_SHL(1)>(qa, qb);
+  th_combine<TC_DPP_ROW_SHL(2)>(qa, qb);
+  th_combine<TC_DPP_ROW_SHL(4)>(qa, qb);
+  th_combine<TC_DPP_ROW_SHL(8)>(qa, qb);
+  const U xin_r = dpp_row<TC_DPP_ROW_SHL(1)>(U(0), qb.re), xin_i = dpp_row<TC_DPP_ROW_SHL(1)>(U(0), qb.im);
+  if (!store) return;
+#pragma unroll
+  for (int r = 0; r < RM; ++r) {
+    if (r < R) {
+      const int m = m0 + r;
+      if (m < nz) col[m] = {(T)(A[r] * xin_r + B[r].re), (T)(A[r] * xin_i + B[r].im)};
+    }
   }
 }
 
-template <typename T>
+// rows per lane of a system of nz rows
+__host__ __device__ __forceinline__ int thomas_rows_per_lane(int nz) {
+  const int r = (nz + TH_L - 1) / TH_L;
+  return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : 16;
+}
+
+// Block = sb systems of TH_L lanes each; R <= RM rows per lane.
+template <typename T, int RM>
+__device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int sb, const double* __restrict__ cl,
+                                            const T* __restrict__ cp, const T* __restrict__ iden,
+                                            const double* __restrict__ cp0, const double* __restrict__ iden0,
+                                            cplx<T>* __restrict__ S, int blk) {
+  const int lane = threadIdx.x & (TH_L - 1);
+  const int s = blk * sb + (int)(threadIdx.x / TH_L);
+  const bool live = s < nsys;
+  const int sc = live ? s : nsys - 1;
+  cplx<T>* col = S + (size_t)sc * nz;
+  const size_t to = (size_t)sc * nz;
+  if constexpr (sizeof(T) == sizeof(double)) {
+    const bool gauge = sc == 0;
+    thomas_rows<double, T, double, RM>(col, gauge ? cp0 : (const double*)cp + to,
+                                       gauge ? iden0 : (const double*)iden + to, cl, nz, R, lane, gauge, live);
+  } else {
+    if (sc == 0) thomas_rows<double, T, double, RM>(col, cp0, iden0, cl, nz, R, lane, true, live);
+    else thomas_rows<T, T, T, RM>(col, cp + to, iden + to, cl, nz, R, lane, false, live);
+  }
+}
+
+template <typename T, int RM>
 __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
                                                         const T* __restrict__ cp, const T* __restrict__ iden,
                                                         const double* __restrict__ cp0,
                                                         const double* __restrict__ iden0,
                                                         cplx<T>* __restrict__ S) {
-  thomas_body<T>(nsys, nz, sb, cl, cp, iden, cp0, iden0, S, xcd_remap(blockIdx.x, gridDim.x));
+  thomas_body<T, RM>(nsys, nz, RM, sb, cl, cp, iden, cp0, iden0, S, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Ensemble forms of P1-P5: global block -> (channel, block of that channel) via prefix sums.
@@ -385,14 +460,14 @@ __global__ __launch_bounds__(256) void k_fft_y_ens(const EnsChan<T>* __restrict_
   fft_y_body<T>(c.nkx, c.d.ny, c.d.nz, c.fy, c.twy, c.spec, c.kzy, sign, gb - start[q]);
 }
 
-template <typename T>
+template <typename T, int RM>
 __global__ __launch_bounds__(256) void k_poisson_thomas_ens(const EnsChan<T>* __restrict__ ch,
                                                             const int* __restrict__ start, int n) {
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
   const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
-  thomas_body<T>(c.nkx * c.d.ny, c.d.nz, c.sb, c.cl, c.th_cp, c.th_iden, c.th_cp0, c.th_iden0, c.spec,
-                 gb - start[q]);
+  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), TH_SYS_PER_BLOCK, c.cl, c.th_cp,
+                     c.th_iden, c.th_cp0, c.th_iden0, c.spec, gb - start[q]);
 }
 
 template <typename T>
@@ -585,7 +660,7 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   hipMemcpy(pl->cr, cr.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   hipMemcpy(pl->K2, k2.data(), sizeof(double) * k2.size(), hipMemcpyHostToDevice);
   // Thomas factorisation of every (kx, ky) system of projection_fft.py:56-82 (rows built in
-  // float64 exactly as the reference builds tri_a/b/c, factorised in long double), [k][system]
+  // float64 exactly as the reference builds tri_a/b/c, factorised in long double), [system][k]
   const int nsys = nkx * ny;
   std::vector<double> cp64((size_t)nz * nsys), id64((size_t)nz * nsys);
   std::vector<double> cp0(nz), id0(nz);
@@ -602,8 +677,8 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
       if (gauge && k == 0) { b = 1.0; c = 0.0; }
       const long double den = (long double)b - (long double)a * cprev;
       const long double cpk = (long double)c / den;
-      cp64[(size_t)k * nsys + sy] = (double)cpk;
-      id64[(size_t)k * nsys + sy] = (double)(1.0L / den);
+      cp64[(size_t)sy * nz + k] = (double)cpk;
+      id64[(size_t)sy * nz + k] = (double)(1.0L / den);
       if (gauge) { cp0[k] = (double)cpk; id0[k] = (double)(1.0L / den); }
       cprev = cpk;
     }
@@ -624,7 +699,6 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   }
   hipMemcpy(pl->th_cp0, cp0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   hipMemcpy(pl->th_iden0, id0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
-  pl->sb = std::max(1, std::min(64, (int)(67584 / ((size_t)(nz + 1) * esz))));
   if (hipMalloc(&pl->spec, esz * (size_t)nkx * ny * nz) != hipSuccess) return TC_ERR_NOMEM;
   // statistics workspace: 6 planes (u, v, w at the two sampled heights), float64
   if (hipMalloc(&pl->planes, sizeof(double) * (size_t)nx * ny * 6) != hipSuccess) return TC_ERR_NOMEM;
@@ -646,7 +720,7 @@ int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* 
   const int KZ = 2 * pairs;
   if (pairs < 1 || (pairs & (pairs - 1)) != 0) return TC_ERR_ARG;
   const int blocks = d.ny * ((nz_in + KZ - 1) / KZ);
-  const size_t lds = sizeof(cplx<T>) * ((size_t)pairs * (d.nx + 1) + d.nx);
+  const size_t lds = fft_lds<T>(pairs, d.nx);
   const Metric<T> mt = g->metric<T>();
   if (from_vel)
     hipLaunchKernelGGL((k_fwd_x<T, true>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,
@@ -665,7 +739,7 @@ int launch_fft_y_t(const Poisson* pl, void* spec, int nz_in, int kz, int sign, h
   const Dims& d = pl->g->d;
   if (kz < 1 || (kz & (kz - 1)) != 0) return TC_ERR_ARG;
   const int blocks = pl->nkx * ((nz_in + kz - 1) / kz);
-  const size_t lds = sizeof(cplx<T>) * ((size_t)kz * (d.ny + 1) + d.ny);
+  const size_t lds = fft_lds<T>(kz, d.ny);
   hipLaunchKernelGGL((k_fft_y<T>), dim3(blocks), dim3(256), lds, st, pl->nkx, d.ny, nz_in, pl->fy,
                      pl->twy, (cplx<T>*)spec, kz, sign);
   TC_CHECK_LAUNCH();
@@ -684,13 +758,19 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 1: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, -1, st);
     case 2: {
       const int nsys = pl->nkx * d.ny;
-      const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : pl->sb;
+      const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : TH_SYS_PER_BLOCK;
       const int blocks = (nsys + sb - 1) / sb;
-      const size_t lds = sizeof(cplx<T>) * (size_t)sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
-      if (lds > 160 * 1024) return TC_ERR_ARG;
-      hipLaunchKernelGGL((k_poisson_thomas<T>), dim3(blocks), dim3(256), lds, st, nsys, d.nz, sb, pl->cl,
-                         (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,
-                         (cplx<T>*)pl->spec);
+      switch (thomas_rows_per_lane(d.nz)) {
+#define TC_TH(RM)                                                                                      \
+  case RM:                                                                                             \
+    hipLaunchKernelGGL((k_poisson_thomas<T, RM>), dim3(blocks), dim3(sb * TH_L), 0, st, nsys, d.nz, sb, \
+                       pl->cl, (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,  \
+                       (cplx<T>*)pl->spec);                                                            \
+    break;
+        TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8) TC_TH(16)
+#undef TC_TH
+        default: return TC_ERR_UNSUPPORTED;
+      }
       TC_CHECK_LAUNCH();
       return TC_OK;
     }
@@ -698,7 +778,7 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 4: {
       const int KZ = 2 * pl->pairs;
       const int blocks5 = d.ny * ((d.nz + KZ - 1) / KZ);
-      const size_t lds = sizeof(cplx<T>) * ((size_t)pl->pairs * (d.nx + 1) + d.nx);
+      const size_t lds = fft_lds<T>(pl->pairs, d.nx);
       const T scale = (T)(1.0 / ((double)d.nx * (double)d.ny));
       hipLaunchKernelGGL((k_inv_x<T>), dim3(blocks5), dim3(256), lds, st, d, pl->fx, pl->twx,
                          (const cplx<T>*)pl->spec, (T*)p, scale, pl->pairs);
@@ -776,28 +856,34 @@ namespace tc {
 // single-channel launchers above and the ensemble.
 void poisson_geometry(const Poisson* pl, int tsz, int* blocks, size_t* lds) {
   const Dims& d = pl->g->d;
-  const size_t csz = 2 * (size_t)tsz;
   const int KZ = 2 * pl->pairs;
-  const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : pl->sb;
   blocks[0] = d.ny * ((d.nz + KZ - 1) / KZ);
-  lds[0] = csz * ((size_t)pl->pairs * (d.nx + 1) + d.nx);
+  lds[0] = tsz == 8 ? fft_lds<double>(pl->pairs, d.nx) : fft_lds<float>(pl->pairs, d.nx);
   blocks[1] = pl->nkx * ((d.nz + pl->kzy - 1) / pl->kzy);
-  lds[1] = csz * ((size_t)pl->kzy * (d.ny + 1) + d.ny);
-  blocks[2] = (pl->nkx * d.ny + sb - 1) / sb;
-  lds[2] = csz * (size_t)sb * (d.nz + 1) + 2 * sizeof(double) * d.nz;
+  lds[1] = tsz == 8 ? fft_lds<double>(pl->kzy, d.ny) : fft_lds<float>(pl->kzy, d.ny);
+  blocks[2] = (pl->nkx * d.ny + TH_SYS_PER_BLOCK - 1) / TH_SYS_PER_BLOCK;
+  lds[2] = 0;
   blocks[3] = blocks[0];
   lds[3] = lds[0];
 }
 
 template <typename T>
 static int ens_poisson_t(const void* dch, int n, const int* const* start, const int* blocks, const size_t* lds,
-                         hipStream_t st) {
+                         int th_rm, hipStream_t st) {
   const EnsChan<T>* c = (const EnsChan<T>*)dch;
   hipLaunchKernelGGL((k_fwd_x_ens<T>), dim3(blocks[0]), dim3(256), lds[0], st, c, start[0], n);
   TC_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_fft_y_ens<T>), dim3(blocks[1]), dim3(256), lds[1], st, c, start[1], n, -1);
   TC_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_poisson_thomas_ens<T>), dim3(blocks[2]), dim3(256), lds[2], st, c, start[2], n);
+  switch (th_rm) {
+#define TC_TH(RM)                                                                                       \
+  case RM:                                                                                              \
+    hipLaunchKernelGGL((k_poisson_thomas_ens<T, RM>), dim3(blocks[2]), dim3(256), 0, st, c, start[2], n); \
+    break;
+    TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8) TC_TH(16)
+#undef TC_TH
+    default: return TC_ERR_UNSUPPORTED;
+  }
   TC_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_fft_y_ens<T>), dim3(blocks[1]), dim3(256), lds[1], st, c, start[1], n, +1);
   TC_CHECK_LAUNCH();
@@ -807,11 +893,13 @@ static int ens_poisson_t(const void* dch, int n, const int* const* start, const 
 }
 
 int ens_launch_poisson(int dtype, const void* dch, int n, const int* const* start, const int* blocks,
-                       const size_t* lds, hipStream_t st) {
+                       const size_t* lds, int th_rm, hipStream_t st) {
   for (int q = 0; q < 4; ++q)
     if (lds[q] > 160 * 1024) return TC_ERR_ARG;
-  if (dtype == TC_F64) return ens_poisson_t<double>(dch, n, start, blocks, lds, st);
-  return ens_poisson_t<float>(dch, n, start, blocks, lds, st);
+  if (dtype == TC_F64) return ens_poisson_t<double>(dch, n, start, blocks, lds, th_rm, st);
+  return ens_poisson_t<float>(dch, n, start, blocks, lds, th_rm, st);
 }
+
+int thomas_rows(int nz) { return thomas_rows_per_lane(nz); }
 
 }  // namespace tc

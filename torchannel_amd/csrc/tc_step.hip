@@ -218,12 +218,11 @@ struct K1Ctx {
   const T* body;
 };
 
-// RHS column write (interior + the two wall-normal ghost rows), and AB2 + CN into `sout`.
-template <typename T, int R, bool FACE>
-__device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], const T (&q)[R], const T (&rp)[R],
-                                          T* rout, T* sout, const T* bcol, int n, int Kc, bool inner,
-                                          const T (&rz0)[R], const T (&rz1)[R], const T (&rz2)[R],
-                                          T wall) {
+// RHS column write (interior + the two wall-normal ghost rows); returns whether the column
+// steps, with the AB2 predictor q + dt*(1.5 rhs - 0.5 rp) (forward Euler on the first step) in x.
+template <typename T, int R>
+__device__ __forceinline__ bool k1_rhs_ab2(const K1Ctx<T, R>& c, T (&rhs)[R], const T (&q)[R], const T (&rp)[R],
+                                           T* rout, const T* bcol, int n, int Kc, bool inner, T (&x)[R]) {
   const int lane = c.lane;
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -238,9 +237,8 @@ __device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], con
     if (lane == 0) rout[0] = c.F + (bcol != nullptr ? bcol[0] : T(0));
     if (lane == 1) rout[Kc - 1] = c.F + (bcol != nullptr ? bcol[Kc - 1] : T(0));
   }
-  if (!(c.mode & TC_K1_STEP) || !inner) return;
+  if (!(c.mode & TC_K1_STEP) || !inner) return false;
   const bool first = (c.mode & TC_K1_FIRST) != 0;
-  T x[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
@@ -251,10 +249,18 @@ __device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], con
     }
     x[r] = val;
   }
-  // CN right-hand side d = x + beta*D2(x) with the rows' metric held in registers:
-  // cells (u, v): rz0 = 1/dz_c[m], rz1 = 1/dz_c[m+1], rz2 = 1/dz_f[m], reflected neighbours;
-  // faces (w): rz0 = 1/dz_f[m], rz1 = 1/dz_f[m+1], rz2 = 1/dz_c[m+1], explicit half centred on
-  // face m with the wall value below row 0 and zero above the top row.
+  return true;
+}
+
+// CN right-hand side d = x + beta*D2(x) with the rows' metric held in registers, the PCR solve
+// and the store of the star column:
+// cells (u, v): rz0 = 1/dz_c[m], rz1 = 1/dz_c[m+1], rz2 = 1/dz_f[m], reflected neighbours;
+// faces (w): rz0 = 1/dz_f[m], rz1 = 1/dz_f[m+1], rz2 = 1/dz_c[m+1], explicit half centred on
+// face m with the wall value below row 0 and zero above the top row.
+template <typename T, int R, bool FACE>
+__device__ __forceinline__ void k1_cn_store(const K1Ctx<T, R>& c, T (&x)[R], T* sout, int n, const T (&rz0)[R],
+                                            const T (&rz1)[R], const T (&rz2)[R], T wall) {
+  const int lane = c.lane;
   T dn[R], up[R];
   if constexpr (!FACE) {
     rows_put<R, T>(c.rows, x, lane);
@@ -300,6 +306,16 @@ __device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], con
   }
   pcr_solve_tab<R, T>(x, c.rows, c.tabl, n, lane);
   col_store<R, T>(sout, x, n, lane);
+}
+
+template <typename T, int R, bool FACE>
+__device__ __forceinline__ void k1_finish(const K1Ctx<T, R>& c, T (&rhs)[R], const T (&q)[R], const T (&rp)[R],
+                                          T* rout, T* sout, const T* bcol, int n, int Kc, bool inner,
+                                          const T (&rz0)[R], const T (&rz1)[R], const T (&rz2)[R],
+                                          T wall) {
+  T x[R];
+  if (k1_rhs_ab2<T, R>(c, rhs, q, rp, rout, bcol, n, Kc, inner, x))
+    k1_cn_store<T, R, FACE>(c, x, sout, n, rz0, rz1, rz2, wall);
 }
 
 // Ghost column of an rhs array: the forcing (and body force) on every entry, no star column.
@@ -422,8 +438,6 @@ __device__ __forceinline__ void k1_pencil_u(const K1Ctx<T, R>& c, int i, int j0,
   UIn<T, R> A;
   fetch(jlo, A);
   for (int j = jlo; j <= jhi; ++j) {
-    UIn<T, R> B;
-    if (j < jhi) fetch(j + 1, B);
     const long long cu = (long long)j * d.syu;
     T rhs[R], q[R];
 #pragma unroll
@@ -451,9 +465,13 @@ __device__ __forceinline__ void k1_pencil_u(const K1Ctx<T, R>& c, int i, int j0,
       uS[r] = uC[r];
       uC[r] = A.uN[r];
     }
-    k1_finish<T, R, false>(c, rhs, q, A.rp, ru + oi + cu, su + oi + cu, bi != nullptr ? bi + cu : nullptr, n, Kc,
-                           true, rz0, rz1, rz2, T(0));
-    A = B;
+    // the next column's loads are issued once this column's inputs are dead, so they are in
+    // flight during the CN solve
+    T x[R];
+    const bool cn = k1_rhs_ab2<T, R>(c, rhs, q, A.rp, ru + oi + cu, bi != nullptr ? bi + cu : nullptr, n, Kc,
+                                     true, x);
+    if (j < jhi) fetch(j + 1, A);
+    if (cn) k1_cn_store<T, R, false>(c, x, su + oi + cu, n, rz0, rz1, rz2, T(0));
   }
   k1_plain_cols<T, R, false>(c, ui, pu + oi, ru + oi, su + oi, bi, d.syu, jhi + 1, j1, 1, 0, n, Kc, rz0, rz1, rz2);
 }
@@ -519,8 +537,6 @@ __device__ __forceinline__ void k1_pencil_v(const K1Ctx<T, R>& c, int i, int j0,
   VIn<T, R> A;
   fetch(jlo, A);
   for (int j = jlo; j <= jhi; ++j) {
-    VIn<T, R> B;
-    if (j < jhi) fetch(j + 1, B);
     const long long cv = (long long)j * d.syv;
     T rhs[R], q[R];
 #pragma unroll
@@ -549,8 +565,10 @@ __device__ __forceinline__ void k1_pencil_v(const K1Ctx<T, R>& c, int i, int j0,
       vS[r] = vC[r];
       vC[r] = A.vN[r];
     }
-    k1_finish<T, R, false>(c, rhs, q, A.rp, rv + oi + cv, sv + oi + cv, nullptr, n, Kc, true, rz0, rz1, rz2, T(0));
-    A = B;
+    T x[R];
+    const bool cn = k1_rhs_ab2<T, R>(c, rhs, q, A.rp, rv + oi + cv, nullptr, n, Kc, true, x);
+    if (j < jhi) fetch(j + 1, A);
+    if (cn) k1_cn_store<T, R, false>(c, x, sv + oi + cv, n, rz0, rz1, rz2, T(0));
   }
   k1_plain_cols<T, R, false>(c, vi, pv + oi, rv + oi, sv + oi, nullptr, d.syv, jhi + 1, j1, 1, d.ny, n, Kc, rz0,
                              rz1, rz2);
@@ -613,8 +631,6 @@ __device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0,
   WIn<T, R> A;
   fetch(jlo, A);
   for (int j = jlo; j <= jhi; ++j) {
-    WIn<T, R> B;
-    if (j < jhi) fetch(j + 1, B);
     const long long cw = (long long)j * d.syw;
     T rhs[R], q[R], wb[R], wt[R], u01[R], um1[R], v01[R];
     rows_pm<R, T>(c.rows, wC, wb, wt, A.wb0, A.wbn, n, lane);
@@ -635,8 +651,10 @@ __device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0,
       wS[r] = wC[r];
       wC[r] = A.wN[r];
     }
-    k1_finish<T, R, true>(c, rhs, q, A.rp, rw + oi + cw, sw + oi + cw, nullptr, n, Kc, true, rz0, rz1, rz2, T(0));
-    A = B;
+    T x[R];
+    const bool cn = k1_rhs_ab2<T, R>(c, rhs, q, A.rp, rw + oi + cw, nullptr, n, Kc, true, x);
+    if (j < jhi) fetch(j + 1, A);
+    if (cn) k1_cn_store<T, R, true>(c, x, sw + oi + cw, n, rz0, rz1, rz2, T(0));
   }
   k1_plain_cols<T, R, true>(c, wi, pw + oi, rw + oi, sw + oi, nullptr, d.syw, jhi + 1, j1, 1, 0, n, Kc, rz0, rz1, rz2);
 }
