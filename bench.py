@@ -212,6 +212,9 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / ((steps // chunk) * chunk)
+    # state after the timed steps (the per-kernel timing below re-runs phases out of order)
+    finite = float(ch._scal[L.S_NONFINITE]) == 0.0
+    dt_final, u_bulk = ch.device_dt(), float(ch._scal[L.S_UBULK])
     n_cells = nx * ny * nz
     words = algorithmic_words(nx, ny, nz)
     per = {}
@@ -225,11 +228,10 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
         avg = e0.elapsed_time(e1) / phase_reps
         byts = words[name] * 4 * n_cells
         per[name] = {"avg_us": round(avg * 1e3, 2), "gbs": round(byts / (avg * 1e-3) / 1e9, 1) if byts else 0.0}
-    finite = float(ch._scal[L.S_NONFINITE]) == 0.0
     return {"workload": "384x384x256, Lx=2pi, Ly=pi, Re_bulk=10000, float32, adaptive dt CFL 0.25 (first dt "
                         f"from the CFL rule), {(steps // chunk) * chunk} steps", "finite": finite,
             "value": round(n_cells / (ms * 1e-3) / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(ms, 4),
-            "dt_final": ch.device_dt(), "u_bulk": float(ch._scal[L.S_UBULK]), "kernels": per}
+            "dt_final": dt_final, "u_bulk": u_bulk, "kernels": per}
 
 
 def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
