@@ -1,4 +1,4 @@
-// LDS-resident batched complex FFT (Stockham autosort, radices 4, 2, 3, 5; N = 2^a 3^b 5^c).
+// LDS-resident batched complex FFT (Stockham autosort, radices 8, 4, 2, 3, 5; N = 2^a 3^b 5^c).
 //
 // A block of 256 threads transforms `batch` sequences of length N held in LDS (sequence b at
 // buf[b*ldb + 0..N)), batch*N <= TC_FFT_ELEMS.  Pass with radix R and span p (p = product of the
@@ -35,6 +35,26 @@ __device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
     const cplx<T> a = x[0], b = x[1];
     x[0] = cadd(a, b);
     x[1] = csub(a, b);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd inputs, then X[k], X[k+4] = E[k] +- w^k O[k] with
+    // w = exp(sign*2*pi*i/8): w = c (1 + s i), w^2 = s i, w^3 = c (-1 + s i), c = 1/sqrt(2)
+    cplx<T> e[4] = {x[0], x[2], x[4], x[6]};
+    cplx<T> o[4] = {x[1], x[3], x[5], x[7]};
+    fft_butterfly<T, 4>(e, sign);
+    fft_butterfly<T, 4>(o, sign);
+    const T c = T(0.70710678118654752440084436210485);
+    const T sg = sign < 0 ? T(-1) : T(1);
+    const cplx<T> o1 = {c * (o[1].re - sg * o[1].im), c * (o[1].im + sg * o[1].re)};
+    const cplx<T> o2 = {-sg * o[2].im, sg * o[2].re};
+    const cplx<T> o3 = {-c * (o[3].re + sg * o[3].im), c * (sg * o[3].re - o[3].im)};
+    x[0] = cadd(e[0], o[0]);
+    x[4] = csub(e[0], o[0]);
+    x[1] = cadd(e[1], o1);
+    x[5] = csub(e[1], o1);
+    x[2] = cadd(e[2], o2);
+    x[6] = csub(e[2], o2);
+    x[3] = cadd(e[3], o3);
+    x[7] = csub(e[3], o3);
   } else if constexpr (R == 4) {
     const cplx<T> a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
     const cplx<T> c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
@@ -133,7 +153,8 @@ __device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, cons
   int p = 1;
   for (int ps = 0; ps < f.npass; ++ps) {
     const int R = f.radix[ps];
-    if (R == 4) fft_pass<T, 4>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    if (R == 8) fft_pass<T, 8>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 4) fft_pass<T, 4>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     else if (R == 2) fft_pass<T, 2>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     else if (R == 3) fft_pass<T, 3>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     else fft_pass<T, 5>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);

@@ -589,6 +589,7 @@ static bool factor_fft(int n, FftDesc& f) {
   f.n = n;
   f.npass = 0;
   int m = n;
+  while (m % 8 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 8; m /= 8; }
   while (m % 4 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 4; m /= 4; }
   while (m % 2 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 2; m /= 2; }
   while (m % 3 == 0 && f.npass < TC_FFT_MAXPASS) { f.radix[f.npass++] = 3; m /= 3; }
