@@ -164,6 +164,13 @@ int tc_solver_project_initial(tc_solver* s, int rescale, void* stream);
 /* Adaptive dt on the device (solver.py:465-496 and the 5% rule of 807-812): reads the state,
  * writes the new dt into scal when it changed by more than 5%. */
 int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream);
+/* Device time-series row of run_simulation's n_out diagnostics (solver.py:823-842, which reads
+ * them on the host each time): appends [step, time, u_bulk, u_tau, F, dt, max|div|, CFL rate] as
+ * row rows[0] of the device buffer `rows` (rows[0] counts the rows; row r occupies
+ * rows[8 (r + 1) .. 8 (r + 1) + 7]; rows past `capacity` are counted but not written).  u_tau is
+ * compute_u_tau (utils.py:176-204) with wall distance z_c1, max|div| is over
+ * compute_divergence (utils.py:226-240).  No host synchronisation. */
+int tc_solver_diagnostics(tc_solver* s, double z_c1, double* rows, int capacity, void* stream);
 /* Capture `steps` IMEX steps (even; AB2 history must exist) into a HIP graph, with the adaptive
  * dt update before every `dt_interval`-th step of the chunk when dt_interval > 0.  Replay
  * advances the state by that many steps on `stream`. */

@@ -49,6 +49,26 @@ def generate_grid(gamma: float, nz: int, Lz: float, stretching_type: str = "symm
     return z_f, z_c, z_f[1:] - z_f[:-1], z_c[1:] - z_c[:-1]      # utils.py:42-43
 
 
+def generate_hybrid_grid(nz_uniform: int, nz_stretched: int, z_transition: float, Lz: float, gamma: float):
+    """Uniform region [0, z_transition] plus one-sided tanh region above; utils.py:47-113.
+
+    Faces: linspace over the uniform part (utils.py:68-69), then
+    z_transition + H (1 - tanh(gamma (1 - xi)) / tanh(gamma)), xi = k/nz_stretched, k >= 1
+    (utils.py:71-82); centres and spacings as in generate_grid (utils.py:84-90).
+    """
+    z_lo = torch.linspace(0, z_transition, nz_uniform + 1, dtype=F64)
+    H = Lz - z_transition
+    xi = torch.linspace(0, nz_stretched, nz_stretched + 1, dtype=F64) / nz_stretched
+    z_hi = H * (1.0 - torch.tanh(gamma * (1.0 - xi)) / torch.tanh(torch.tensor(gamma, dtype=F64)))
+    z_f = torch.cat([z_lo, z_transition + z_hi[1:]])
+    mid = 0.5 * (z_f[:-1] + z_f[1:])
+    z_c = torch.cat([torch.tensor([-mid[0]], dtype=F64), mid, torch.tensor([2 * z_f[-1] - mid[-1]], dtype=F64)])
+    dz_f = z_f[1:] - z_f[:-1]
+    if bool(torch.any(dz_f <= 0)) or bool(torch.any(z_f[1:] <= z_f[:-1])):   # utils.py:106-111
+        raise ValueError("non-monotone hybrid grid")
+    return z_f, z_c, dz_f, z_c[1:] - z_c[:-1]
+
+
 def initialize_flow(nx, ny, nz, z_c, Lz, U_bulk=1.0, perturbation_intensity=0.0,
                     top_wall_bc_type="dirichlet", seed=None):
     """Laminar parabola plus uniform noise; initflow.py:6-26, 84-102.
@@ -563,7 +583,7 @@ class Channel:
 
     def __init__(self, nx, ny, nz, Lx, Ly, Lz=2.0, gamma=2.6, Re=2870.0, U_bulk=1.0,
                  perturbation=0.1, seed=0, top_wall_bc_type="dirichlet", branch="gpu",
-                 gauge_zero_mode=False):
+                 gauge_zero_mode=False, grid=None):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nu = 1.0 / Re
@@ -571,7 +591,8 @@ class Channel:
         self.top = top_wall_bc_type
         self.branch = branch
         self.gauge = gauge_zero_mode
-        self.z_f, self.z_c, self.dz_f, self.dz_c = generate_grid(gamma, nz, Lz)
+        # grid: (z_f, z_c, dz_f, dz_c) of another generator (the hybrid grid), else the tanh grid
+        self.z_f, self.z_c, self.dz_f, self.dz_c = grid if grid is not None else generate_grid(gamma, nz, Lz)
         self.dx, self.dy = Lx / nx, Ly / ny
         self.cell_vol = (self.dx * self.dy * self.dz_f.view(1, 1, -1)).expand(nx, ny, nz)
         self.total_volume = Lx * Ly * Lz
@@ -660,6 +681,7 @@ class Channel:
 
 PINS = {
     "generate_grid": "closed form tanh + sum(dz_f)=Lz + golden trajectory grid",
+    "generate_hybrid_grid": "uniform part = linspace; stretched part: endpoints, sum(dz_f)=Lz, growth, tanh at a node",
     "initialize_flow": "golden fixture from the reference (seeded) + parabola closed form",
     "apply_bc_all": "brute-force loop over the reference's order + golden trajectory",
     "rhs_fused_imex": "golden fixture (reference fused op) + sin-field O(h^2) convergence",

@@ -86,6 +86,26 @@ def test_steps_match_oracle(tc, shape, top):
         assert float(div.abs().max()) < 1e-10
 
 
+def test_hybrid_grid_steps_match_oracle(tc):
+    """stretching_type 'hybrid' (solver.py:192-211): nz comes from the two regions."""
+    nx, ny, nzu, nzs = 16, 12, 10, 14
+    c = cfg(nx, ny, 8)
+    c["grid"].update({"nz_uniform": nzu, "nz_stretched": nzs})
+    c["domain"].update({"stretching_type": "hybrid", "z_transition": 0.4, "gamma_stretched": 1.5})
+    print("[step] hybrid grid 16x12x(10+14)", flush=True)
+    ch = tc.ChannelFlow(config=c)
+    assert ch.nz == nzu + nzs
+    grid = O.generate_hybrid_grid(nzu, nzs, 0.4, 2.0, 1.5)
+    ref = O.Channel(nx, ny, nzu + nzs, 4 * math.pi, 4 * math.pi / 3, seed=0, gauge_zero_mode=True, grid=grid)
+    assert torch.equal(ch.dz_f.cpu(), ref.dz_f)
+    for s in range(4):
+        ch.step_imex(0.005)
+        ref.step_imex(0.005)
+        for q, r, name in ((ch.u, ref.u, "u"), (ch.v, ref.v, "v"), (ch.w, ref.w, "w")):
+            assert rel(q, r) < TOL, (name, s)
+        assert rel(_pmod(ch.p), _pmod(ref.p)) < 1e-10
+
+
 def test_three_steps_match_reference_trajectory(tc, golden_dir):
     """The reference's own ChannelFlow (CPU branch) on a seeded 16x12x16 channel."""
     g = np.load(os.path.join(golden_dir, "step_trajectory.npz"))

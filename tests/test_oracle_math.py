@@ -20,6 +20,39 @@ def test_grid_closed_form_and_volume():
     assert len(dz_c) == nz + 1 and len(z_c) == nz + 2
 
 
+@pytest.mark.parametrize("nzu,nzs,zt,g", [(75, 75, 0.25, 1.8), (10, 14, 0.4, 1.5), (3, 40, 0.05, 2.6)])
+def test_hybrid_grid_structure(nzu, nzs, zt, g):
+    """utils.py:47-113: uniform cells up to z_transition; above it, faces whose inverse tanh map
+    recovers equally spaced xi (an independent inversion of the stretching law)."""
+    Lz = 2.0
+    z_f, z_c, dz_f, dz_c = O.generate_hybrid_grid(nzu, nzs, zt, Lz, g)
+    assert len(z_f) == nzu + nzs + 1 and len(z_c) == nzu + nzs + 2 and len(dz_c) == nzu + nzs + 1
+    assert np.abs(z_f[:nzu + 1].numpy() - np.linspace(0, zt, nzu + 1)).max() < 1e-15
+    assert float(z_f[nzu]) == zt and abs(float(z_f[-1]) - Lz) < 1e-15
+    assert abs(float(dz_f.sum()) - Lz) < 1e-13
+    up = dz_f[nzu:].numpy()
+    assert np.all(np.diff(up) > 0)                      # fine at the transition, coarse at the top
+    H = Lz - zt
+    zeta = (z_f[nzu:].numpy() - zt) / H
+    xi = 1.0 - np.arctanh((1.0 - zeta) * np.tanh(g)) / g
+    assert np.abs(xi - np.arange(nzs + 1) / nzs).max() < 1e-12
+    assert abs(float(z_c[0]) + float(z_c[1])) < 1e-15
+    assert abs(float(z_c[-1]) - (2 * Lz - float(z_c[-2]))) < 1e-14
+
+
+def test_hybrid_grid_rejects_folded_grid():
+    # a transition above the top wall makes the upper region run downwards
+    with pytest.raises(ValueError):
+        O.generate_hybrid_grid(8, 8, 2.5, 2.0, 1.5)
+
+
+def test_package_hybrid_grid_matches_oracle():
+    import torchannel_amd as tc
+    for args in ((75, 75, 0.25, 2.0, 1.8), (10, 14, 0.4, 2.0, 1.5)):
+        for a, b in zip(tc.generate_hybrid_grid(*args), O.generate_hybrid_grid(*args)):
+            assert torch.equal(a, b)
+
+
 def _banded(a, b, c, d):
     n = len(b)
     ab = np.zeros((3, n))

@@ -18,11 +18,11 @@ from .projection_fft import initialize_fft_solver, solve_poisson_fft, solve_trid
 from .solver import ChannelFlow, apply_bc_all
 from .tridiag import pcr_solve
 from .turbstats import TurbulenceStats
-from .utils import compute_bulk_velocity, compute_divergence, compute_u_tau, generate_grid
+from .utils import compute_bulk_velocity, compute_divergence, compute_u_tau, generate_grid, generate_hybrid_grid
 
 __all__ = [
     "ChannelFlow", "apply_bc_all", "compute_bulk_velocity", "compute_cfl_fused", "compute_divergence",
-    "compute_momentum_rhs_fused_imex", "compute_momentum_rhs_separate", "compute_u_tau", "generate_grid",
+    "compute_momentum_rhs_fused_imex", "compute_momentum_rhs_separate", "compute_u_tau", "generate_grid", "generate_hybrid_grid",
     "initialize_fft_solver", "initialize_flow", "pcr_solve", "project_velocity", "solve_implicit_diffusion_u",
     "solve_implicit_diffusion_v", "solve_implicit_diffusion_w", "solve_poisson_fft", "solve_tridiagonal",
     "solve_tridiagonal_batch", "TurbulenceStats", "WallShearAdjoint", "wall_shear_objective", "tune", "ChannelEnsemble",

@@ -429,6 +429,16 @@ int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double d
                     dt_min, dt_max, kMaxRedBlocks, (hipStream_t)stream);
 }
 
+int tc_solver_diagnostics(tc_solver* s, double z_c1, double* rows, int capacity, void* stream) {
+  if (!s || !rows || capacity < 0) return TC_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  // u_tau into rows[1], max|div| into rows[2] (both through the solver's reduction partials,
+  // in stream order), then the row itself
+  TC_RC(launch_u_tau(s->g, s->u, z_c1, rows + 1, s->cfl_partials, s->counters + 1, kMaxRedBlocks, st));
+  TC_RC(launch_div_max(s->g, s->u, s->v, s->w, rows + 2, s->cfl_partials, kMaxRedBlocks, st));
+  return launch_diag_row(s->scal, rows, capacity, st);
+}
+
 int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_target, double dt_min,
                       double dt_max, void* stream) {
   if (!s || steps < 2 || (steps & 1) || s->newest < 0) return TC_ERR_ARG;

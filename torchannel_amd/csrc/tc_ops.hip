@@ -123,6 +123,49 @@ __global__ __launch_bounds__(256) void k_divergence(Dims d, Metric<T> mt, const 
   div[t] = (u[cu] - u[cu - d.sxu]) * mt.rdx + (v[cv] - v[cv - d.syv]) * mt.rdy + (w[cw] - w[cw - 1]) * mt.rdzf[k];
 }
 
+// max |compute_divergence| (utils.py:235-240) over the interior: per-block maxima (the maximum
+// does not depend on the order), folded by k_finalize.
+template <typename T>
+__global__ __launch_bounds__(256) void k_div_max(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                                 const T* __restrict__ v, const T* __restrict__ w,
+                                                 double* __restrict__ partials) {
+  __shared__ double sh[4];
+  const long long N = (long long)d.nx * d.ny * d.nz;
+  double best = 0.0;
+  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long long)gridDim.x * 256) {
+    const int k = (int)(t % d.nz);
+    const long long ij = t / d.nz;
+    const int j = (int)(ij % d.ny), i = (int)(ij / d.ny);
+    const int cu = (i + 1) * d.sxu + (j + 1) * d.syu + k + 1;
+    const int cv = (i + 1) * d.sxv + (j + 1) * d.syv + k + 1;
+    const int cw = (i + 1) * d.sxw + (j + 1) * d.syw + k + 1;
+    const T dv = (u[cu] - u[cu - d.sxu]) * mt.rdx + (v[cv] - v[cv - d.syv]) * mt.rdy + (w[cw] - w[cw - 1]) * mt.rdzf[k];
+    best = fmax(best, (double)fabs(dv));
+  }
+  const double bm = block_reduce(best, true, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = bm;
+}
+
+// One row of the device time series: [step, time, u_bulk, u_tau, F, dt, max|div|, CFL rate] from
+// the scalar block and the two values the reductions left in rows[1] (u_tau) and rows[2]
+// (max|div|); rows[0] counts the rows written, rows beyond `capacity` are dropped.
+__global__ void k_diag_row(const double* __restrict__ scal, double* __restrict__ rows, int capacity) {
+  if (threadIdx.x != 0) return;
+  const int c = (int)rows[0];
+  if (c < capacity) {
+    double* r = rows + 8 * (c + 1);
+    r[0] = scal[TC_S_STEP];
+    r[1] = scal[TC_S_TIME];
+    r[2] = scal[TC_S_UBULK];
+    r[3] = rows[1];
+    r[4] = scal[TC_S_F];
+    r[5] = scal[TC_S_DT];
+    r[6] = rows[2];
+    r[7] = scal[TC_S_CFL];
+  }
+  rows[0] = (double)(c + 1);
+}
+
 // project_velocity (projection.py:154-163) on the interior, in place.
 template <typename T>
 __global__ __launch_bounds__(256) void k_project(Dims d, Metric<T> mt, T* __restrict__ u, T* __restrict__ v,
@@ -389,6 +432,29 @@ int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0,
   TC_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_BULK, (const double*)partials, blocks, out,
                      (double*)nullptr, inv_vol, 0.0, 0.0, 0);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_div_max(const Grid* gr, const void* u, const void* v, const void* w, double* out, double* partials,
+                   int nblocks_max, hipStream_t st) {
+  const long long N = (long long)gr->d.nx * gr->d.ny * gr->d.nz;
+  const int blocks = (int)std::min<long long>(nblocks_max, (N + 255) / 256);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_div_max<double>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<double>(),
+                       (const double*)u, (const double*)v, (const double*)w, partials);
+  else
+    hipLaunchKernelGGL(k_div_max<float>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<float>(),
+                       (const float*)u, (const float*)v, (const float*)w, partials);
+  TC_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_CFL, (const double*)partials, blocks, out,
+                     (double*)nullptr, 0.0, 0.0, 0.0, 0);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_diag_row(const double* scal, double* rows, int capacity, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_row, dim3(1), dim3(64), 0, st, scal, rows, capacity);
   TC_CHECK_LAUNCH();
   return TC_OK;
 }

@@ -25,7 +25,7 @@ from .initflow import initialize_flow, initialize_flow_from_file
 from .projection_fft import initialize_fft_solver
 from .turbstats import TurbulenceStats
 from .utils import (append_timeseries, compute_bulk_velocity, compute_divergence, compute_u_tau, generate_grid,
-                    save_flow_fields)
+                    generate_hybrid_grid, save_flow_fields)
 
 
 def apply_bc_all(u, v, w, top_wall_bc_type="dirichlet"):
@@ -75,8 +75,9 @@ class ChannelFlow:
         d = config["domain"]
         self.Lx, self.Ly, self.Lz = float(d["Lx"]), float(d["Ly"]), float(d["Lz"])
         self.stretching_type = d.get("stretching_type", "symmetric")
-        if self.stretching_type not in ("symmetric", "bottom"):
-            raise ValueError(f"Invalid stretching type: {self.stretching_type}")
+        if self.stretching_type not in ("symmetric", "bottom", "hybrid"):
+            raise ValueError(f"Invalid stretching type: {self.stretching_type}. "
+                             "Must be 'symmetric', 'bottom', or 'hybrid'")
         fl = config["flow"]
         self.nu = 1.0 / fl["Re"]
         self.Re_tau = fl.get("Re_tau", 180.0)
@@ -98,10 +99,19 @@ class ChannelFlow:
         self.n_out = out.get("n_out", out.get("print_interval", 10))
         self.n_save = out.get("n_save", out.get("slice_interval", 100))
 
-        # grid (utils.py:5-45), built on the host in float64 and moved to the device
-        z_f, z_c, dz_f, dz_c = generate_grid(self.gamma, self.nz, self.Lz, device="cpu",
-                                             stretching_type=self.stretching_type)
+        # grid (utils.py:5-113), built on the host in float64 and moved to the device; the hybrid
+        # grid sets nz from its two regions (solver.py:192-211)
+        if self.stretching_type == "hybrid":
+            nz_u, nz_s = int(g.get("nz_uniform", 75)), int(g.get("nz_stretched", 75))
+            z_f, z_c, dz_f, dz_c = generate_hybrid_grid(nz_u, nz_s, float(d.get("z_transition", 0.25)), self.Lz,
+                                                        float(d.get("gamma_stretched", 1.8)), device="cpu",
+                                                        verbose=verbose)
+            self.nz = nz_u + nz_s
+        else:
+            z_f, z_c, dz_f, dz_c = generate_grid(self.gamma, self.nz, self.Lz, device="cpu",
+                                                 stretching_type=self.stretching_type)
         self.z_f, self.z_c, self.dz_f, self.dz_c = (x.to(self.device) for x in (z_f, z_c, dz_f, dz_c))
+        self._zc1 = float(z_c[1])      # wall distance of compute_u_tau (utils.py:188)
         self.dx, self.dy = self.Lx / self.nx, self.Ly / self.ny
         self.cell_vol = (self.dx * self.dy * self.dz_f.view(1, 1, -1)).expand(self.nx, self.ny, self.nz)
         self.cell_vol_ratio = self.cell_vol
@@ -265,58 +275,104 @@ class ChannelFlow:
             L.call("tc_solver_replay", self._solver, self._stream())
         self.current_step += n_chunks * chunk_steps
 
+    def record_diagnostics(self, rows, capacity):
+        """Append the n_out diagnostics row [step, time, u_bulk, u_tau, F, dt, max|div|, CFL rate]
+        to the device buffer `rows` (rows[0] counts rows; see tc_solver_diagnostics), no sync."""
+        L.call("tc_solver_diagnostics", self._solver, self._zc1, L.ptr(rows), int(capacity), self._stream())
+
+    def _advance(self, k):
+        """k IMEX steps with the adaptive-dt cadence: HIP-graph chunks when the chunk boundaries
+        line up with the dt cadence and no body force is set, eager steps otherwise."""
+        iv = self.dt_update_interval
+        chunk = self.n_out if self.n_out >= 2 and self.n_out % 2 == 0 else 0
+        if (chunk and self._newest >= 0 and self.body_force is None and k % chunk == 0
+                and (iv <= 0 or (chunk % iv == 0 and self.current_step % iv == 0))):
+            self.run_graph(k // chunk, chunk, dt_update_interval=max(iv, 0))
+            return
+        for _ in range(k):
+            self.current_step += 1
+            if iv > 0 and self.current_step % iv == 0:
+                self.update_dt_device()
+            self.step_imex(None)
+
+    def _drain(self, rows, ts):
+        """One host sync: move the device time-series rows into `ts` and reset the buffer."""
+        host = rows.cpu().numpy()
+        n = min(int(host[0]), (len(host) // 8) - 1)
+        for r in range(n):
+            step, tm, ub, ut, f, dt, div, _ = host[8 * (r + 1): 8 * (r + 2)]
+            for key, val in (("step", int(step)), ("time", tm), ("u_bulk", ub), ("u_tau", ut), ("forcing", f)):
+                ts[key].append(val)
+            if self.verbose:
+                print(f"{int(step):6d} {tm:10.6f} {dt:10.6f} {div:12.3e} {ub:10.6f} {ut:10.6f} {f:12.3e}",
+                      flush=True)
+        rows[0] = 0.0
+        self.time = float(self._scal[L.S_TIME].item())
+        self.dt = float(self._scal[L.S_DT].item())
+
     def run_simulation(self):
         """The loop of solver.py:738-970: adaptive dt every dt_update_interval steps (evaluated on
-        the device), diagnostics and the timeseries every n_out steps, statistics every n_stats
-        steps, and every n_save steps the non-finite check (a device flag set by the projection
-        kernel), timeseries.npz append, fields.npz and the statistics state.  Host syncs happen
-        only at those cadences; the t_max test uses the device time read at them."""
+        the device), diagnostics every n_out steps, statistics every n_stats steps, and every
+        n_save steps the non-finite check (a device flag set by the projection kernel),
+        timeseries.npz append, fields.npz and the statistics state.
+
+        The steps between two events run as HIP-graph chunks of n_out steps where the cadences
+        allow it.  The n_out diagnostics (u_bulk, u_tau, forcing, max|div|) are rows of a device
+        time series written without a host sync; the host reads them, and the device time for
+        the t_max test, once per n_save steps (and at the statistics cadence, whose t_stats test
+        needs the time)."""
         if self.time_scheme != "IMEX":
             raise ValueError(f"Unknown time stepping scheme: {self.time_scheme}. Use 'IMEX'.")
         os.makedirs(self.results_folder, exist_ok=True)
         ts = {k: [] for k in ("step", "time", "u_bulk", "u_tau", "forcing")}
-        step = self.current_step
+        n_out, n_save = max(1, int(self.n_out)), max(1, int(self.n_save))
+        stats_on = self.turbulence_stats is not None and self.n_stats > 0
+        cap = n_save // n_out + 4
+        rows = torch.zeros(8 * (cap + 1), dtype=torch.float64, device=self.device)
         t0 = _time.time()
-        ub = f = None
+        step = self.current_step
+        if self._newest < 0 and step < self.n_steps:        # forward-Euler start, eagerly
+            self._advance(1)
+            step = self.current_step
+            if step % n_out == 0:
+                self.record_diagnostics(rows, cap)
+
+        def ceil_to(x, m):
+            return ((x + m - 1) // m) * m
+
         while step < self.n_steps and self.time < self.t_max:
-            step += 1
-            self.current_step = step
-            if self.dt_update_interval > 0 and step % self.dt_update_interval == 0:
-                self.update_dt_device()
-            ub, f = self.step_imex(None)
-            need_host = step % self.n_out == 0 or step % self.n_save == 0 or (
-                self.n_stats > 0 and step % self.n_stats == 0)
-            if need_host:
+            nxt = min(ceil_to(step + 1, n_out), ceil_to(step + 1, n_save), self.n_steps)
+            if stats_on:
+                nxt = min(nxt, ceil_to(step + 1, self.n_stats))
+            self._advance(nxt - step)
+            step = self.current_step
+            if step % n_out == 0:
+                self.record_diagnostics(rows, cap)
+            if stats_on and step % self.n_stats == 0:
                 self.time = float(self._scal[L.S_TIME].item())
-                self.dt = float(self._scal[L.S_DT].item())
-            if step % self.n_out == 0:
-                div = compute_divergence(self.u, self.v, self.w, self.nx, self.ny, self.nz, self.dx, self.dy, self.dz_f)
-                ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
-                for k, val in (("step", step), ("time", self.time), ("u_bulk", float(ub)), ("u_tau", float(ut)),
-                               ("forcing", float(f))):
-                    ts[k].append(val)
-                if self.verbose:
-                    print(f"{step:6d} {self.time:10.6f} {self.dt:10.6f} {float(div.abs().max()):12.3e} "
-                          f"{float(ub):10.6f} {float(ut):10.6f} {float(f):12.3e}", flush=True)
-            if (self.turbulence_stats is not None and self.n_stats > 0 and self.time >= self.t_stats
-                    and step % self.n_stats == 0):
-                self.turbulence_stats.accumulate_statistics(self.u, self.v, self.w)
-            if step % self.n_save == 0:
+                if self.time >= self.t_stats:
+                    self.turbulence_stats.accumulate_statistics(self.u, self.v, self.w)
+            if step % n_save == 0:
+                self._drain(rows, ts)
                 ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
                 if float(self._scal[L.S_NONFINITE].item()) != 0.0:          # solver.py:862-881
                     if self.verbose:
                         print(f"ERROR: NaN or Inf detected in solution at step {step}", flush=True)
+                    ubn = float(self._scal[L.S_UBULK].item())
                     save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step,
-                                     self.time, ut, (self.U_bulk - float(ub)) / self.dt, self.results_folder,
+                                     self.time, ut, (self.U_bulk - ubn) / self.dt, self.results_folder,
                                      "fields_error.npz")
                     break
                 if ts["step"]:
                     append_timeseries(os.path.join(self.results_folder, "timeseries.npz"), **ts)
                     ts = {k: [] for k in ts}
                 save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step,
-                                 self.time, ut, f, self.results_folder, "fields.npz")
+                                 self.time, ut, float(self._scal[L.S_F].item()), self.results_folder, "fields.npz")
                 if self.turbulence_stats is not None and self.turbulence_stats.n_samples > 0:
                     self.turbulence_stats.save_state(self.stats_state_path)
+        self._drain(rows, ts)
+        if ts["step"]:
+            append_timeseries(os.path.join(self.results_folder, "timeseries.npz"), **ts)
         self.time = float(self._scal[L.S_TIME].item())
         self.dt = float(self._scal[L.S_DT].item())
         if self.verbose:

@@ -27,6 +27,36 @@ def generate_grid(gamma, nz, Lz, device="cpu", stretching_type="symmetric", dtyp
     return z_f, z_c, z_f[1:] - z_f[:-1], z_c[1:] - z_c[:-1]
 
 
+def generate_hybrid_grid(nz_uniform, nz_stretched, z_transition, Lz, gamma, device="cpu", dtype=torch.float64,
+                         verbose=False):
+    """Uniform cells on [0, z_transition] under one-sided tanh cells on [z_transition, Lz]
+    (utils.py:47-113): z_f[nz+1], z_c[nz+2], dz_f[nz], dz_c[nz+1] with nz = nz_uniform +
+    nz_stretched.  The upper region is fine at the transition and coarse at the top.  Raises
+    ValueError for a non-monotone grid, as the reference does; with verbose, prints the relative
+    jump of dz at the transition (the reference's C1 check, 1% threshold)."""
+    z_lo = torch.linspace(0, z_transition, nz_uniform + 1, device=device, dtype=dtype)
+    H = Lz - z_transition
+    xi = torch.linspace(0, nz_stretched, nz_stretched + 1, device=device, dtype=dtype) / nz_stretched
+    g = torch.tensor(gamma, device=device, dtype=dtype)
+    z_hi = H * (1.0 - torch.tanh(gamma * (1.0 - xi)) / torch.tanh(g))
+    z_f = torch.cat([z_lo, z_transition + z_hi[1:]])
+    mid = 0.5 * (z_f[:-1] + z_f[1:])
+    z_c = torch.cat([torch.tensor([-mid[0]], device=device, dtype=dtype), mid,
+                     torch.tensor([2 * z_f[-1] - mid[-1]], device=device, dtype=dtype)])
+    dz_f = z_f[1:] - z_f[:-1]
+    dz_c = z_c[1:] - z_c[:-1]
+    if verbose:
+        before, after = float(dz_f[nz_uniform - 1]), float(dz_f[nz_uniform])
+        jump = abs(after - before) / before
+        state = "OK" if jump <= 0.01 else "consider adjusting gamma"
+        print(f"Hybrid grid C1 continuity at z={z_transition}: {jump * 100:.3f}% ({state})", flush=True)
+    if bool(torch.any(dz_f <= 0)):
+        raise ValueError("Non-positive cell spacing detected in hybrid grid!")
+    if bool(torch.any(z_f[1:] <= z_f[:-1])):
+        raise ValueError("Non-monotonic grid detected!")
+    return z_f, z_c, dz_f, dz_c
+
+
 def _dims_u(u):
     return u.shape[0] - 1, u.shape[1] - 2, u.shape[2] - 2
 
