@@ -77,6 +77,12 @@ int tc_apply_bc(const tc_grid* g, void* u, void* v, void* w, void* stream);
  * every entry of ru, rv, rw (zero outside the interior). */
 int tc_rhs_imex(const tc_grid* g, int branch, const void* u, const void* v, const void* w,
                 void* ru, void* rv, void* rw, void* stream);
+/* compute_momentum_rhs_fused_v2 (operators.py:817-1034; the explicit right-hand side of the
+ * reference's FE scheme, solver.py:498-534): advection and the full 3-D diffusion, including
+ * its w control-volume quirk (dz_c[k-1], operators.py:1021-1026).  Writes every entry of ru,
+ * rv, rw (zero outside the interiors). */
+int tc_rhs_fused_v2(const tc_grid* g, const void* u, const void* v, const void* w, void* ru, void* rv,
+                    void* rw, void* stream);
 
 /* solve_implicit_diffusion_u / _v / _w (operators.py:563-811) for comp 0 / 1 / 2: the interior
  * columns of q_out receive the theta-scheme solution for q_in; other entries of q_out are not

@@ -185,6 +185,32 @@ def rhs_fused_imex(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
     return ru, rv, rw
 
 
+def rhs_fused_v2(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
+    """compute_momentum_rhs_fused_v2 (operators.py:817-1034), the FE scheme's explicit RHS.
+
+    The advection and xy-diffusion of rhs_fused_imex on the same interiors, plus the explicit z
+    second derivative: for u and v (operators.py:904-909, 962-963) over 1/dz_c[m], 1/dz_c[m+1]
+    and 1/dz_f[m] at cell m+1; for w (operators.py:1021-1026) dividing by dz_f[k-1], dz_f[k] and
+    the control volume dz_c[k-1] (not dz_c[k]).
+    """
+    ru, rv, rw = rhs_fused_imex(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
+    J, K = slice(1, ny + 1), slice(1, nz + 1)
+    rzc = (1.0 / dz_c).view(1, 1, -1)
+    rzf = (1.0 / dz_f).view(1, 1, -1)
+
+    def dzz(q):
+        return nu * ((q[:, :, 2:nz + 2] - q[:, :, K]) * rzc[:, :, 1:nz + 1]
+                     - (q[:, :, K] - q[:, :, 0:nz]) * rzc[:, :, 0:nz]) * rzf
+
+    ru[1:nx, J, K] += dzz(u[1:nx, J, :])
+    rv[1:nx + 1, 1:ny, K] += dzz(v[1:nx + 1, 1:ny, :])
+    wc = w[1:nx + 1, J, :]
+    dzw = nu * ((wc[:, :, 2:nz + 1] - wc[:, :, 1:nz]) / dz_f[1:nz].view(1, 1, -1)
+                - (wc[:, :, 1:nz] - wc[:, :, 0:nz - 1]) / dz_f[0:nz - 1].view(1, 1, -1)) / dz_c[0:nz - 1].view(1, 1, -1)
+    rw[1:nx + 1, J, 1:nz] += dzw
+    return ru, rv, rw
+
+
 def rhs_separate(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
     """CPU-branch explicit RHS: diffusion_xy_* - advection_* (solver.py:601-622).
 
@@ -686,6 +712,7 @@ PINS = {
     "apply_bc_all": "brute-force loop over the reference's order + golden trajectory",
     "rhs_fused_imex": "golden fixture (reference fused op) + sin-field O(h^2) convergence",
     "rhs_separate": "golden fixture (reference separate ops) + interior equality with fused",
+    "rhs_fused_v2": "fused-IMEX terms (pinned) + z second derivative: O(h^2) convergence on analytic profiles",
     "pcr_solve": "scipy.linalg.solve_banded on random diagonally dominant systems",
     "thomas_solve": "scipy.linalg.solve_banded on random diagonally dominant systems",
     "solve_implicit_diffusion_uv": "dense numpy solve of the CN system + golden trajectory",

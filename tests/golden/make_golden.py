@@ -10,9 +10,10 @@ vectors for the parts of the IMEX path that closed forms and invariants leave op
 * ``ops.npz`` -- compute_momentum_rhs_fused_imex (GPU-branch RHS, including the u-face nx /
   v-face ny behaviour), solve_implicit_diffusion_w (shifted explicit half), and
   TurbulenceStats.accumulate_statistics on one random field.
+* ``ops_v2.npz`` -- compute_momentum_rhs_fused_v2 (FE-scheme RHS) on random fields.
 
 Only data is written: inputs and the reference's outputs.  Usage:
-    PYTORCH_JIT=0 MPLBACKEND=Agg python tests/golden/make_golden.py
+    PYTORCH_JIT=0 MPLBACKEND=Agg python tests/golden/make_golden.py [ops] [trajectory] [c1] [v2]
 """
 import os
 import sys
@@ -119,9 +120,32 @@ def make_ops():
         u_tau=fin["u_tau"])
 
 
+def make_v2():
+    """ops_v2.npz: compute_momentum_rhs_fused_v2 (FE-scheme RHS, full explicit 3-D diffusion) on
+    random fields over a stretched grid, where its w control volume dz_c[k-1] differs from
+    dz_c[k]."""
+    g = torch.Generator().manual_seed(4321)
+    nx, ny, nz = 12, 10, 14
+    Lx, Ly, Lz, nu = 2.0, 1.5, 2.0, 1.0 / 2870.0
+    dx, dy = Lx / nx, Ly / ny
+    z_f, z_c, dz_f, dz_c = generate_grid(2.6, nz, Lz)
+    u = torch.rand(nx + 1, ny + 2, nz + 2, generator=g) - 0.5
+    v = torch.rand(nx + 2, ny + 1, nz + 2, generator=g) - 0.5
+    w = torch.rand(nx + 2, ny + 2, nz + 1, generator=g) - 0.5
+    ru, rv, rw = ref_ops.compute_momentum_rhs_fused_v2(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
+    np.savez_compressed(os.path.join(HERE, "ops_v2.npz"), nx=nx, ny=ny, nz=nz, Lx=Lx, Ly=Ly, Lz=Lz, nu=nu,
+                        u=np64(u), v=np64(v), w=np64(w), rhs_u=np64(ru), rhs_v=np64(rv), rhs_w=np64(rw))
+
+
 if __name__ == "__main__":
+    only = sys.argv[1:]
     with tempfile.TemporaryDirectory() as tmp:
-        make_ops()
-        make_trajectory(tmp)
-        make_c1(tmp)
+        if not only or "ops" in only:
+            make_ops()
+        if not only or "trajectory" in only:
+            make_trajectory(tmp)
+        if not only or "c1" in only:
+            make_c1(tmp)
+        if not only or "v2" in only:
+            make_v2()
     print("fixtures written to", HERE)

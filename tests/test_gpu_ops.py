@@ -78,6 +78,29 @@ def test_rhs_matches_reference_fixture(tc, golden_dir):
     assert rel(w_cn, torch.from_numpy(g["w_cn"])) < TOL
 
 
+def test_fused_v2_rhs(tc, golden_dir):
+    """compute_momentum_rhs_fused_v2 against the reference's own output and, on the larger
+    shapes, against the oracle."""
+    g = np.load(os.path.join(golden_dir, "ops_v2.npz"))
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    _, _, dz_f, dz_c = O.generate_grid(2.6, nz, float(g["Lz"]))
+    u, v, w = (torch.from_numpy(g[k]).cuda() for k in ("u", "v", "w"))
+    got = tc.compute_momentum_rhs_fused_v2(u, v, w, nx, ny, nz, float(g["Lx"]) / nx, float(g["Ly"]) / ny,
+                                           dz_c.cuda(), dz_f.cuda(), float(g["nu"]))
+    for a, key in zip(got, ("rhs_u", "rhs_v", "rhs_w")):
+        assert rel(a, torch.from_numpy(g[key])) < TOL, key
+    for shape in SHAPES:
+        nx, ny, nz = shape
+        print(f"[rhs_v2] shape={shape}", flush=True)
+        _, _, dz_f, dz_c = O.generate_grid(2.6, nz, 2.0)
+        u, v, w = fields(nx, ny, nz, 3)
+        want = O.rhs_fused_v2(u, v, w, nx, ny, nz, 0.1, 0.12, dz_c, dz_f, 1 / 2870.0)
+        got = tc.compute_momentum_rhs_fused_v2(u.cuda(), v.cuda(), w.cuda(), nx, ny, nz, 0.1, 0.12, dz_c.cuda(),
+                                               dz_f.cuda(), 1 / 2870.0)
+        for a, b_, key in zip(got, want, ("u", "v", "w")):
+            assert rel(a, b_) < TOL, (shape, key)
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("top", ["dirichlet", "neumann"])
 def test_cn(tc, shape, top):

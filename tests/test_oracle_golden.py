@@ -31,6 +31,17 @@ def test_fused_rhs_matches_reference(ops):
     assert np.all(ops["rhs_u"][nx] == 0.0) and np.all(ops["rhs_v"][:, ny] == 0.0)
 
 
+def test_fused_v2_rhs_matches_reference(golden_dir):
+    """compute_momentum_rhs_fused_v2 on a stretched grid (its w control volume dz_c[k-1])."""
+    g = np.load(os.path.join(golden_dir, "ops_v2.npz"))
+    nx, ny, nz = int(g["nx"]), int(g["ny"]), int(g["nz"])
+    _, _, dz_f, dz_c = O.generate_grid(2.6, nz, float(g["Lz"]))
+    u, v, w = (torch.from_numpy(g[k]) for k in ("u", "v", "w"))
+    got = O.rhs_fused_v2(u, v, w, nx, ny, nz, float(g["Lx"]) / nx, float(g["Ly"]) / ny, dz_c, dz_f, float(g["nu"]))
+    for a, key in zip(got, ("rhs_u", "rhs_v", "rhs_w")):
+        assert _rel(a.numpy(), g[key]) < 1e-13, key
+
+
 def test_separate_rhs_equals_fused_on_interior(ops):
     """test_fused_kernels.py:81-83/176-178 idea: the two branches agree off the faces nx/ny."""
     nx, ny, nz = int(ops["nx"]), int(ops["ny"]), int(ops["nz"])

@@ -53,6 +53,39 @@ def test_package_hybrid_grid_matches_oracle():
             assert torch.equal(a, b)
 
 
+def _v2_error(nz, gamma, comp):
+    """max error of the oracle's FE-scheme RHS on a single-mode z profile against its analytic
+    value: u = cos(pi z) (v = w = 0) gives nu u''; w = sin(pi z / 2)^2 on faces (u = v = 0)
+    gives nu w'' - (w^2)'."""
+    nx, ny, Lz, nu = 4, 4, 2.0, 0.1
+    z_f, z_c, dz_f, dz_c = O.generate_grid(gamma, nz, Lz)
+    u = torch.zeros(nx + 1, ny + 2, nz + 2, dtype=torch.float64)
+    v = torch.zeros(nx + 2, ny + 1, nz + 2, dtype=torch.float64)
+    w = torch.zeros(nx + 2, ny + 2, nz + 1, dtype=torch.float64)
+    if comp == "u":
+        u[:] = torch.cos(math.pi * z_c).view(1, 1, -1)
+        ru, _, _ = O.rhs_fused_v2(u, v, w, nx, ny, nz, 1.0, 1.0, dz_c, dz_f, nu)
+        exact = -nu * math.pi ** 2 * torch.cos(math.pi * z_c[1:nz + 1])
+        return float((ru[2, 2, 1:nz + 1] - exact).abs().max())
+    f = lambda z: torch.sin(0.5 * math.pi * z) ** 2
+    w[:] = f(z_f).view(1, 1, -1)
+    _, _, rw = O.rhs_fused_v2(u, v, w, nx, ny, nz, 1.0, 1.0, dz_c, dz_f, nu)
+    z = z_f[1:nz]
+    w2 = 0.5 * math.pi ** 2 * torch.cos(math.pi * z)                 # d2/dz2 sin^2(pi z/2)
+    ww = 2 * f(z) * 0.5 * math.pi * torch.sin(math.pi * z)          # d/dz (w^2) = 2 w w'
+    return float((rw[2, 2, 1:nz] - (nu * w2 - ww)).abs().max())
+
+
+@pytest.mark.parametrize("comp,gamma,order", [("u", 1e-6, 2), ("w", 1e-6, 2), ("u", 2.6, 1), ("w", 2.6, 1)])
+def test_fused_v2_converges_on_smooth_profiles(comp, gamma, order):
+    """operators.py:904-909 / 1021-1026: the explicit z terms converge at second order on a
+    uniform grid and at first order on the tanh-stretched one (three-point differences of
+    centre values over uneven cells)."""
+    e = [_v2_error(nz, gamma, comp) for nz in (32, 64, 128)]
+    lo = 3.5 if order == 2 else 1.8
+    assert e[0] / e[1] > lo and e[1] / e[2] > lo, e
+
+
 def _banded(a, b, c, d):
     n = len(b)
     ab = np.zeros((3, n))

@@ -47,6 +47,7 @@ SIGNATURES = {
     "tc_grid_get_tuning": [_vp, _i, ctypes.POINTER(_i)],
     "tc_apply_bc": [_vp, _vp, _vp, _vp, _vp],
     "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "tc_rhs_fused_v2": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "tc_cn_solve": [_vp, _i, _d, _vp, _d, _vp, _vp, _vp],
     "tc_pcr_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "tc_thomas_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],

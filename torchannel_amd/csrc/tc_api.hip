@@ -194,6 +194,12 @@ int tc_rhs_imex(const tc_grid* h, int branch, const void* u, const void* v, cons
                    nullptr, nullptr, g.scal, nullptr, TC_K1_RHS, branch, 0.5, (hipStream_t)stream);
 }
 
+int tc_rhs_fused_v2(const tc_grid* h, const void* u, const void* v, const void* w, void* ru, void* rv, void* rw,
+                    void* stream) {
+  if (!h || !u || !v || !w || !ru || !rv || !rw) return TC_ERR_ARG;
+  return launch_rhs_v2(&h->g, u, v, w, ru, rv, rw, (hipStream_t)stream);
+}
+
 int tc_cn_solve(const tc_grid* h, int comp, double dt, const double* dt_dev, double theta,
                 const void* q_in, void* q_out, void* stream) {
   if (!h || !q_in || !q_out || comp < 0 || comp > 2) return TC_ERR_ARG;

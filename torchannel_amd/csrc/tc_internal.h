@@ -96,6 +96,8 @@ int launch_project(const Grid* gr, void* u, void* v, void* w, const void* p, dou
 int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0, long long ws1,
                 long long ws2, double inv_vol, double* out, double* partials, unsigned* counter,
                 int nblocks_max, hipStream_t st);
+int launch_rhs_v2(const Grid* gr, const void* u, const void* v, const void* w, void* ru, void* rv, void* rw,
+                  hipStream_t st);
 int launch_div_max(const Grid* gr, const void* u, const void* v, const void* w, double* out, double* partials,
                    int nblocks_max, hipStream_t st);
 int launch_diag_row(const double* scal, double* rows, int capacity, hipStream_t st);

@@ -123,6 +123,92 @@ __global__ __launch_bounds__(256) void k_divergence(Dims d, Metric<T> mt, const 
   div[t] = (u[cu] - u[cu - d.sxu]) * mt.rdx + (v[cv] - v[cv - d.syv]) * mt.rdy + (w[cw] - w[cw - 1]) * mt.rdzf[k];
 }
 
+// compute_momentum_rhs_fused_v2 (operators.py:817-1034): advection plus the full 3-D diffusion
+// (the z second derivative explicit too), on new arrays that are zero outside the interiors
+// u: i = 1..nx-1, j = 1..ny, k = 1..nz; v: i = 1..nx, j = 1..ny-1, k = 1..nz;
+// w: i = 1..nx, j = 1..ny, k = 1..nz-1.  One thread per output entry, the three arrays in turn.
+// The w z-terms divide by dz_f / dz_c as the reference does, with its control volume dz_c[k-1]
+// for the w second derivative (operators.py:1021-1026).
+template <typename T>
+__global__ __launch_bounds__(256) void k_rhs_v2(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                                const T* __restrict__ v, const T* __restrict__ w,
+                                                T* __restrict__ ru, T* __restrict__ rv, T* __restrict__ rw) {
+  const long long nu_ = (long long)(d.nx + 1) * d.sxu, nv_ = (long long)(d.nx + 2) * d.sxv;
+  const long long nw_ = (long long)(d.nx + 2) * d.sxw;
+  long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+  const T h = T(0.5), nu = (T)mt.nu;
+  if (t < nu_) {
+    const int i = (int)(t / d.sxu), r = (int)(t - (long long)i * d.sxu), j = r / d.syu, k = r - j * d.syu;
+    T val = T(0);
+    if (i >= 1 && i <= d.nx - 1 && j >= 1 && j <= d.ny && k >= 1 && k <= d.nz) {
+      const int c = (int)t, X = d.sxu, Y = d.syu;
+      const int cv = i * d.sxv + j * d.syv + k, cw = i * d.sxw + j * d.syw + k;
+      const T ur = h * (u[c] + u[c + X]), ul = h * (u[c - X] + u[c]);
+      const T duudx = (ur * ur - ul * ul) * mt.rdx;
+      const T vt = h * (v[cv] + v[cv + d.sxv]), ut = h * (u[c] + u[c + Y]);
+      const T vb = h * (v[cv - d.syv] + v[cv + d.sxv - d.syv]), ub = h * (u[c - Y] + u[c]);
+      const T dvudy = (vt * ut - vb * ub) * mt.rdy;
+      const T wt = h * (w[cw] + w[cw + d.sxw]), uzt = h * (u[c] + u[c + 1]);
+      const T wb = h * (w[cw - 1] + w[cw + d.sxw - 1]), uzb = h * (u[c - 1] + u[c]);
+      const T dwudz = (wt * uzt - wb * uzb) * mt.rdzf[k - 1];
+      const T adv = duudx + dvudy + dwudz;
+      const T dxx = (u[c + X] - T(2) * u[c] + u[c - X]) * mt.kxx;
+      const T dyy = (u[c + Y] - T(2) * u[c] + u[c - Y]) * mt.kyy;
+      const T dzz = nu * ((u[c + 1] - u[c]) * mt.rdzc[k] - (u[c] - u[c - 1]) * mt.rdzc[k - 1]) * mt.rdzf[k - 1];
+      val = dxx + dyy + dzz - adv;
+    }
+    ru[t] = val;
+    return;
+  }
+  t -= nu_;
+  if (t < nv_) {
+    const int i = (int)(t / d.sxv), r = (int)(t - (long long)i * d.sxv), j = r / d.syv, k = r - j * d.syv;
+    T val = T(0);
+    if (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny - 1 && k >= 1 && k <= d.nz) {
+      const int c = (int)t, X = d.sxv, Y = d.syv;
+      const int cu = i * d.sxu + j * d.syu + k, cw = i * d.sxw + j * d.syw + k;
+      const T ur = h * (u[cu] + u[cu + d.syu]), vxr = h * (v[c] + v[c + X]);
+      const T ul = h * (u[cu - d.sxu] + u[cu - d.sxu + d.syu]), vxl = h * (v[c - X] + v[c]);
+      const T duvdx = (ur * vxr - ul * vxl) * mt.rdx;
+      const T vtp = h * (v[c] + v[c + Y]), vbt = h * (v[c - Y] + v[c]);
+      const T dvvdy = (vtp * vtp - vbt * vbt) * mt.rdy;
+      const T wt = h * (w[cw] + w[cw + d.syw]), vzt = h * (v[c] + v[c + 1]);
+      const T wb = h * (w[cw - 1] + w[cw + d.syw - 1]), vzb = h * (v[c - 1] + v[c]);
+      const T dwvdz = (wt * vzt - wb * vzb) * mt.rdzf[k - 1];
+      const T adv = duvdx + dvvdy + dwvdz;
+      const T dxx = (v[c + X] - T(2) * v[c] + v[c - X]) * mt.kxx;
+      const T dyy = (v[c + Y] - T(2) * v[c] + v[c - Y]) * mt.kyy;
+      const T dzz = nu * ((v[c + 1] - v[c]) * mt.rdzc[k] - (v[c] - v[c - 1]) * mt.rdzc[k - 1]) * mt.rdzf[k - 1];
+      val = dxx + dyy + dzz - adv;
+    }
+    rv[t] = val;
+    return;
+  }
+  t -= nv_;
+  if (t < nw_) {
+    const int i = (int)(t / d.sxw), r = (int)(t - (long long)i * d.sxw), j = r / d.syw, k = r - j * d.syw;
+    T val = T(0);
+    if (i >= 1 && i <= d.nx && j >= 1 && j <= d.ny && k >= 1 && k <= d.nz - 1) {
+      const int c = (int)t, X = d.sxw, Y = d.syw;
+      const int cu = i * d.sxu + j * d.syu + k, cv = i * d.sxv + j * d.syv + k;
+      const T uzr = h * (u[cu] + u[cu + 1]), wxr = h * (w[c] + w[c + X]);
+      const T uzl = h * (u[cu - d.sxu] + u[cu - d.sxu + 1]), wxl = h * (w[c - X] + w[c]);
+      const T duwdx = (uzr * wxr - uzl * wxl) * mt.rdx;
+      const T vzt = h * (v[cv] + v[cv + 1]), wyt = h * (w[c] + w[c + Y]);
+      const T vzb = h * (v[cv - d.syv] + v[cv - d.syv + 1]), wyb = h * (w[c - Y] + w[c]);
+      const T dvwdy = (vzt * wyt - vzb * wyb) * mt.rdy;
+      const T wtp = h * (w[c] + w[c + 1]), wbt = h * (w[c - 1] + w[c]);
+      const T dwwdz = (wtp * wtp - wbt * wbt) / (T)mt.dzc[k];
+      const T adv = duwdx + dvwdy + dwwdz;
+      const T dxx = (w[c + X] - T(2) * w[c] + w[c - X]) * mt.kxx;
+      const T dyy = (w[c + Y] - T(2) * w[c] + w[c - Y]) * mt.kyy;
+      const T dzz = nu * ((w[c + 1] - w[c]) / (T)mt.dzf[k] - (w[c] - w[c - 1]) / (T)mt.dzf[k - 1]) / (T)mt.dzc[k - 1];
+      val = dxx + dyy + dzz - adv;
+    }
+    rw[t] = val;
+  }
+}
+
 // max |compute_divergence| (utils.py:235-240) over the interior: per-block maxima (the maximum
 // does not depend on the order), folded by k_finalize.
 template <typename T>
@@ -432,6 +518,21 @@ int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0,
   TC_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_BULK, (const double*)partials, blocks, out,
                      (double*)nullptr, inv_vol, 0.0, 0.0, 0);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+int launch_rhs_v2(const Grid* gr, const void* u, const void* v, const void* w, void* ru, void* rv, void* rw,
+                  hipStream_t st) {
+  const Dims& d = gr->d;
+  const long long n = (long long)(d.nx + 1) * d.sxu + (long long)(d.nx + 2) * d.sxv + (long long)(d.nx + 2) * d.sxw;
+  const int blocks = (int)((n + 255) / 256);
+  if (gr->dtype == TC_F64)
+    hipLaunchKernelGGL(k_rhs_v2<double>, dim3(blocks), dim3(256), 0, st, d, gr->metric<double>(), (const double*)u,
+                       (const double*)v, (const double*)w, (double*)ru, (double*)rv, (double*)rw);
+  else
+    hipLaunchKernelGGL(k_rhs_v2<float>, dim3(blocks), dim3(256), 0, st, d, gr->metric<float>(), (const float*)u,
+                       (const float*)v, (const float*)w, (float*)ru, (float*)rv, (float*)rw);
   TC_CHECK_LAUNCH();
   return TC_OK;
 }

@@ -32,6 +32,17 @@ def compute_momentum_rhs_separate(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
     return _rhs(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu, L.TC_BRANCH_CPU)
 
 
+def compute_momentum_rhs_fused_v2(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
+    """operators.py:817-1034: diffusion (x, y and z, all explicit) minus advection for the FE
+    scheme's right-hand side, zero outside the interiors (u-face nx and v-face ny included)."""
+    L.require_device(u, v, w)
+    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
+    ru, rv, rw = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+    L.call("tc_rhs_fused_v2", g.handle, L.ptr(u), L.ptr(v), L.ptr(w), L.ptr(ru), L.ptr(rv), L.ptr(rw),
+           L.stream_of(u))
+    return ru, rv, rw
+
+
 def _cn(q, comp, dt, nx, ny, nz, dz_c, dz_f, nu, theta, top):
     L.require_device(q)
     g = _grid_from_u(q, nx, ny, nz, 1.0, 1.0, dz_c, dz_f, nu, top)
