@@ -464,6 +464,57 @@ def initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f, top_wall_bc_type="diri
             "top_wall_bc_type": top_wall_bc_type}
 
 
+def build_poisson_matrix(nx, ny, nz, dx, dy, dz_c, dz_f, pin_pressure=True):
+    """Dense 7-point Laplacian of the direct solver; projection.py:3-84.
+
+    Unknown order i fastest: idx = i + j*nx + k*nx*ny (0-based).  Periodic in x and y; in z the
+    rows use 1/(dz_f[k] dz_c[k]) below and 1/(dz_f[k] dz_c[k+1]) above, and a wall drops its
+    neighbour (Neumann at both walls, whatever the top-wall type).  pin_pressure replaces row 0
+    by p_0 (projection.py:80-82).  Small grids only: N^2 entries.
+    """
+    N = nx * ny * nz
+    A = torch.zeros(N, N, dtype=F64)
+    ii, jj, kk = torch.meshgrid(torch.arange(nx), torch.arange(ny), torch.arange(nz), indexing="ij")
+    ii, jj, kk = ii.reshape(-1), jj.reshape(-1), kk.reshape(-1)
+    row = ii + jj * nx + kk * nx * ny
+    diag = torch.zeros(N, dtype=F64)
+    for di, dj, cf in ((-1, 0, 1.0 / dx ** 2), (1, 0, 1.0 / dx ** 2), (0, -1, 1.0 / dy ** 2), (0, 1, 1.0 / dy ** 2)):
+        col = (ii + di) % nx + ((jj + dj) % ny) * nx + kk * nx * ny
+        A[row, col] += cf
+        diag -= cf
+    dzf, dzc = dz_f.to(F64), dz_c.to(F64)
+    lo = kk > 0
+    cf = 1.0 / (dzf[kk] * dzc[kk])
+    A[row[lo], row[lo] - nx * ny] = cf[lo]
+    diag[lo] -= cf[lo]
+    hi = kk < nz - 1
+    cf = 1.0 / (dzf[kk] * dzc[kk + 1])
+    A[row[hi], row[hi] + nx * ny] = cf[hi]
+    diag[hi] -= cf[hi]
+    A[row, row] += diag
+    if pin_pressure:
+        A[0, :] = 0.0
+        A[0, 0] = 1.0
+    return A
+
+
+def solve_poisson(A, div, nx, ny, nz, top_wall_bc_type="dirichlet"):
+    """Direct solve of projection.py:87-140: least squares on A (the right-hand side is not
+    zeroed at a pinned row, so the pinned equation reads p_0 = b_0), then the ghosts: periodic
+    x, y; bottom Neumann; top Neumann for a no-slip wall, p = 0 for a free-slip one."""
+    b = div.to(F64).permute(2, 1, 0).reshape(-1)
+    sol = torch.linalg.lstsq(A, b.unsqueeze(1), driver="gelsd").solution.reshape(-1)
+    p = torch.zeros(nx + 2, ny + 2, nz + 2, dtype=F64)
+    p[1:nx + 1, 1:ny + 1, 1:nz + 1] = sol.reshape(nz, ny, nx).permute(2, 1, 0)
+    p[0] = p[nx]
+    p[nx + 1] = p[1]
+    p[:, 0] = p[:, ny]
+    p[:, ny + 1] = p[:, 1]
+    p[:, :, 0] = p[:, :, 1]
+    p[:, :, nz + 1] = 0.0 if top_wall_bc_type == "neumann" else p[:, :, nz]
+    return p
+
+
 def solve_poisson_fft(div, fft_data, gauge_zero_mode=False):
     """rfft2 -> per-mode Thomas -> irfft2 -> ghost fill; projection_fft.py:108-165.
 
@@ -723,6 +774,8 @@ PINS = {
     "project_velocity": "divergence <= 1e-12 after projection + golden trajectory",
     "initialize_fft_solver": "dense 7-point operator (projection.py:3-84) restated as a matrix",
     "solve_poisson_fft": "dense lstsq on the 7-point matrix (test_fft_vs_direct.py:97) + numpy.fft",
+    "build_poisson_matrix": "brute-force loop over the stencil; constant and linear-in-x/y null behaviour",
+    "solve_poisson": "residual of the dense system; FFT solve modulo the mean (unpinned matrix)",
     "compute_cfl_fused": "brute-force loop",
     "TurbulenceStats": "single-mode fields with known spectra + golden fixture from the reference",
     "Channel.step_imex": "golden 3-step trajectory from the reference ChannelFlow (cpu branch)",

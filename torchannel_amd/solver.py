@@ -94,6 +94,12 @@ class ChannelFlow:
         self.dt_max = t.get("dt_max", 0.01)
         self.dt_min = t.get("dt_min", 0.0001)
         self.time_scheme = t.get("scheme", "IMEX")
+        # the step's Poisson solve is the FFT one (solver.py:281-294); the reference's dense
+        # direct solver (projection.py:3-140) is small-grid test tooling here, not a step option
+        self.solver_type = config.get("solver", {}).get("type", "fft")
+        if self.solver_type != "fft":
+            raise ValueError(f"Unsupported solver type: {self.solver_type}. torchannel_amd steps with the FFT "
+                             "Poisson solver ('fft')")
         out = config.get("output", {})
         self.results_folder = out.get("results_folder", "results")
         self.n_out = out.get("n_out", out.get("print_interval", 10))
