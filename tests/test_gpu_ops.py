@@ -200,7 +200,8 @@ def test_bc_div_project_bulk_cfl_utau(tc, shape):
         assert abs(float(ut) - float(O.compute_u_tau(u, z_c, nu, top))) < 1e-14
 
 
-POISSON = [(16, 12, 16), (8, 6, 10), (64, 48, 40), (48, 96, 32), (192, 64, 20), (24, 18, 130)]
+POISSON = [(16, 12, 16), (8, 6, 10), (64, 48, 40), (48, 96, 32), (192, 64, 20), (24, 18, 130), (10, 8, 129),
+           (16, 12, 200), (8, 6, 256)]
 
 
 @pytest.mark.parametrize("shape", POISSON)
@@ -231,6 +232,28 @@ def test_poisson(tc, shape):
            + ((pc[1:-1, 1:-1, 2:] - pc[1:-1, 1:-1, 1:-1]) / dz_c[1:].view(1, 1, -1)
               - (pc[1:-1, 1:-1, 1:-1] - pc[1:-1, 1:-1, :-2]) / dz_c[:-1].view(1, 1, -1)) / dz_f.view(1, 1, -1))
     assert float((lap - div).abs().max()) < 1e-7 * float(div.abs().max())
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 40), (10, 8, 129), (16, 12, 200), (8, 6, 256)])
+def test_poisson_fp32(tc, shape):
+    """The float32 solve (float64 gauge mode) against the float64 oracle: float32 round-off
+    amplified by the low-wavenumber systems stays well below 1e-4."""
+    nx, ny, nz = shape
+    print(f"[poisson32] shape={shape}", flush=True)
+    Lx, Ly = 4 * math.pi, 4 * math.pi / 3
+    dx, dy = Lx / nx, Ly / ny
+    _, _, dz_f, dz_c = O.generate_grid(2.6, nz, 2.0)
+    rng = np.random.default_rng(8)
+    div = rng.standard_normal((nx, ny, nz))
+    div -= (div * dz_f.numpy()).sum() / (dz_f.numpy().sum() * nx * ny)
+    div = torch.from_numpy(div)
+    fd = O.initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f)
+    ref = O.solve_poisson_fft(div.float().double(), fd, gauge_zero_mode=True).clone()
+    gfd = tc.initialize_fft_solver(nx, ny, nz, dx, dy, dz_c.cuda(), dz_f.cuda(), dtype=torch.float32)
+    p = tc.solve_poisson_fft(div.float().cuda(), gfd)
+    assert p.dtype == torch.float32
+    pc = p.cpu().double()
+    assert rel(pc - pc.mean(), ref - ref.mean()) < 1e-4
 
 
 @pytest.mark.parametrize("shape", [(16, 12, 16), (64, 48, 40)])

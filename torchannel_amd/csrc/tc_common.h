@@ -88,16 +88,30 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (TC_WAVE - 1); }
 #define TC_DPP_ROW_SHL(n) (0x100 + (n))
 #define TC_DPP_ROW_SHR(n) (0x110 + (n))
 
-template <int CTRL>
+// row_bcast:15 (every lane of the rows enabled by the row mask takes lane 15 of the row below)
+#define TC_DPP_ROW_BCAST15 0x142
+
+template <int CTRL, int ROWMASK = 0xf>
 __device__ __forceinline__ float dpp_row(float old, float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, 0xf, 0xf, false));
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(x), CTRL, ROWMASK, 0xf, false));
 }
 
-template <int CTRL>
+template <int CTRL, int ROWMASK = 0xf>
 __device__ __forceinline__ double dpp_row(double old, double x) {
   const long long o = __double_as_longlong(old), v = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)v, CTRL, ROWMASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, ROWMASK, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// value of lane l (uniform l) for every lane
+__device__ __forceinline__ float readlane(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ double readlane(double x, int l) {
+  const long long v = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)v, l), hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 

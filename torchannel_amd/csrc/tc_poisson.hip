@@ -298,25 +298,20 @@ __device__ __forceinline__ void pcr_general(U (&a)[R], U (&b)[R], U (&c)[R], U (
   }
 }
 
-// P3: one wall-normal Thomas solve per (kx, ky) system (projection_fft.py:184-215), the
-// reference's algorithm: PCR loses ~4 digits on the ill-conditioned low modes of this operator,
-// Thomas does not.  The factorisation depends only on the grid and (kx, ky), so c'_k and 1/pivot_k
-// are computed once per plan (extended precision on the host) and stored [k][system]; per step
-// each system is one forward and one backward sweep.  A block stages `sb` consecutive systems
-// (one contiguous chunk of S) in LDS; lane l of wave 0 sweeps system s0 + l.  The singular
-// (0,0) system is gauge-fixed (first unknown pinned to zero) and swept in float64.
 // P3: one wall-normal solve per (kx, ky) system (projection_fft.py:184-215) with the Thomas
 // factorisation computed once per plan (extended precision on the host, stored [system][k]):
 // forward y_k = (d_k - a_k y_{k-1}) / pivot_k, backward x_k = y_k - c'_k x_{k+1}.  Both are
 // first-order linear recurrences, evaluated here as scans of affine maps y -> A y + B, whose
-// composition is associative: a system is one DPP row of 16 lanes, lane g holds rows
-// g*R .. g*R+R-1, composes its R maps in registers, and the 16 lane maps are combined by a
-// Hillis-Steele scan over row_shr (forward) / row_shl (backward) moves.  The products of
-// |a_k / pivot_k| <= 1 and |c'_k| <= 1 (diagonally dominant rows) keep every partial map
-// bounded, so the scan carries the sequential sweep's round-off.  The singular (0,0) system is
-// gauge-fixed (row 0 pinned to zero, row 1 decoupled from it) and solved in float64.
-#define TH_L 16
-#define TH_SYS_PER_BLOCK (256 / TH_L)
+// composition is associative: a system is a group of L = 16 lanes (one DPP row; L = 32, two rows,
+// above 128 rows), lane g holds rows g*R .. g*R+R-1, composes its R maps in registers, and the L
+// lane maps are combined by a Hillis-Steele scan over row_shr (forward) / row_shl (backward)
+// moves, with one more stage across the two rows of a 32-lane group (row_bcast:15 forward,
+// lanes 16 / 48 through scalar registers backward; readlane only reads, every lane of the
+// wavefront executes it).  Rows L*R > nz are identity maps.  The products of |a_k / pivot_k| <= 1
+// and |c'_k| <= 1 (diagonally dominant rows) keep every partial map bounded, so the scan carries
+// the sequential sweep's round-off.  The singular (0,0) system is gauge-fixed (row 0 pinned to
+// zero, row 1 decoupled from it) and solved in float64.
+#define TH_THREADS 256
 
 template <int CTRL, typename U>
 __device__ __forceinline__ void th_combine(U& a, cplx<U>& b) {
@@ -329,7 +324,8 @@ __device__ __forceinline__ void th_combine(U& a, cplx<U>& b) {
 template <typename U, typename T, typename TT, int RM>
 __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT* __restrict__ cps,
                                             const TT* __restrict__ ids, const double* __restrict__ cl, int nz,
-                                            int R, int g, bool gauge, bool store) {
+                                            int R, int L, int g, bool gauge, bool store) {
+  const int wl = lane_id();
   const int m0 = g * R;
   U A[RM], C[RM];
   cplx<U> B[RM];
@@ -365,7 +361,20 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   th_combine<TC_DPP_ROW_SHR(2)>(pa, pb);
   th_combine<TC_DPP_ROW_SHR(4)>(pa, pb);
   th_combine<TC_DPP_ROW_SHR(8)>(pa, pb);
-  const U ein_r = dpp_row<TC_DPP_ROW_SHR(1)>(U(0), pb.re), ein_i = dpp_row<TC_DPP_ROW_SHR(1)>(U(0), pb.im);
+  if (L == 32) {
+    // rows 1 and 3 take the prefix of rows 0 and 2 (their last lanes); the A part is not needed
+    const U br = dpp_row<TC_DPP_ROW_BCAST15, 0xa>(U(0), pb.re), bi = dpp_row<TC_DPP_ROW_BCAST15, 0xa>(U(0), pb.im);
+    pb = {pa * br + pb.re, pa * bi + pb.im};
+  }
+  U ein_r = dpp_row<TC_DPP_ROW_SHR(1)>(U(0), pb.re), ein_i = dpp_row<TC_DPP_ROW_SHR(1)>(U(0), pb.im);
+  if (L == 32) {
+    // the first lane of the group's second row continues from the last lane of its first row
+    const U r15 = readlane(pb.re, 15), i15 = readlane(pb.im, 15), r47 = readlane(pb.re, 47), i47 = readlane(pb.im, 47);
+    if ((wl & 31) == 16) {
+      ein_r = wl < 32 ? r15 : r47;
+      ein_i = wl < 32 ? i15 : i47;
+    }
+  }
   // y rows; backward maps x -> -c'_k x + y_k (rows past nz: x = 0)
 #pragma unroll
   for (int r = 0; r < RM; ++r) {
@@ -392,7 +401,19 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   th_combine<TC_DPP_ROW_SHL(2)>(qa, qb);
   th_combine<TC_DPP_ROW_SHL(4)>(qa, qb);
   th_combine<TC_DPP_ROW_SHL(8)>(qa, qb);
-  const U xin_r = dpp_row<TC_DPP_ROW_SHL(1)>(U(0), qb.re), xin_i = dpp_row<TC_DPP_ROW_SHL(1)>(U(0), qb.im);
+  U sr = U(0), si = U(0);
+  if (L == 32) {
+    // rows 0 and 2 continue into the suffix of rows 1 and 3 (held by lanes 16 / 48)
+    const U r16 = readlane(qb.re, 16), i16 = readlane(qb.im, 16), r48 = readlane(qb.re, 48), i48 = readlane(qb.im, 48);
+    sr = wl < 32 ? r16 : r48;
+    si = wl < 32 ? i16 : i48;
+    if ((wl & 16) == 0) qb = {qa * sr + qb.re, qa * si + qb.im};
+  }
+  U xin_r = dpp_row<TC_DPP_ROW_SHL(1)>(U(0), qb.re), xin_i = dpp_row<TC_DPP_ROW_SHL(1)>(U(0), qb.im);
+  if (L == 32 && (wl & 31) == 15) {
+    xin_r = sr;
+    xin_i = si;
+  }
   if (!store) return;
 #pragma unroll
   for (int r = 0; r < RM; ++r) {
@@ -403,20 +424,22 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   }
 }
 
-// rows per lane of a system of nz rows
+// lanes per system (16, or 32 above 128 rows) and rows per lane of a system of nz rows
+__host__ __device__ __forceinline__ int thomas_lanes(int nz) { return nz <= 128 ? 16 : 32; }
 __host__ __device__ __forceinline__ int thomas_rows_per_lane(int nz) {
-  const int r = (nz + TH_L - 1) / TH_L;
-  return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : r <= 8 ? 8 : 16;
+  const int L = thomas_lanes(nz);
+  const int r = (nz + L - 1) / L;
+  return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : 8;
 }
 
-// Block = sb systems of TH_L lanes each; R <= RM rows per lane.
+// Block = sb systems of L lanes each; R <= RM rows per lane.
 template <typename T, int RM>
-__device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int sb, const double* __restrict__ cl,
+__device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int L, int sb, const double* __restrict__ cl,
                                             const T* __restrict__ cp, const T* __restrict__ iden,
                                             const double* __restrict__ cp0, const double* __restrict__ iden0,
                                             cplx<T>* __restrict__ S, int blk) {
-  const int lane = threadIdx.x & (TH_L - 1);
-  const int s = blk * sb + (int)(threadIdx.x / TH_L);
+  const int lane = threadIdx.x & (L - 1);
+  const int s = blk * sb + (int)(threadIdx.x / L);
   const bool live = s < nsys;
   const int sc = live ? s : nsys - 1;
   cplx<T>* col = S + (size_t)sc * nz;
@@ -424,10 +447,10 @@ __device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int sb, con
   if constexpr (sizeof(T) == sizeof(double)) {
     const bool gauge = sc == 0;
     thomas_rows<double, T, double, RM>(col, gauge ? cp0 : (const double*)cp + to,
-                                       gauge ? iden0 : (const double*)iden + to, cl, nz, R, lane, gauge, live);
+                                       gauge ? iden0 : (const double*)iden + to, cl, nz, R, L, lane, gauge, live);
   } else {
-    if (sc == 0) thomas_rows<double, T, double, RM>(col, cp0, iden0, cl, nz, R, lane, true, live);
-    else thomas_rows<T, T, T, RM>(col, cp + to, iden + to, cl, nz, R, lane, false, live);
+    if (sc == 0) thomas_rows<double, T, double, RM>(col, cp0, iden0, cl, nz, R, L, lane, true, live);
+    else thomas_rows<T, T, T, RM>(col, cp + to, iden + to, cl, nz, R, L, lane, false, live);
   }
 }
 
@@ -437,7 +460,8 @@ __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb
                                                         const double* __restrict__ cp0,
                                                         const double* __restrict__ iden0,
                                                         cplx<T>* __restrict__ S) {
-  thomas_body<T, RM>(nsys, nz, RM, sb, cl, cp, iden, cp0, iden0, S, xcd_remap(blockIdx.x, gridDim.x));
+  thomas_body<T, RM>(nsys, nz, RM, thomas_lanes(nz), sb, cl, cp, iden, cp0, iden0, S,
+                     xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Ensemble forms of P1-P5: global block -> (channel, block of that channel) via prefix sums.
@@ -466,7 +490,8 @@ __global__ __launch_bounds__(256) void k_poisson_thomas_ens(const EnsChan<T>* __
   const int gb = xcd_remap(blockIdx.x, gridDim.x);
   const int q = ens_chan(start, n, gb);
   const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
-  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), TH_SYS_PER_BLOCK, c.cl, c.th_cp,
+  const int L = thomas_lanes(c.d.nz);
+  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), L, TH_THREADS / L, c.cl, c.th_cp,
                      c.th_iden, c.th_cp0, c.th_iden0, c.spec, gb - start[q]);
 }
 
@@ -759,16 +784,17 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 1: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, -1, st);
     case 2: {
       const int nsys = pl->nkx * d.ny;
-      const int sb = pl->g->thomas_sb > 0 ? pl->g->thomas_sb : TH_SYS_PER_BLOCK;
+      const int L = thomas_lanes(d.nz);
+      const int sb = std::min(pl->g->thomas_sb > 0 ? pl->g->thomas_sb : 16, TH_THREADS / L);
       const int blocks = (nsys + sb - 1) / sb;
       switch (thomas_rows_per_lane(d.nz)) {
-#define TC_TH(RM)                                                                                      \
-  case RM:                                                                                             \
-    hipLaunchKernelGGL((k_poisson_thomas<T, RM>), dim3(blocks), dim3(sb * TH_L), 0, st, nsys, d.nz, sb, \
-                       pl->cl, (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,  \
-                       (cplx<T>*)pl->spec);                                                            \
+#define TC_TH(RM)                                                                                     \
+  case RM:                                                                                            \
+    hipLaunchKernelGGL((k_poisson_thomas<T, RM>), dim3(blocks), dim3(sb * L), 0, st, nsys, d.nz, sb,  \
+                       pl->cl, (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0, \
+                       (cplx<T>*)pl->spec);                                                           \
     break;
-        TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8) TC_TH(16)
+        TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8)
 #undef TC_TH
         default: return TC_ERR_UNSUPPORTED;
       }
@@ -862,7 +888,8 @@ void poisson_geometry(const Poisson* pl, int tsz, int* blocks, size_t* lds) {
   lds[0] = tsz == 8 ? fft_lds<double>(pl->pairs, d.nx) : fft_lds<float>(pl->pairs, d.nx);
   blocks[1] = pl->nkx * ((d.nz + pl->kzy - 1) / pl->kzy);
   lds[1] = tsz == 8 ? fft_lds<double>(pl->kzy, d.ny) : fft_lds<float>(pl->kzy, d.ny);
-  blocks[2] = (pl->nkx * d.ny + TH_SYS_PER_BLOCK - 1) / TH_SYS_PER_BLOCK;
+  const int tsb = TH_THREADS / thomas_lanes(d.nz);
+  blocks[2] = (pl->nkx * d.ny + tsb - 1) / tsb;
   lds[2] = 0;
   blocks[3] = blocks[0];
   lds[3] = lds[0];
@@ -881,7 +908,7 @@ static int ens_poisson_t(const void* dch, int n, const int* const* start, const 
   case RM:                                                                                              \
     hipLaunchKernelGGL((k_poisson_thomas_ens<T, RM>), dim3(blocks[2]), dim3(256), 0, st, c, start[2], n); \
     break;
-    TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8) TC_TH(16)
+    TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8)
 #undef TC_TH
     default: return TC_ERR_UNSUPPORTED;
   }

@@ -1,8 +1,10 @@
 """FFT + tridiagonal pressure Poisson solver (projection_fft.py) on the HIP library.
 
 The solve is hand-written: LDS-resident batched FFTs along x (real, two z-rows per complex
-transform) and y, one wavefront-level PCR per (kx, ky) wall-normal system, inverse FFTs, and the
-ghost images of p written by the last kernel.  The singular (0,0) mode is gauge-fixed in float64.
+transform) and y, one wall-normal Thomas solve per (kx, ky) system (its two sweeps evaluated as
+parallel scans of affine maps across 16 or 32 lanes, factors precomputed per plan), inverse FFTs,
+and the ghost images of p written by the last kernel.  The singular (0,0) mode is gauge-fixed in
+float64.
 """
 import math
 
