@@ -47,7 +47,7 @@ def _pmod(p):
     return p - p.mean()
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 16), (32, 24, 48), (24, 18, 130), (16, 12, 256)])
+@pytest.mark.parametrize("shape", [(16, 12, 16), (32, 24, 48), (24, 18, 130), (16, 12, 256), (288, 6, 12)])
 @pytest.mark.parametrize("top", ["dirichlet", "neumann"])
 def test_steps_match_oracle(tc, shape, top):
     nx, ny, nz = shape

@@ -1,13 +1,16 @@
 // LDS-resident batched complex FFT (Stockham autosort, radices 8, 4, 2, 3, 5; N = 2^a 3^b 5^c).
 //
 // A block of 256 threads transforms `batch` sequences of length N held in LDS (sequence b at
-// buf[b*ldb + 0..N)), batch*N <= TC_FFT_ELEMS.  Pass with radix R and span p (p = product of the
+// buf[b*ldb + 0..N)), batch*N <= the pass's element budget (TC_FFT_ELEMS unless stated).  Pass with radix R and span p (p = product of the
 // radices already applied): butterfly idx in [0, N/R), k = idx mod p, inputs x[idx + r*N/R],
 // twiddles exp(sign*2*pi*i*r*k/(p*R)), outputs at (idx-k)*R + k + r*p.
 #pragma once
 #include "tc_common.h"
 
 #define TC_FFT_ELEMS 2048
+// budget of the forward x transform of long rows (nx > 256): twice the z rows per block, so each
+// x row is read in segments twice as long; its passes hold twice the registers
+#define TC_FFT_ELEMS_WIDE 4096
 #define TC_FFT_MAXPASS 12
 
 struct FftDesc {
@@ -96,10 +99,10 @@ __device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
   }
 }
 
-template <typename T, int R>
+template <typename T, int R, int ELEMS>
 __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N, int p, unsigned mag_m,
                                          unsigned mag_p, const cplx<T>* __restrict__ twl, int sign) {
-  constexpr int MB = (TC_FFT_ELEMS / R + 255) / 256;
+  constexpr int MB = (ELEMS / R + 255) / 256;
   const int M = N / R;
   const int total = batch * M;
   const int tstride = N / (p * R);
@@ -147,17 +150,18 @@ __device__ __forceinline__ void fft_stage_twiddles(cplx<T>* twl, const double2* 
   }
 }
 
-template <typename T>
+// batch * f.n <= ELEMS
+template <typename T, int ELEMS = TC_FFT_ELEMS>
 __device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const cplx<T>* __restrict__ twl,
                         int sign) {
   int p = 1;
   for (int ps = 0; ps < f.npass; ++ps) {
     const int R = f.radix[ps];
-    if (R == 8) fft_pass<T, 8>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 4) fft_pass<T, 4>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 2) fft_pass<T, 2>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 3) fft_pass<T, 3>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else fft_pass<T, 5>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    if (R == 8) fft_pass<T, 8, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 4) fft_pass<T, 4, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 2) fft_pass<T, 2, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 3) fft_pass<T, 3, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else fft_pass<T, 5, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     p *= R;
   }
 }

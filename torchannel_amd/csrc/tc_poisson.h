@@ -20,6 +20,8 @@ struct Poisson {
   double* K2 = nullptr;        // kx~^2 + ky~^2 per (m, j) (projection_fft.py:30-39)
   void* spec = nullptr;        // S[nkx][ny][nz] complex, working precision
   int pairs = 1;               // z-row pairs per x-transform block
+  int pairs_fx = 1;            // ... of the solve's forward x transform
+  bool wide_fx = false;        // that transform runs with the TC_FFT_ELEMS_WIDE budget
   int kzy = 1;                 // z-rows per y-transform block
   void* th_cp = nullptr;       // Thomas c'_k per system, [k][system], working precision
   void* th_iden = nullptr;     // Thomas 1/pivot_k per system, [k][system]

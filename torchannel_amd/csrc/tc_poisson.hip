@@ -30,7 +30,7 @@ __host__ __device__ __forceinline__ size_t fft_lds(int rows, int n) {
 // ghost columns are not stored: x-face 0 is the image of face nx, y-face 0 of face ny, and the
 // w walls are zero).  Otherwise rhs is read from a (nx, ny, nz_in) array (planes of nz_in).
 // ---------------------------------------------------------------------------------------------
-template <typename T, bool FROM_VEL>
+template <typename T, bool FROM_VEL, int ELEMS = TC_FFT_ELEMS>
 __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __restrict__ u,
                                                const T* __restrict__ v, const T* __restrict__ w,
                                                const T* __restrict__ rin, int nz_in,
@@ -87,7 +87,7 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
     }
   }
   __syncthreads();
-  lds_fft<T>(Z, ldb, pairs, fx, twl, -1);
+  lds_fft<T, ELEMS>(Z, ldb, pairs, fx, twl, -1);
   const int nkx = nx / 2 + 1;
   for (int t = threadIdx.x; t < nkx * KZ; t += 256) {
     const int kk = t & (KZ - 1), m = t >> lkz;
@@ -102,14 +102,15 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
   }
 }
 
-template <typename T, bool FROM_VEL>
+template <typename T, bool FROM_VEL, int ELEMS>
 __global__ __launch_bounds__(256) void k_fwd_x(Dims d, Metric<T> mt, const T* __restrict__ u,
                                                const T* __restrict__ v, const T* __restrict__ w,
                                                const T* __restrict__ rin, int nz_in,
                                                const double* __restrict__ scal, FftDesc fx,
                                                const double2* __restrict__ twx, cplx<T>* __restrict__ S,
                                                int pairs) {
-  fwd_x_body<T, FROM_VEL>(d, mt, u, v, w, rin, nz_in, scal, fx, twx, S, pairs, xcd_remap(blockIdx.x, gridDim.x));
+  fwd_x_body<T, FROM_VEL, ELEMS>(d, mt, u, v, w, rin, nz_in, scal, fx, twx, S, pairs,
+                                 xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // P2 / P4: complex FFT along y for every (kx, z) column.  Block = (kx, chunk of KZ z-rows).
@@ -644,6 +645,21 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   pl->pairs = std::min(pl->pairs, pow2_floor(std::max(1, (d.nz + 1) / 2)));
   pl->kzy = std::min(16, pow2_floor(std::max(1, TC_FFT_ELEMS / d.ny)));
   pl->kzy = std::min(pl->kzy, pow2_floor(d.nz));
+  // the forward x transform of the solve takes the wide budget where that doubles the z rows per
+  // block within 64 KiB of LDS: at nx = 384 each x row is then read in segments of 16 z rows
+  // instead of 8 (the inverse transform and the y transforms keep the narrow budget, which needs
+  // half the registers)
+  pl->pairs_fx = pl->pairs;
+  pl->wide_fx = false;
+  {
+    int pw = std::min(8, pow2_floor(std::max(1, TC_FFT_ELEMS_WIDE / d.nx)));
+    pw = std::min(pw, pow2_floor(std::max(1, (d.nz + 1) / 2)));
+    const size_t lw = g->dtype == TC_F64 ? fft_lds<double>(pw, d.nx) : fft_lds<float>(pw, d.nx);
+    if (pw > pl->pairs && lw <= 65536) {
+      pl->pairs_fx = pw;
+      pl->wide_fx = true;
+    }
+  }
   // twiddles exp(-2 pi i t / N) in float64
   auto make_tw = [](int n, double2** out) -> int {
     std::vector<double2> h(n);
@@ -738,24 +754,32 @@ void poisson_free(Poisson* pl) {
   hipFree(pl->th_cp); hipFree(pl->th_iden); hipFree(pl->th_cp0); hipFree(pl->th_iden0);
 }
 
+// pairs * nx must fit the budget the kernel is built for (TC_FFT_ELEMS, or TC_FFT_ELEMS_WIDE
+// with `wide`)
 template <typename T>
 int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* v, const void* w,
-                   const void* rin, int nz_in, const double* scal, void* spec, int pairs, hipStream_t st) {
+                   const void* rin, int nz_in, const double* scal, void* spec, int pairs, bool wide,
+                   hipStream_t st) {
   const Grid* g = pl->g;
   const Dims& d = g->d;
   const int KZ = 2 * pairs;
   if (pairs < 1 || (pairs & (pairs - 1)) != 0) return TC_ERR_ARG;
+  if (pairs * d.nx > (wide ? TC_FFT_ELEMS_WIDE : TC_FFT_ELEMS)) return TC_ERR_ARG;
   const int blocks = d.ny * ((nz_in + KZ - 1) / KZ);
   const size_t lds = fft_lds<T>(pairs, d.nx);
   const Metric<T> mt = g->metric<T>();
-  if (from_vel)
-    hipLaunchKernelGGL((k_fwd_x<T, true>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,
-                       (const T*)v, (const T*)w, (const T*)rin, nz_in, scal, pl->fx, pl->twx,
-                       (cplx<T>*)spec, pairs);
-  else
-    hipLaunchKernelGGL((k_fwd_x<T, false>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,
-                       (const T*)v, (const T*)w, (const T*)rin, nz_in, scal, pl->fx, pl->twx,
-                       (cplx<T>*)spec, pairs);
+#define TC_FWDX(FV, EL)                                                                               \
+  hipLaunchKernelGGL((k_fwd_x<T, FV, EL>), dim3(blocks), dim3(256), lds, st, d, mt, (const T*)u,     \
+                     (const T*)v, (const T*)w, (const T*)rin, nz_in, scal, pl->fx, pl->twx, (cplx<T>*)spec, \
+                     pairs)
+  if (from_vel) {
+    if (wide) TC_FWDX(true, TC_FFT_ELEMS_WIDE);
+    else TC_FWDX(true, TC_FFT_ELEMS);
+  } else {
+    if (wide) TC_FWDX(false, TC_FFT_ELEMS_WIDE);
+    else TC_FWDX(false, TC_FFT_ELEMS);
+  }
+#undef TC_FWDX
   TC_CHECK_LAUNCH();
   return TC_OK;
 }
@@ -780,7 +804,8 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
   const Grid* g = pl->g;
   const Dims& d = g->d;
   switch (ph) {
-    case 0: return launch_fwd_x_t<T>(pl, from_vel, u, v, w, rin, d.nz, scal, pl->spec, pl->pairs, st);
+    case 0:
+      return launch_fwd_x_t<T>(pl, from_vel, u, v, w, rin, d.nz, scal, pl->spec, pl->pairs_fx, pl->wide_fx, st);
     case 1: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, -1, st);
     case 2: {
       const int nsys = pl->nkx * d.ny;
@@ -836,7 +861,7 @@ int plane_fft(const Poisson* pl, hipStream_t st) {
   int rc;
   const int pairs = std::min(2, pow2_floor(std::max(1, TC_FFT_ELEMS / pl->g->d.nx)));
   if ((rc = launch_fwd_x_t<double>(pl, false, nullptr, nullptr, nullptr, pl->planes, 6, nullptr,
-                                   pl->plane_spec, pairs, st)) != TC_OK)
+                                   pl->plane_spec, pairs, false, st)) != TC_OK)
     return rc;
   const int kz = std::min(pow2_floor(std::max(1, TC_FFT_ELEMS / pl->g->d.ny)), 4);
   return launch_fft_y_t<double>(pl, pl->plane_spec, 6, kz, -1, st);
