@@ -8,7 +8,7 @@
 
 namespace tc {
 int launch_stats(const Poisson* pl, const void* u, const void* v, const void* w, int kb, int kt,
-                 double* prof, double* spec, double* means, double* partials, unsigned* counter,
+                 double* prof, double* spec, double* means, double* partials,
                  int nblocks, hipStream_t st);
 }
 
@@ -47,7 +47,6 @@ struct tc_solver {
   void *su = nullptr, *sv = nullptr, *sw = nullptr;     // post-CN velocities
   double* tabs = nullptr;                                // CN tables: cell, face
   double* partials = nullptr;                            // K5 bulk partials
-  unsigned* counters = nullptr;                          // [0] K5, [1] CFL
   double* cfl_partials = nullptr;
   double* init_scal = nullptr;                           // dt = 1 block for the initial projection
   int newest = -1;                                       // rhs set holding the newest rhs
@@ -116,7 +115,6 @@ int tc_grid_create(int nx, int ny, int nz, double dx, double dy, double nu, int 
             hipMalloc(&g.rdzf32, sizeof(float) * nz) == hipSuccess &&
             hipMalloc(&g.rdzc32, sizeof(float) * (nz + 1)) == hipSuccess &&
             hipMalloc(&g.scratch, sizeof(double) * g.scratch_len) == hipSuccess &&
-            hipMalloc(&g.counter, sizeof(unsigned) * 8) == hipSuccess &&
             hipMalloc(&g.tabs, sizeof(double) * 2 * TC_TAB_STRIDE) == hipSuccess &&
             hipMalloc(&g.scal, sizeof(double) * TC_NSCAL) == hipSuccess;
   if (ok) {
@@ -127,7 +125,6 @@ int tc_grid_create(int nx, int ny, int nz, double dx, double dy, double nu, int 
          hipMemcpy(g.wbulk, wb.data(), sizeof(double) * nz, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(g.rdzf32, rf32.data(), sizeof(float) * nz, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(g.rdzc32, rc32.data(), sizeof(float) * (nz + 1), hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemset(g.counter, 0, sizeof(unsigned) * 8) == hipSuccess &&
          hipMemset(g.scal, 0, sizeof(double) * TC_NSCAL) == hipSuccess;
   }
   if (!ok) {
@@ -142,7 +139,7 @@ int tc_grid_destroy(tc_grid* h) {
   if (!h) return TC_OK;
   Grid& g = h->g;
   hipFree(g.dzf64); hipFree(g.dzc64); hipFree(g.rdzf64); hipFree(g.rdzc64); hipFree(g.wbulk);
-  hipFree(g.rdzf32); hipFree(g.rdzc32); hipFree(g.scratch); hipFree(g.counter); hipFree(g.tabs);
+  hipFree(g.rdzf32); hipFree(g.rdzc32); hipFree(g.scratch); hipFree(g.tabs);
   hipFree(g.scal);
   delete h;
   return TC_OK;
@@ -243,7 +240,7 @@ int tc_bulk_velocity(const tc_grid* h, const void* u, const double* wgt, int64_t
                      int64_t ws2, double total_volume, double* out_dev, void* stream) {
   if (!h || !u || !wgt || !out_dev || !(total_volume != 0.0)) return TC_ERR_ARG;
   const Grid& g = h->g;
-  return launch_bulk(&g, u, wgt, ws0, ws1, ws2, 1.0 / total_volume, out_dev, g.scratch, g.counter,
+  return launch_bulk(&g, u, wgt, ws0, ws1, ws2, 1.0 / total_volume, out_dev, g.scratch,
                      kMaxRedBlocks, (hipStream_t)stream);
 }
 
@@ -251,14 +248,14 @@ int tc_cfl_rate(const tc_grid* h, const void* u, const void* v, const void* w, d
   if (!h || !u || !v || !w || !out_dev) return TC_ERR_ARG;
   const Grid& g = h->g;
   // the rate lands in scal[TC_S_CFL]; point scal so that slot is out_dev
-  return launch_cfl(&g, u, v, w, out_dev - TC_S_CFL, g.scratch, g.counter + 1, 0, 0.0, 0.0, 0.0,
+  return launch_cfl(&g, u, v, w, out_dev - TC_S_CFL, g.scratch, 0, 0.0, 0.0, 0.0,
                     kMaxRedBlocks, (hipStream_t)stream);
 }
 
 int tc_u_tau(const tc_grid* h, const void* u, double z_c1, double* out_dev, void* stream) {
   if (!h || !u || !out_dev) return TC_ERR_ARG;
   const Grid& g = h->g;
-  return launch_u_tau(&g, u, z_c1, out_dev, g.scratch, g.counter + 2, 2 * kMaxRedBlocks, (hipStream_t)stream);
+  return launch_u_tau(&g, u, z_c1, out_dev, g.scratch, 2 * kMaxRedBlocks, (hipStream_t)stream);
 }
 
 int tc_poisson_create(const tc_grid* h, tc_poisson** out) {
@@ -302,7 +299,7 @@ int tc_stats_accumulate(tc_poisson* pl, const void* u, const void* v, const void
   if (!pl || !u || !v || !w || !prof || !spec) return TC_ERR_ARG;
   const Grid* g = pl->p.g;
   if (k_bot < 1 || k_bot > g->d.nz || k_top < 1 || k_top > g->d.nz) return TC_ERR_ARG;
-  return launch_stats(&pl->p, u, v, w, k_bot, k_top, prof, spec, pl->means, g->scratch, g->counter + 3,
+  return launch_stats(&pl->p, u, v, w, k_bot, k_top, prof, spec, pl->means, g->scratch,
                       kStatsBlocks, (hipStream_t)stream);
 }
 
@@ -336,14 +333,12 @@ int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w
             hipMalloc(&s->sw, esz * nw_) == hipSuccess &&
             hipMalloc(&s->tabs, sizeof(double) * 2 * TC_TAB_STRIDE) == hipSuccess &&
             hipMalloc(&s->partials, sizeof(double) * k5_blocks(&g)) == hipSuccess &&
-            hipMalloc(&s->counters, sizeof(unsigned) * 4) == hipSuccess &&
             hipMalloc(&s->cfl_partials, sizeof(double) * kMaxRedBlocks) == hipSuccess &&
             hipMalloc(&s->init_scal, sizeof(double) * TC_NSCAL) == hipSuccess &&
             hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) == hipSuccess;
   if (ok) {
     double init[TC_NSCAL] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    ok = hipMemset(s->counters, 0, sizeof(unsigned) * 4) == hipSuccess &&
-         hipMemcpy(s->init_scal, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess;
+    ok = hipMemcpy(s->init_scal, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess;
   }
   if (!ok) {
     tc_solver_destroy(s);
@@ -359,7 +354,7 @@ int tc_solver_destroy(tc_solver* s) {
   if (s->graph) hipGraphDestroy(s->graph);
   if (s->cap_stream) hipStreamDestroy(s->cap_stream);
   hipFree(s->su); hipFree(s->sv); hipFree(s->sw); hipFree(s->tabs); hipFree(s->partials);
-  hipFree(s->counters); hipFree(s->cfl_partials); hipFree(s->init_scal);
+  hipFree(s->cfl_partials); hipFree(s->init_scal);
   delete s;
   return TC_OK;
 }
@@ -376,7 +371,7 @@ static int solver_step_impl(tc_solver* s, double dt, int first, const void* body
                   s->su, s->sv, s->sw, s->tabs, s->tabs + TC_TAB_STRIDE, s->scal, body, mode,
                   s->branch, 0.5, st));
   TC_RC(poisson_solve(s->pl, true, s->su, s->sv, s->sw, nullptr, s->scal, s->p, st));
-  TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->scal, s->partials, s->counters,
+  TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->scal, s->partials,
                   s->inv_volume, TC_K5_BULK | TC_K5_FORCING, st));
   s->newest = cur;
   return TC_OK;
@@ -414,7 +409,7 @@ int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
   // u_b of the initial field into scal[TC_S_UBULK] (cell volume dx*dy*dz_f expanded, solver.py:231)
   if (rescale) {
     TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
-                      g->counter, kMaxRedBlocks, st));
+                      kMaxRedBlocks, st));
     hipLaunchKernelGGL(k_rescale_u, dim3(1024), dim3(256), 0, st,
                        g->dtype == TC_F64 ? (double*)s->u : nullptr, g->dtype == TC_F64 ? nullptr : (float*)s->u,
                        nu_, s->scal);
@@ -425,13 +420,13 @@ int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
   TC_HIP(hipMemcpyAsync(s->sw, s->w, esz * nw_, hipMemcpyDeviceToDevice, st));
   TC_RC(poisson_solve(s->pl, true, s->su, s->sv, s->sw, nullptr, s->init_scal, s->p, st));
   TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->init_scal, s->partials,
-                  s->counters, s->inv_volume, 0, st));
+                  s->inv_volume, 0, st));
   return TC_OK;
 }
 
 int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream) {
   if (!s) return TC_ERR_ARG;
-  return launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, s->counters + 1, 1, cfl_target,
+  return launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, 1, cfl_target,
                     dt_min, dt_max, kMaxRedBlocks, (hipStream_t)stream);
 }
 
@@ -440,7 +435,7 @@ int tc_solver_diagnostics(tc_solver* s, double z_c1, double* rows, int capacity,
   hipStream_t st = (hipStream_t)stream;
   // u_tau into rows[1], max|div| into rows[2] (both through the solver's reduction partials,
   // in stream order), then the row itself
-  TC_RC(launch_u_tau(s->g, s->u, z_c1, rows + 1, s->cfl_partials, s->counters + 1, kMaxRedBlocks, st));
+  TC_RC(launch_u_tau(s->g, s->u, z_c1, rows + 1, s->cfl_partials, kMaxRedBlocks, st));
   TC_RC(launch_div_max(s->g, s->u, s->v, s->w, rows + 2, s->cfl_partials, kMaxRedBlocks, st));
   return launch_diag_row(s->scal, rows, capacity, st);
 }
@@ -458,7 +453,7 @@ int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_targe
   int rc = TC_OK;
   for (int i = 0; i < steps && rc == TC_OK; ++i) {
     if (dt_interval > 0 && (i + 1) % dt_interval == 0)
-      rc = launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, s->counters + 1, 1, cfl_target,
+      rc = launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, 1, cfl_target,
                       dt_min, dt_max, kMaxRedBlocks, cs);
     if (rc == TC_OK) rc = solver_step_impl(s, -1.0, 0, nullptr, cs);
   }
@@ -495,11 +490,11 @@ int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream) {
         TC_RC(poisson_phase(s->pl, phase - 2, true, s->su, s->sv, s->sw, nullptr, s->scal, s->p, st));
         break;
       case 7:
-        TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->scal, s->partials, s->counters,
+        TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, s->u, s->v, s->w, s->scal, s->partials,
                         s->inv_volume, TC_K5_BULK, st));
         break;
       case 8:
-        TC_RC(launch_cfl(g, s->u, s->v, s->w, s->scal, s->cfl_partials, s->counters + 1, 0, 0.25, 1e-4, 0.01,
+        TC_RC(launch_cfl(g, s->u, s->v, s->w, s->scal, s->cfl_partials, 0, 0.25, 1e-4, 0.01,
                          kMaxRedBlocks, st));
         break;
       default:
@@ -619,7 +614,7 @@ int tc_adjoint_step(tc_adjoint* a, const void* u, const void* v, const void* w, 
   TC_RC(adj_launch_pre(g, 1.0 / dt, s->inv_volume, a->ascal, a->lq, ys, st));
   TC_RC(poisson_solve(s->pl, true, s->su, s->sv, s->sw, nullptr, a->one_scal, s->p, st));
   TC_RC(launch_k5(g, s->su, s->sv, s->sw, s->p, a->z[0], a->z[1], a->z[2], a->one_scal, s->partials,
-                  s->counters, s->inv_volume, 0, st));
+                  s->inv_volume, 0, st));
   const double c1 = first ? 1.0 : 1.5, c0 = first ? 0.0 : 0.5;
   TC_RC(adj_launch_cn(g, a->z, s->tabs, s->tabs + TC_TAB_STRIDE, dt, c1, c0, a->qs, a->lr, a->acc, grad,
                       a->partials, st));

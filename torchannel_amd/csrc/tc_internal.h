@@ -45,7 +45,6 @@ struct Grid {
   // scratch for the stand-alone reductions and CN tables
   double* scratch = nullptr;      // partial sums / maxima
   int scratch_len = 0;
-  unsigned* counter = nullptr;    // last-block counters (several)
   double* tabs = nullptr;         // CN tables: cell, face
   double* scal = nullptr;         // scalar block for stand-alone ops
   int k1_jb = 0;                  // j-columns per K1 pencil (0: default 8; set by the tuner)
@@ -83,7 +82,7 @@ int launch_k1(const Grid* gr, const void* u, const void* v, const void* w, const
 int launch_cn_only(const Grid* gr, int comp, const void* q, void* out, const double* tab,
                    const double* scal, double theta, hipStream_t st);
 int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p, void* u,
-              void* v, void* w, double* scal, double* partials, unsigned* counter, double inv_volume,
+              void* v, void* w, double* scal, double* partials, double inv_volume,
               int flags, hipStream_t st);
 int k5_blocks(const Grid* gr);
 
@@ -94,7 +93,7 @@ int launch_divergence(const Grid* gr, const void* u, const void* v, const void* 
 int launch_project(const Grid* gr, void* u, void* v, void* w, const void* p, double dt_host,
                    const double* dt_dev, hipStream_t st);
 int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0, long long ws1,
-                long long ws2, double inv_vol, double* out, double* partials, unsigned* counter,
+                long long ws2, double inv_vol, double* out, double* partials,
                 int nblocks_max, hipStream_t st);
 int launch_rhs_v2(const Grid* gr, const void* u, const void* v, const void* w, void* ru, void* rv, void* rw,
                   hipStream_t st);
@@ -102,9 +101,9 @@ int launch_div_max(const Grid* gr, const void* u, const void* v, const void* w, 
                    int nblocks_max, hipStream_t st);
 int launch_diag_row(const double* scal, double* rows, int capacity, hipStream_t st);
 int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double* partials,
-                 unsigned* counter, int nblocks_max, hipStream_t st);
+                 int nblocks_max, hipStream_t st);
 int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, double* scal,
-               double* partials, unsigned* counter, int update_dt, double cfl_target, double dt_min,
+               double* partials, int update_dt, double cfl_target, double dt_min,
                double dt_max, int nblocks_max, hipStream_t st);
 int cfl_blocks(const Grid* gr);
 

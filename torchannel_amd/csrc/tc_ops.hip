@@ -1,7 +1,7 @@
 // Stand-alone operators of the IMEX path and the device-side reductions: boundary conditions,
 // divergence, projection, bulk velocity, CFL rate (with the adaptive-dt rule), wall shear and the
-// turbulence-statistics accumulation.  Reductions write per-block partials and the last block
-// to finish folds them in a fixed order, so results do not depend on scheduling.
+// turbulence-statistics accumulation.  Reductions are two-pass: blocks write per-block partials
+// and a one-block kernel folds them in a fixed order, so results do not depend on scheduling.
 #include "tc_internal.h"
 #include "tc_poisson.h"
 
@@ -290,8 +290,8 @@ __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, c
 }
 
 // compute_cfl_fused (operators.py:1262-1331): per-cell convective rates in x, y, z; maximum.
-// With update_dt, the last block also applies compute_cfl_dt (solver.py:483-494) and the 5% rule
-// (solver.py:811-812) to the scalar block.
+// With update_dt, the fold (k_finalize) also applies compute_cfl_dt (solver.py:483-494) and the 5%
+// rule (solver.py:811-812) to the scalar block.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cfl(Dims d, Metric<T> mt, const T* __restrict__ u,
                                              const T* __restrict__ v, const T* __restrict__ w,
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void k_u_tau(Dims d, const T* __restrict__ u, 
 // Turbulence statistics (turbstats.py:114-228).  Pass A: xy means of the cell-centred u, v, w per
 // z.  Pass B: xy means of the fluctuation products, and the fluctuation planes at the two
 // sampled heights (top-wall w negated).  Both passes: block b accumulates columns b, b+nb, ...
-// with one thread per z row, then the last block folds the per-block rows in order.
+// with one thread per z row, then k_stats_fold folds the per-block rows in order.
 // ---------------------------------------------------------------------------------------------
 template <typename T, bool PASS_B>
 __global__ __launch_bounds__(256) void k_stats(Dims d, const T* __restrict__ u, const T* __restrict__ v,
@@ -488,7 +488,7 @@ int cfl_blocks(const Grid* gr) {
 }
 
 int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, double* scal, double* partials,
-               unsigned* counter, int update_dt, double cfl_target, double dt_min, double dt_max,
+               int update_dt, double cfl_target, double dt_min, double dt_max,
                int nblocks_max, hipStream_t st) {
   const int blocks = std::min(cfl_blocks(gr), nblocks_max);
   if (gr->dtype == TC_F64)
@@ -505,7 +505,7 @@ int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, doub
 }
 
 int launch_bulk(const Grid* gr, const void* u, const double* wgt, long long ws0, long long ws1,
-                long long ws2, double inv_vol, double* out, double* partials, unsigned* counter,
+                long long ws2, double inv_vol, double* out, double* partials,
                 int nblocks_max, hipStream_t st) {
   const long long N = (long long)gr->d.nx * gr->d.ny * gr->d.nz;
   const int blocks = (int)std::min<long long>(nblocks_max, (N + 255) / 256);
@@ -561,7 +561,7 @@ int launch_diag_row(const double* scal, double* rows, int capacity, hipStream_t 
 }
 
 int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double* partials,
-                 unsigned* counter, int nblocks_max, hipStream_t st) {
+                 int nblocks_max, hipStream_t st) {
   const int P = (gr->d.nx + 1) * (gr->d.ny + 2);
   const int blocks = std::min(nblocks_max / 2, (P + 255) / 256);
   if (gr->dtype == TC_F64)
@@ -577,7 +577,7 @@ int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double
 
 template <typename T>
 static int stats_t(const Poisson* pl, const void* u, const void* v, const void* w, int kb, int kt,
-                   double* prof, double* spec, double* means, double* partials, unsigned* counter,
+                   double* prof, double* spec, double* means, double* partials,
                    int nblocks, hipStream_t st) {
   const Grid* gr = pl->g;
   const Dims& d = gr->d;
@@ -603,11 +603,11 @@ static int stats_t(const Poisson* pl, const void* u, const void* v, const void* 
 }
 
 int launch_stats(const Poisson* pl, const void* u, const void* v, const void* w, int kb, int kt,
-                 double* prof, double* spec, double* means, double* partials, unsigned* counter,
+                 double* prof, double* spec, double* means, double* partials,
                  int nblocks, hipStream_t st) {
   if (pl->g->dtype == TC_F64)
-    return stats_t<double>(pl, u, v, w, kb, kt, prof, spec, means, partials, counter, nblocks, st);
-  return stats_t<float>(pl, u, v, w, kb, kt, prof, spec, means, partials, counter, nblocks, st);
+    return stats_t<double>(pl, u, v, w, kb, kt, prof, spec, means, partials, nblocks, st);
+  return stats_t<float>(pl, u, v, w, kb, kt, prof, spec, means, partials, nblocks, st);
 }
 
 }  // namespace tc

@@ -1053,7 +1053,7 @@ int launch_cn_only(const Grid* gr, int comp, const void* q, void* out, const dou
 
 template <typename T>
 static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p,
-                       void* u, void* v, void* w, double* scal, double* partials, unsigned* counter,
+                       void* u, void* v, void* w, double* scal, double* partials,
                        double inv_volume, int flags, hipStream_t st) {
   const Dims& d = gr->d;
   const int blocks = k5_blocks(gr);
@@ -1082,11 +1082,11 @@ static int launch_k5_t(const Grid* gr, const void* su, const void* sv, const voi
 int k5_blocks(const Grid* gr) { return 3 * k5_bands_i(gr->d.nx) * k5_bands_j(gr->d.ny); }
 
 int launch_k5(const Grid* gr, const void* su, const void* sv, const void* sw, const void* p, void* u,
-              void* v, void* w, double* scal, double* partials, unsigned* counter, double inv_volume,
+              void* v, void* w, double* scal, double* partials, double inv_volume,
               int flags, hipStream_t st) {
   if (gr->dtype == TC_F64)
-    return launch_k5_t<double>(gr, su, sv, sw, p, u, v, w, scal, partials, counter, inv_volume, flags, st);
-  return launch_k5_t<float>(gr, su, sv, sw, p, u, v, w, scal, partials, counter, inv_volume, flags, st);
+    return launch_k5_t<double>(gr, su, sv, sw, p, u, v, w, scal, partials, inv_volume, flags, st);
+  return launch_k5_t<float>(gr, su, sv, sw, p, u, v, w, scal, partials, inv_volume, flags, st);
 }
 
 }  // namespace tc
