@@ -59,6 +59,30 @@ def algorithmic_words(nx, ny, nz):
     }
 
 
+def phase_times(ch, reps):
+    """Average duration (ms) of each kernel as it runs inside the step sequence: the CFL pass, the
+    CN tables, K1, the five Poisson passes and K5, in step order, `reps` times, with HIP events
+    recorded between the launches on the solver's stream (torch's current stream), so every
+    kernel meets the cache state the preceding kernels of a step leave behind."""
+    import torch
+    from torchannel_amd import _lib as L
+
+    order = [8, 0, 1, 2, 3, 4, 5, 6, 7]
+    st = ch._stream()
+    for i in order:
+        L.call("tc_solver_phase", ch._solver, i, 1, st)
+    torch.cuda.synchronize()
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(order) + 1)] for _ in range(reps)]
+    for r in range(reps):
+        ev[r][0].record()
+        for q, i in enumerate(order):
+            L.call("tc_solver_phase", ch._solver, i, 1, st)
+            ev[r][q + 1].record()
+    torch.cuda.synchronize()
+    return {PHASES[i]: sum(ev[r][q].elapsed_time(ev[r][q + 1]) for r in range(reps)) / reps
+            for q, i in enumerate(order)}
+
+
 def run_gpu(precision, steps, warmup, phase_reps=20):
     import torch
     import torchannel_amd as tc
@@ -111,20 +135,11 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
         "u_bulk": float(ch._scal[L.S_UBULK]),
         "stats_samples": ch.turbulence_stats.n_samples,
     }
-    # per-kernel average duration, HIP events on the launch stream
+    # per-kernel average duration inside the step sequence, HIP events on the launch stream
     per = {}
     words = algorithmic_words(NX, NY, NZ)
     wsize = 8 if precision == "fp64" else 4
-    for i, name in enumerate(PHASES):
-        L.call("tc_solver_phase", ch._solver, i, 2, ch._stream())
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        L.call("tc_solver_phase", ch._solver, i, phase_reps, ch._stream())
-        e1.record()
-        torch.cuda.synchronize()
-        avg_ms = e0.elapsed_time(e1) / phase_reps
+    for name, avg_ms in phase_times(ch, phase_reps).items():
         byts = words[name] * wsize * n_cells
         per[name] = {"avg_us": avg_ms * 1e3, "alg_bytes": byts,
                      "gbs": byts / (avg_ms * 1e-3) / 1e9 if byts else 0.0}
@@ -218,14 +233,7 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
     n_cells = nx * ny * nz
     words = algorithmic_words(nx, ny, nz)
     per = {}
-    for i, name in enumerate(PHASES):
-        L.call("tc_solver_phase", ch._solver, i, 2, ch._stream())
-        torch.cuda.synchronize()
-        e0.record()
-        L.call("tc_solver_phase", ch._solver, i, phase_reps, ch._stream())
-        e1.record()
-        torch.cuda.synchronize()
-        avg = e0.elapsed_time(e1) / phase_reps
+    for name, avg in phase_times(ch, phase_reps).items():
         byts = words[name] * 4 * n_cells
         per[name] = {"avg_us": round(avg * 1e3, 2), "gbs": round(byts / (avg * 1e-3) / 1e9, 1) if byts else 0.0}
     return {"workload": "384x384x256, Lx=2pi, Ly=pi, Re_bulk=10000, float32, adaptive dt CFL 0.25 (first dt "
@@ -321,6 +329,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--c3-only", action="store_true",
+                    help="run only the 384x384x256 leg (for profiling it on its own)")
+    ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
                          "kernel for scaling HBM byte counters")
@@ -343,6 +354,10 @@ def main():
         torch.cuda.synchronize()
         print(f"[bench] copy calibration: 5 x {args.copy_calib_mb} MiB read + write", file=sys.stderr, flush=True)
         del src, dst
+    if args.c3_only:
+        c3 = run_c3(steps=max(10, (args.c3_steps // 10) * 10))
+        print(json.dumps({"re_tau_550": c3}), flush=True)
+        return
     main_run = run_gpu(args.precision, steps, warmup)
     print(f"[bench] {args.precision}: {main_run['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
     other = None
@@ -351,7 +366,7 @@ def main():
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
     c3 = None
     if not args.no_c3:
-        c3 = run_c3()
+        c3 = run_c3(steps=max(10, (args.c3_steps // 10) * 10))
         print(f"[bench] 384x384x256 fp32: {c3['value']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
     ens = None
     if not args.no_ensemble:
