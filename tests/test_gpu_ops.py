@@ -201,7 +201,7 @@ def test_bc_div_project_bulk_cfl_utau(tc, shape):
 
 
 POISSON = [(16, 12, 16), (8, 6, 10), (64, 48, 40), (48, 96, 32), (192, 64, 20), (24, 18, 130), (10, 8, 129),
-           (16, 12, 200), (8, 6, 256), (320, 6, 18)]
+           (16, 12, 200), (8, 6, 256), (320, 6, 18), (16, 12, 128), (12, 10, 64), (8, 6, 32)]
 
 
 @pytest.mark.parametrize("shape", POISSON)
@@ -234,7 +234,8 @@ def test_poisson(tc, shape):
     assert float((lap - div).abs().max()) < 1e-7 * float(div.abs().max())
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 40), (10, 8, 129), (16, 12, 200), (8, 6, 256), (320, 6, 18)])
+@pytest.mark.parametrize("shape", [(16, 12, 40), (10, 8, 129), (16, 12, 200), (8, 6, 256), (320, 6, 18),
+                                   (16, 12, 128), (12, 10, 64)])
 def test_poisson_fp32(tc, shape):
     """The float32 solve (float64 gauge mode) against the float64 oracle: float32 round-off
     amplified by the low-wavenumber systems stays well below 1e-4."""

@@ -322,7 +322,35 @@ __device__ __forceinline__ void th_combine(U& a, cplx<U>& b) {
   a = a * a2;
 }
 
-template <typename U, typename T, typename TT, int RM>
+// RM consecutive elements from / to a 16-byte aligned address, as 16-byte vector accesses
+template <typename E, int RM>
+__device__ __forceinline__ void vload_rows(const E* __restrict__ p, E (&x)[RM]) {
+  constexpr int PER = 16 / (int)sizeof(E);
+  static_assert(PER >= 1 && RM % PER == 0, "whole 16-byte vectors");
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+#pragma unroll
+  for (int v = 0; v < RM / PER; ++v) {
+    const uint4 w = q[v];
+    __builtin_memcpy(&x[v * PER], &w, 16);
+  }
+}
+template <typename E, int RM>
+__device__ __forceinline__ void vstore_rows(E* __restrict__ p, const E (&x)[RM]) {
+  constexpr int PER = 16 / (int)sizeof(E);
+  static_assert(PER >= 1 && RM % PER == 0, "whole 16-byte vectors");
+  uint4* q = reinterpret_cast<uint4*>(p);
+#pragma unroll
+  for (int v = 0; v < RM / PER; ++v) {
+    uint4 w;
+    __builtin_memcpy(&w, &x[v * PER], 16);
+    q[v] = w;
+  }
+}
+
+// VEC: nz == L * RM (R == RM, every lane holds RM rows of the system) and every lane's first row
+// sits on a 16-byte boundary of each array, so a lane's rows move as 16-byte vectors instead of
+// RM strided element accesses.
+template <typename U, typename T, typename TT, int RM, bool VEC>
 __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT* __restrict__ cps,
                                             const TT* __restrict__ ids, const double* __restrict__ cl, int nz,
                                             int R, int L, int g, bool gauge, bool store) {
@@ -330,20 +358,35 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   const int m0 = g * R;
   U A[RM], C[RM];
   cplx<U> B[RM];
-  // every load is issued up front and unconditionally (rows past nz read row nz - 1 and act as
-  // identity maps)
+  cplx<T> dv[RM];
+  // row m (mm = m clamped into the system) -> the forward map y -> A y + B and the backward
+  // factor C; rows past nz are identity maps
+  auto row = [&](int r, cplx<T> d, TT idv, TT cpv, double clv) {
+    const int m = m0 + r, mm = min(m, nz - 1);
+    C[r] = (U)cpv;
+    const U be = (gauge && mm == 0) ? U(0) : (U)idv;
+    const U a = (mm == 0 || (gauge && mm == 1)) ? U(0) : (U)clv;
+    const bool in = m < nz;
+    A[r] = in ? -a * be : U(1);
+    B[r] = in ? cplx<U>{be * (U)d.re, be * (U)d.im} : cplx<U>{U(0), U(0)};
+  };
+  if constexpr (VEC) {
+    TT idv[RM], cpv[RM];
+    double clv[RM];
+    vload_rows<cplx<T>, RM>(col + m0, dv);
+    vload_rows<TT, RM>(ids + m0, idv);
+    vload_rows<TT, RM>(cps + m0, cpv);
+    vload_rows<double, RM>(cl + m0, clv);
 #pragma unroll
-  for (int r = 0; r < RM; ++r) {
-    if (r < R) {
-      const int m = m0 + r, mm = min(m, nz - 1);
-      const cplx<T> dv = col[mm];
-      const U idv = (U)ids[mm], clv = (U)cl[mm];
-      C[r] = (U)cps[mm];
-      const U be = (gauge && mm == 0) ? U(0) : idv;
-      const U a = (mm == 0 || (gauge && mm == 1)) ? U(0) : clv;
-      const bool in = m < nz;
-      A[r] = in ? -a * be : U(1);
-      B[r] = in ? cplx<U>{be * (U)dv.re, be * (U)dv.im} : cplx<U>{U(0), U(0)};
+    for (int r = 0; r < RM; ++r) row(r, dv[r], idv[r], cpv[r], clv[r]);
+  } else {
+    // every load is issued up front and unconditionally (rows past nz read row nz - 1)
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      if (r < R) {
+        const int mm = min(m0 + r, nz - 1);
+        row(r, col[mm], ids[mm], cps[mm], cl[mm]);
+      }
     }
   }
   // forward: lane-local prefix maps, then the scan of the 16 lane maps
@@ -416,11 +459,17 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
     xin_i = si;
   }
   if (!store) return;
+  if constexpr (VEC) {
 #pragma unroll
-  for (int r = 0; r < RM; ++r) {
-    if (r < R) {
-      const int m = m0 + r;
-      if (m < nz) col[m] = {(T)(A[r] * xin_r + B[r].re), (T)(A[r] * xin_i + B[r].im)};
+    for (int r = 0; r < RM; ++r) dv[r] = {(T)(A[r] * xin_r + B[r].re), (T)(A[r] * xin_i + B[r].im)};
+    vstore_rows<cplx<T>, RM>(col + m0, dv);
+  } else {
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+      if (r < R) {
+        const int m = m0 + r;
+        if (m < nz) col[m] = {(T)(A[r] * xin_r + B[r].re), (T)(A[r] * xin_i + B[r].im)};
+      }
     }
   }
 }
@@ -434,7 +483,7 @@ __host__ __device__ __forceinline__ int thomas_rows_per_lane(int nz) {
 }
 
 // Block = sb systems of L lanes each; R <= RM rows per lane.
-template <typename T, int RM>
+template <typename T, int RM, bool VEC = false>
 __device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int L, int sb, const double* __restrict__ cl,
                                             const T* __restrict__ cp, const T* __restrict__ iden,
                                             const double* __restrict__ cp0, const double* __restrict__ iden0,
@@ -447,22 +496,23 @@ __device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int L, int 
   const size_t to = (size_t)sc * nz;
   if constexpr (sizeof(T) == sizeof(double)) {
     const bool gauge = sc == 0;
-    thomas_rows<double, T, double, RM>(col, gauge ? cp0 : (const double*)cp + to,
-                                       gauge ? iden0 : (const double*)iden + to, cl, nz, R, L, lane, gauge, live);
+    thomas_rows<double, T, double, RM, VEC>(col, gauge ? cp0 : (const double*)cp + to,
+                                            gauge ? iden0 : (const double*)iden + to, cl, nz, R, L, lane, gauge,
+                                            live);
   } else {
-    if (sc == 0) thomas_rows<double, T, double, RM>(col, cp0, iden0, cl, nz, R, L, lane, true, live);
-    else thomas_rows<T, T, T, RM>(col, cp + to, iden + to, cl, nz, R, L, lane, false, live);
+    if (sc == 0) thomas_rows<double, T, double, RM, VEC>(col, cp0, iden0, cl, nz, R, L, lane, true, live);
+    else thomas_rows<T, T, T, RM, VEC>(col, cp + to, iden + to, cl, nz, R, L, lane, false, live);
   }
 }
 
-template <typename T, int RM>
+template <typename T, int RM, bool VEC>
 __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
                                                         const T* __restrict__ cp, const T* __restrict__ iden,
                                                         const double* __restrict__ cp0,
                                                         const double* __restrict__ iden0,
                                                         cplx<T>* __restrict__ S) {
-  thomas_body<T, RM>(nsys, nz, RM, thomas_lanes(nz), sb, cl, cp, iden, cp0, iden0, S,
-                     xcd_remap(blockIdx.x, gridDim.x));
+  thomas_body<T, RM, VEC>(nsys, nz, RM, thomas_lanes(nz), sb, cl, cp, iden, cp0, iden0, S,
+                          xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Ensemble forms of P1-P5: global block -> (channel, block of that channel) via prefix sums.
@@ -812,14 +862,28 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
       const int L = thomas_lanes(d.nz);
       const int sb = std::min(pl->g->thomas_sb > 0 ? pl->g->thomas_sb : 16, TH_THREADS / L);
       const int blocks = (nsys + sb - 1) / sb;
-      switch (thomas_rows_per_lane(d.nz)) {
-#define TC_TH(RM)                                                                                     \
-  case RM:                                                                                            \
-    hipLaunchKernelGGL((k_poisson_thomas<T, RM>), dim3(blocks), dim3(sb * L), 0, st, nsys, d.nz, sb,  \
-                       pl->cl, (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0, \
-                       (cplx<T>*)pl->spec);                                                           \
-    break;
-        TC_TH(1) TC_TH(2) TC_TH(4) TC_TH(8)
+      const int rm = thomas_rows_per_lane(d.nz);
+      // 16-byte row vectors when every lane holds rm whole rows that start on 16-byte boundaries
+      // (all arrays are [system][nz] with 256-byte aligned bases)
+      const bool vec = d.nz == L * rm && (rm * (int)sizeof(T)) % 16 == 0;
+      switch (rm) {
+#define TC_TH(RM, V)                                                                                   \
+  hipLaunchKernelGGL((k_poisson_thomas<T, RM, V>), dim3(blocks), dim3(sb * L), 0, st, nsys, d.nz, sb,  \
+                     pl->cl, (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,    \
+                     (cplx<T>*)pl->spec)
+        case 1: TC_TH(1, false); break;
+        case 2:
+          if (vec) TC_TH(2, sizeof(T) == 8);
+          else TC_TH(2, false);
+          break;
+        case 4:
+          if (vec) TC_TH(4, true);
+          else TC_TH(4, false);
+          break;
+        case 8:
+          if (vec) TC_TH(8, true);
+          else TC_TH(8, false);
+          break;
 #undef TC_TH
         default: return TC_ERR_UNSUPPORTED;
       }
