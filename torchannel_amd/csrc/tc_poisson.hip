@@ -5,7 +5,9 @@
 //   P1 k_fwd_x : div (or div/dt computed from the velocities) -> x half spectra, two z-rows per
 //                complex transform (real/imag packing)
 //   P2 k_fft_y : forward complex FFT along y
-//   P3 k_poisson_pcr : one wavefront per (kx, ky) system, PCR along z, (0,0) gauge-fixed in f64
+//   P3 k_poisson_thomas : one Thomas solve per (kx, ky) system along z (the two recurrences as
+//                affine-map scans over 16 or 32 lanes, factors precomputed per plan), (0,0)
+//                gauge-fixed in f64
 //   P4 k_fft_y : inverse complex FFT along y
 //   P5 k_inv_x : half spectra -> real rows, 1/(nx*ny), p written with every ghost image
 #include "tc_ensemble.h"
@@ -846,7 +848,7 @@ int launch_fft_y_t(const Poisson* pl, void* spec, int nz_in, int kz, int sign, h
   return TC_OK;
 }
 
-// One stage of the Poisson solve: 0 = P1 forward x, 1 = P2 forward y, 2 = P3 PCR, 3 = P4 inverse y,
+// One stage of the Poisson solve: 0 = P1 forward x, 1 = P2 forward y, 2 = P3 Thomas, 3 = P4 inverse y,
 // 4 = P5 inverse x + ghosts.
 template <typename T>
 static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void* u, const void* v,

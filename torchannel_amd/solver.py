@@ -5,7 +5,7 @@ runs the fused HIP path:
 
   k_cn_tables  -> CN/PCR coefficient tables for the current dt (device scalar)
   k_explicit_cn-> explicit RHS + forcing, AB2, Crank-Nicolson z-diffusion      (solver.py:588-668)
-  k_fwd_x, k_fft_y, k_poisson_pcr, k_fft_y, k_inv_x -> pressure Poisson solve (solver.py:672-682)
+  k_fwd_x, k_fft_y, k_poisson_thomas, k_fft_y, k_inv_x -> pressure Poisson solve (solver.py:672-682)
   k_project_bc -> projection, boundary images, bulk velocity, forcing update     (solver.py:692-708)
 
 dt, the forcing and the bulk velocity live in a float64 device scalar block, so steps need no
