@@ -42,6 +42,15 @@ def tc():
     return torchannel_amd
 
 
+@pytest.mark.parametrize("shape", [(14, 12, 16), (16, 15, 16), (16, 12, 300), (16, 12, 3)])
+def test_unsupported_grid_is_rejected(tc, shape):
+    """x/y sizes must be even with prime factors 2, 3, 5 only, and 4 <= nz <= 256: anything else
+    raises (with the library's message) before any step runs."""
+    print(f"[reject] shape={shape}", flush=True)
+    with pytest.raises(RuntimeError, match="unsupported grid size"):
+        tc.ChannelFlow(config=cfg(*shape))
+
+
 def _pmod(p):
     p = torch.as_tensor(p).double().cpu()
     return p - p.mean()

@@ -21,10 +21,11 @@ namespace tc {
 // elements each thread loads before storing any, in the staging loops of P1, P2/P4 and P5
 #define FFT_U 4
 
-// LDS of the x and y FFT blocks: `rows` rows of n + 1 entries, the n twiddles, one spare entry.
+// LDS of the x and y FFT blocks: `rows` rows of fft_ld<T>(n) entries, the n twiddles, one
+// spare entry.
 template <typename T>
 __host__ __device__ __forceinline__ size_t fft_lds(int rows, int n) {
-  return sizeof(cplx<T>) * ((size_t)rows * (n + 1) + n + 1);
+  return sizeof(cplx<T>) * ((size_t)rows * fft_ld<T>(n) + n + 1);
 }
 
 // P1: real rows along x -> half spectra.  Block = (y row j, chunk of KZ = 2*pairs z-rows).
@@ -46,7 +47,7 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
   const int KZ = 2 * pairs;
   const int nch = (nz_in + KZ - 1) / KZ;
   const int j = blk / nch, k0 = (blk - j * nch) * KZ;
-  const int ldb = nx + 1;
+  const int ldb = fft_ld<T>(nx);
   const int lkz = 31 - __clz(KZ);   // KZ is a power of two (checked by the launchers)
   cplx<T>* twl = Z + pairs * ldb;
   fft_stage_twiddles<T>(twl, twx, nx);
@@ -85,7 +86,7 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
       const int t = t0 + 256 * q;
       const int kk = t & (KZ - 1), i = t >> lkz;
       const T x = (k0 + kk < nz_in) ? val[q] : T(0);
-      zr[t < tot ? 2 * ((kk >> 1) * ldb + i) + (kk & 1) : spare] = x;
+      zr[t < tot ? 2 * ((kk >> 1) * ldb + fpad<T>(i)) + (kk & 1) : spare] = x;
     }
   }
   __syncthreads();
@@ -95,8 +96,8 @@ __device__ __forceinline__ void fwd_x_body(Dims d, Metric<T> mt, const T* __rest
     const int kk = t & (KZ - 1), m = t >> lkz;
     const int k = k0 + kk;
     if (k >= nz_in) continue;
-    const cplx<T> z1 = Z[(kk >> 1) * ldb + m];
-    const cplx<T> z2 = Z[(kk >> 1) * ldb + (m == 0 ? 0 : nx - m)];
+    const cplx<T> z1 = Z[(kk >> 1) * ldb + fpad<T>(m)];
+    const cplx<T> z2 = Z[(kk >> 1) * ldb + fpad<T>(m == 0 ? 0 : nx - m)];
     cplx<T> o;
     if ((kk & 1) == 0) o = {T(0.5) * (z1.re + z2.re), T(0.5) * (z1.im - z2.im)};
     else o = {T(0.5) * (z1.im + z2.im), T(-0.5) * (z1.re - z2.re)};
@@ -125,7 +126,7 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const Fft
   cplx<T>* Y = reinterpret_cast<cplx<T>*>(smem_raw);
   const int nch = (nz_in + KZ - 1) / KZ;
   const int m = blk / nch, k0 = (blk - m * nch) * KZ;
-  const int ldb = ny + 1;
+  const int ldb = fft_ld<T>(ny);
   const int lkz = 31 - __clz(KZ);   // KZ is a power of two (checked by the launchers)
   cplx<T>* twl = Y + KZ * ldb;
   fft_stage_twiddles<T>(twl, twy, ny);
@@ -144,7 +145,7 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const Fft
     for (int q = 0; q < FFT_U; ++q) {
       const int t = t0 + 256 * q;
       const int kk = t & (KZ - 1), jj = t >> lkz;
-      Y[t < tot ? kk * ldb + jj : spare] = (k0 + kk < nz_in) ? val[q] : cplx<T>{T(0), T(0)};
+      Y[t < tot ? kk * ldb + fpad<T>(jj) : spare] = (k0 + kk < nz_in) ? val[q] : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -152,7 +153,7 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const Fft
   for (int t = threadIdx.x; t < ny * KZ; t += 256) {
     const int kk = t & (KZ - 1), j = t >> lkz;
     const int k = k0 + kk;
-    if (k < nz_in) S[((size_t)m * ny + j) * nz_in + k] = Y[kk * ldb + j];
+    if (k < nz_in) S[((size_t)m * ny + j) * nz_in + k] = Y[kk * ldb + fpad<T>(j)];
   }
 }
 
@@ -177,7 +178,7 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
   const int KZ = 2 * pairs;
   const int nch = (nz + KZ - 1) / KZ;
   const int j = blk / nch, k0 = (blk - j * nch) * KZ;
-  const int ldb = nx + 1;
+  const int ldb = fft_ld<T>(nx);
   const int h = nx / 2;
   const int lpr = 31 - __clz(pairs);   // pairs is a power of two (checked by the launchers)
   cplx<T>* twl = Z + pairs * ldb;
@@ -205,8 +206,8 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
       cplx<T> B = kb < nz ? vb[q] : cplx<T>{T(0), T(0)};
       if (m == 0 || m == h) { A.im = T(0); B.im = T(0); }
       const bool in = t < tot;
-      Z[in ? qq * ldb + m : spare] = {A.re - B.im, A.im + B.re};
-      if (m > 0 && m < h) Z[in ? qq * ldb + nx - m : spare] = {A.re + B.im, B.re - A.im};
+      Z[in ? qq * ldb + fpad<T>(m) : spare] = {A.re - B.im, A.im + B.re};
+      if (m > 0 && m < h) Z[in ? qq * ldb + fpad<T>(nx - m) : spare] = {A.re + B.im, B.re - A.im};
     }
   }
   __syncthreads();
@@ -215,7 +216,7 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
     const int kk = t & (KZ - 1), i = t >> (lpr + 1);
     const int k = k0 + kk;
     if (k >= nz) continue;
-    const cplx<T> z = Z[(kk >> 1) * ldb + i];
+    const cplx<T> z = Z[(kk >> 1) * ldb + fpad<T>(i)];
     const T val = ((kk & 1) ? z.im : z.re) * scale;
     const int I = i + 1, J = j + 1, K = k + 1;
     p[I * d.sxp + J * d.syp + K] = val;
@@ -691,6 +692,7 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   pl->g = g;
   const Dims& d = g->d;
   pl->nkx = d.nx / 2 + 1;
+  if ((d.nx & 1) || (d.ny & 1)) return TC_ERR_SHAPE;
   if (!factor_fft(d.nx, pl->fx) || !factor_fft(d.ny, pl->fy)) return TC_ERR_SHAPE;
   if (d.nx > TC_FFT_ELEMS || d.ny > TC_FFT_ELEMS) return TC_ERR_SHAPE;
   pl->pairs = std::min(8, pow2_floor(std::max(1, TC_FFT_ELEMS / d.nx)));
