@@ -162,6 +162,25 @@ def test_thomas(tc, n, cplx):
     assert rel(torch.view_as_real(got) if cplx else got, torch.view_as_real(ref) if cplx else ref) < TOL
 
 
+@pytest.mark.parametrize("where", ["interior", "corner_lo", "corner_hi"])
+def test_cfl_maximum_anywhere(tc, where):
+    """The CFL rate is a maximum over every cell and direction: a spike placed in the interior or
+    at either corner of the grid (first / last i, j, k) is found, at a size with many blocks."""
+    nx, ny, nz = 66, 300, 20
+    print(f"[cfl] where={where}", flush=True)
+    Lx, Ly = 4 * math.pi, 4 * math.pi / 3
+    dx, dy = Lx / nx, Ly / ny
+    _, _, dz_f, dz_c = O.generate_grid(2.6, nz, 2.0)
+    u, v, w = fields(nx, ny, nz, 3)
+    i, j, k = {"interior": (31, 157, 9), "corner_lo": (1, 1, 1), "corner_hi": (nx, ny, nz)}[where]
+    u[i, j, k] = 40.0
+    v[i, j - 1, k] = -35.0
+    w[i, j, k - 1] = 30.0
+    ref = O.compute_cfl_fused(u, v, w, nx, ny, nz, dx, dy, dz_f, dz_c)
+    got = tc.compute_cfl_fused(u.cuda(), v.cuda(), w.cuda(), nx, ny, nz, dx, dy, dz_f.cuda(), dz_c.cuda())
+    assert abs(got - ref) < 1e-13 * ref
+
+
 @pytest.mark.parametrize("shape", SHAPES)
 def test_bc_div_project_bulk_cfl_utau(tc, shape):
     nx, ny, nz = shape

@@ -26,6 +26,7 @@ using namespace tc;
   } while (0)
 
 static const int kMaxRedBlocks = 1024;   // partial slots for the stand-alone reductions
+static const int kCflBlocks = 4096;      // ... and for the CFL pencils (at most this many blocks)
 static const int kStatsBlocks = 256;
 
 struct tc_grid {
@@ -106,7 +107,7 @@ int tc_grid_create(int nx, int ny, int nz, double dx, double dy, double nu, int 
     rc[k] = 1.0 / dzc[k];
     rc32[k] = (float)rc[k];
   }
-  g.scratch_len = std::max(2 * kMaxRedBlocks, kStatsBlocks * 4 * nz);
+  g.scratch_len = std::max(std::max(2 * kMaxRedBlocks, kCflBlocks), kStatsBlocks * 4 * nz);
   bool ok = hipMalloc(&g.dzf64, sizeof(double) * nz) == hipSuccess &&
             hipMalloc(&g.dzc64, sizeof(double) * (nz + 1)) == hipSuccess &&
             hipMalloc(&g.rdzf64, sizeof(double) * nz) == hipSuccess &&
@@ -249,7 +250,7 @@ int tc_cfl_rate(const tc_grid* h, const void* u, const void* v, const void* w, d
   const Grid& g = h->g;
   // the rate lands in scal[TC_S_CFL]; point scal so that slot is out_dev
   return launch_cfl(&g, u, v, w, out_dev - TC_S_CFL, g.scratch, 0, 0.0, 0.0, 0.0,
-                    kMaxRedBlocks, (hipStream_t)stream);
+                    kCflBlocks, (hipStream_t)stream);
 }
 
 int tc_u_tau(const tc_grid* h, const void* u, double z_c1, double* out_dev, void* stream) {
@@ -333,7 +334,7 @@ int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w
             hipMalloc(&s->sw, esz * nw_) == hipSuccess &&
             hipMalloc(&s->tabs, sizeof(double) * 2 * TC_TAB_STRIDE) == hipSuccess &&
             hipMalloc(&s->partials, sizeof(double) * k5_blocks(&g)) == hipSuccess &&
-            hipMalloc(&s->cfl_partials, sizeof(double) * kMaxRedBlocks) == hipSuccess &&
+            hipMalloc(&s->cfl_partials, sizeof(double) * std::max(kMaxRedBlocks, kCflBlocks)) == hipSuccess &&
             hipMalloc(&s->init_scal, sizeof(double) * TC_NSCAL) == hipSuccess &&
             hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) == hipSuccess;
   if (ok) {
@@ -427,7 +428,7 @@ int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
 int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream) {
   if (!s) return TC_ERR_ARG;
   return launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, 1, cfl_target,
-                    dt_min, dt_max, kMaxRedBlocks, (hipStream_t)stream);
+                    dt_min, dt_max, kCflBlocks, (hipStream_t)stream);
 }
 
 int tc_solver_diagnostics(tc_solver* s, double z_c1, double* rows, int capacity, void* stream) {
@@ -454,7 +455,7 @@ int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_targe
   for (int i = 0; i < steps && rc == TC_OK; ++i) {
     if (dt_interval > 0 && (i + 1) % dt_interval == 0)
       rc = launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, 1, cfl_target,
-                      dt_min, dt_max, kMaxRedBlocks, cs);
+                      dt_min, dt_max, kCflBlocks, cs);
     if (rc == TC_OK) rc = solver_step_impl(s, -1.0, 0, nullptr, cs);
   }
   hipGraph_t graph = nullptr;
@@ -495,7 +496,7 @@ int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream) {
         break;
       case 8:
         TC_RC(launch_cfl(g, s->u, s->v, s->w, s->scal, s->cfl_partials, 0, 0.25, 1e-4, 0.01,
-                         kMaxRedBlocks, st));
+                         kCflBlocks, st));
         break;
       default:
         return TC_ERR_ARG;

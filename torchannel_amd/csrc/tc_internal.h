@@ -105,7 +105,6 @@ int launch_u_tau(const Grid* gr, const void* u, double dist, double* out, double
 int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, double* scal,
                double* partials, int update_dt, double cfl_target, double dt_min,
                double dt_max, int nblocks_max, hipStream_t st);
-int cfl_blocks(const Grid* gr);
 
 // tc_adjoint.hip (float64)
 int adj_launch_pre(const Grid* g, double rdt, double inv_volume, const double* ascal, void* const* lq,

@@ -291,35 +291,56 @@ __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, c
 
 // compute_cfl_fused (operators.py:1262-1331): per-cell convective rates in x, y, z; maximum.
 // With update_dt, the fold (k_finalize) also applies compute_cfl_dt (solver.py:483-494) and the 5%
-// rule (solver.py:811-812) to the scalar block.
-template <typename T>
-__global__ __launch_bounds__(256) void k_cfl(Dims d, Metric<T> mt, const T* __restrict__ u,
-                                             const T* __restrict__ v, const T* __restrict__ w,
-                                             double* __restrict__ partials) {
-  __shared__ double sh[4];
-  const long long N = (long long)d.nx * d.ny * d.nz;
-  double best = 0.0;
+// rule (solver.py:811-812) to the scalar block.  Column pencils: a block holds TC_CFL_WAVES
+// consecutive i, one wavefront per i marching j over jb columns, lane rows k = r*64 + lane + 1
+// (r < R), so the i +- 1 and j +- 1 neighbour columns come back from the CU's caches.  The loads of
+// two rows are issued together (rows past nz read row nz and are masked out).  The maximum does not
+// depend on the order, so the result is the same for any decomposition.
+#define TC_CFL_WAVES 4
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_CFL_WAVES) void k_cfl(Dims d, Metric<T> mt, const T* __restrict__ u,
+                                                           const T* __restrict__ v, const T* __restrict__ w,
+                                                           int jb, double* __restrict__ partials) {
+  __shared__ double sh[TC_CFL_WAVES];
+  const int lane = lane_id();
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int nbj = (d.ny + jb - 1) / jb;
+  const int bi = blockIdx.x / nbj, bj = blockIdx.x - bi * nbj;
+  const int i = 1 + bi * TC_CFL_WAVES + wv;
+  const int j0 = 1 + bj * jb, j1 = min(j0 + jb, d.ny + 1);
   const T q = T(0.25);
-  for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < N; t += (long long)gridDim.x * 256) {
-    const int k = (int)(t % d.nz) + 1;
-    const long long ij = t / d.nz;
-    const int j = (int)(ij % d.ny) + 1, i = (int)(ij / d.ny) + 1;
-    const int cu = i * d.sxu + j * d.syu + k, cv = i * d.sxv + j * d.syv + k, cw = i * d.sxw + j * d.syw + k;
-    const T rzf = mt.rdzf[k - 1];
-    const T ex = fabs(u[cu]) * mt.rdx +
-                 q * fabs(v[cv] + v[cv - d.syv] + v[cv + d.sxv] + v[cv + d.sxv - d.syv]) * mt.rdy +
-                 q * fabs(w[cw] + w[cw - 1] + w[cw + d.sxw] + w[cw + d.sxw - 1]) * rzf;
-    const T ey = q * fabs(u[cu] + u[cu + d.syu] + u[cu - d.sxu + d.syu] + u[cu - d.sxu]) * mt.rdx +
-                 fabs(v[cv]) * mt.rdy +
-                 q * fabs(w[cw] + w[cw + d.syw] + w[cw + d.syw - 1] + w[cw - 1]) * rzf;
-    double m = fmax((double)ex, (double)ey);
-    if (k < d.nz) {
-      const T ez = q * fabs(u[cu] + u[cu - d.sxu] + u[cu - d.sxu + 1] + u[cu + 1]) * mt.rdx +
-                   q * fabs(v[cv] + v[cv - d.syv] + v[cv - d.syv + 1] + v[cv + 1]) * mt.rdy +
-                   fabs(w[cw]) * mt.rdzc[k];
-      m = fmax(m, (double)ez);
+  double best = 0.0;
+  if (i <= d.nx) {
+    for (int j = j0; j < j1; ++j) {
+      const T* uc = u + (long long)i * d.sxu + (long long)j * d.syu;
+      const T* vc = v + (long long)i * d.sxv + (long long)j * d.syv;
+      const T* wc = w + (long long)i * d.sxw + (long long)j * d.syw;
+      // two rows per batch: all R = 4 rows at once spills the CU's caches (measured 2x slower at
+      // 384x384x256)
+#pragma unroll 1
+      for (int r0 = 0; r0 < R; r0 += 2)
+#pragma unroll
+      for (int rr = 0; rr < 2; ++rr) {
+        const int r = r0 + rr;
+        if (r >= R) break;
+        const int kr = r * TC_WAVE + lane + 1;
+        const int k = min(kr, d.nz);
+        const T rzf = mt.rdzf[k - 1];
+        const T ex = fabs(uc[k]) * mt.rdx +
+                     q * fabs(vc[k] + vc[k - d.syv] + vc[k + d.sxv] + vc[k + d.sxv - d.syv]) * mt.rdy +
+                     q * fabs(wc[k] + wc[k - 1] + wc[k + d.sxw] + wc[k + d.sxw - 1]) * rzf;
+        const T ey = q * fabs(uc[k] + uc[k + d.syu] + uc[k - d.sxu + d.syu] + uc[k - d.sxu]) * mt.rdx +
+                     fabs(vc[k]) * mt.rdy +
+                     q * fabs(wc[k] + wc[k + d.syw] + wc[k + d.syw - 1] + wc[k - 1]) * rzf;
+        // rows k = nz read the ghost row nz + 1 of u and v here; their ez is not used
+        const T ez = q * fabs(uc[k] + uc[k - d.sxu] + uc[k - d.sxu + 1] + uc[k + 1]) * mt.rdx +
+                     q * fabs(vc[k] + vc[k - d.syv] + vc[k - d.syv + 1] + vc[k + 1]) * mt.rdy +
+                     fabs(wc[k]) * mt.rdzc[k];
+        double m = fmax((double)ex, (double)ey);
+        if (k < d.nz) m = fmax(m, (double)ez);
+        if (kr <= d.nz) best = fmax(best, m);
+      }
     }
-    best = fmax(best, m);
   }
   const double bm = block_reduce(best, true, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = bm;
@@ -482,21 +503,46 @@ int launch_project(const Grid* gr, void* u, void* v, void* w, const void* p, dou
   return TC_OK;
 }
 
-int cfl_blocks(const Grid* gr) {
-  const long long N = (long long)gr->d.nx * gr->d.ny * gr->d.nz;
-  return (int)std::min<long long>(1024, (N + 255) / 256);
+// j columns per CFL pencil: at least 8, and few enough bands that the block count fits nblocks_max
+static int cfl_jb(const Grid* gr, int nblocks_max) {
+  const int nbi = (gr->d.nx + TC_CFL_WAVES - 1) / TC_CFL_WAVES;
+  const int nbj_max = std::max(1, nblocks_max / nbi);
+  return std::max(8, (gr->d.ny + nbj_max - 1) / nbj_max);
+}
+
+static int cfl_blocks(const Grid* gr, int nblocks_max) {
+  const int jb = cfl_jb(gr, nblocks_max);
+  return ((gr->d.nx + TC_CFL_WAVES - 1) / TC_CFL_WAVES) * ((gr->d.ny + jb - 1) / jb);
 }
 
 int launch_cfl(const Grid* gr, const void* u, const void* v, const void* w, double* scal, double* partials,
                int update_dt, double cfl_target, double dt_min, double dt_max,
                int nblocks_max, hipStream_t st) {
-  const int blocks = std::min(cfl_blocks(gr), nblocks_max);
-  if (gr->dtype == TC_F64)
-    hipLaunchKernelGGL(k_cfl<double>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<double>(),
-                       (const double*)u, (const double*)v, (const double*)w, partials);
-  else
-    hipLaunchKernelGGL(k_cfl<float>, dim3(blocks), dim3(256), 0, st, gr->d, gr->metric<float>(),
-                       (const float*)u, (const float*)v, (const float*)w, partials);
+  const int jb = cfl_jb(gr, nblocks_max);
+  const int blocks = cfl_blocks(gr, nblocks_max);
+  if (blocks > nblocks_max) return TC_ERR_SHAPE;
+  const int R = (gr->d.nz + TC_WAVE - 1) / TC_WAVE;
+#define TC_CFL(TT, RR)                                                                                  \
+  hipLaunchKernelGGL((k_cfl<TT, RR>), dim3(blocks), dim3(64 * TC_CFL_WAVES), 0, st, gr->d, gr->metric<TT>(), \
+                     (const TT*)u, (const TT*)v, (const TT*)w, jb, partials)
+  if (gr->dtype == TC_F64) {
+    switch (R) {
+      case 1: TC_CFL(double, 1); break;
+      case 2: TC_CFL(double, 2); break;
+      case 3: TC_CFL(double, 3); break;
+      case 4: TC_CFL(double, 4); break;
+      default: return TC_ERR_UNSUPPORTED;
+    }
+  } else {
+    switch (R) {
+      case 1: TC_CFL(float, 1); break;
+      case 2: TC_CFL(float, 2); break;
+      case 3: TC_CFL(float, 3); break;
+      case 4: TC_CFL(float, 4); break;
+      default: return TC_ERR_UNSUPPORTED;
+    }
+  }
+#undef TC_CFL
   TC_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, TC_FIN_CFL, (const double*)partials, blocks,
                      scal + TC_S_CFL, scal, cfl_target, dt_min, dt_max, update_dt);
