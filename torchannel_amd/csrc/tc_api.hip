@@ -360,12 +360,16 @@ int tc_solver_destroy(tc_solver* s) {
   return TC_OK;
 }
 
-static int solver_step_impl(tc_solver* s, double dt, int first, const void* body, hipStream_t st) {
+// tables: rebuild the CN tables from the device dt (or from dt > 0, which also sets it).  Eager
+// steps always do; a captured chunk does at its first step and after each adaptive-dt update,
+// the only places where the device dt can have changed.
+static int solver_step_impl(tc_solver* s, double dt, int first, const void* body, hipStream_t st,
+                            bool tables = true) {
   const Grid* g = s->g;
   if (!first && s->newest < 0) return TC_ERR_ARG;
   const int cur = first ? 0 : 1 - s->newest;
   const int prev = first ? -1 : s->newest;
-  TC_RC(launch_tables(g, s->scal, dt, 0.5, s->tabs, s->tabs + TC_TAB_STRIDE, st));
+  if (tables || dt > 0.0) TC_RC(launch_tables(g, s->scal, dt, 0.5, s->tabs, s->tabs + TC_TAB_STRIDE, st));
   int mode = TC_K1_RHS | TC_K1_STEP | TC_K1_FORCE | (first ? TC_K1_FIRST : 0);
   void* const* pr = prev >= 0 ? s->rs[prev] : s->rs[cur];
   TC_RC(launch_k1(g, s->u, s->v, s->w, pr[0], pr[1], pr[2], s->rs[cur][0], s->rs[cur][1], s->rs[cur][2],
@@ -453,10 +457,11 @@ int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_targe
   TC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   int rc = TC_OK;
   for (int i = 0; i < steps && rc == TC_OK; ++i) {
-    if (dt_interval > 0 && (i + 1) % dt_interval == 0)
+    const bool cfl = dt_interval > 0 && (i + 1) % dt_interval == 0;
+    if (cfl)
       rc = launch_cfl(s->g, s->u, s->v, s->w, s->scal, s->cfl_partials, 1, cfl_target,
                       dt_min, dt_max, kCflBlocks, cs);
-    if (rc == TC_OK) rc = solver_step_impl(s, -1.0, 0, nullptr, cs);
+    if (rc == TC_OK) rc = solver_step_impl(s, -1.0, 0, nullptr, cs, i == 0 || cfl);
   }
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(cs, &graph);
@@ -795,11 +800,13 @@ int tc_ensemble_destroy(tc_ensemble* e) {
   return TC_OK;
 }
 
-static int ensemble_step_impl(tc_ensemble* e, double dt, int first, hipStream_t st) {
+// tables: as in solver_step_impl (the ensemble steps at a fixed dt: a captured chunk rebuilds
+// them at its first step only)
+static int ensemble_step_impl(tc_ensemble* e, double dt, int first, hipStream_t st, bool tables = true) {
   if (!first && e->newest < 0) return TC_ERR_ARG;
   const int cur = first ? 0 : 1 - e->newest;
   const int prev = first ? -1 : e->newest;
-  TC_RC(ens_launch_tables(e->dtype, e->dch, e->n, dt, st));
+  if (tables || dt > 0.0) TC_RC(ens_launch_tables(e->dtype, e->dch, e->n, dt, st));
   const int mode = TC_K1_RHS | TC_K1_STEP | TC_K1_FORCE | (first ? TC_K1_FIRST : 0);
   for (auto& g : e->groups)
     TC_RC(ens_launch_k1(e->dtype, g.R, e->dch, g.chans, g.start_k1, g.n, g.blocks_k1, cur, prev, mode, e->branch, st));
@@ -827,7 +834,7 @@ int tc_ensemble_capture(tc_ensemble* e, int steps, void* stream) {
   const int newest0 = e->newest;
   TC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
   int rc = TC_OK;
-  for (int i = 0; i < steps && rc == TC_OK; ++i) rc = ensemble_step_impl(e, -1.0, 0, cs);
+  for (int i = 0; i < steps && rc == TC_OK; ++i) rc = ensemble_step_impl(e, -1.0, 0, cs, i == 0);
   hipGraph_t graph = nullptr;
   hipError_t err = hipStreamEndCapture(cs, &graph);
   e->newest = newest0;
