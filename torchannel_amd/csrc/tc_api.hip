@@ -679,8 +679,9 @@ static void fill_chan(EnsChan<T>& c, tc_solver* s) {
   c.fx = pl->fx; c.fy = pl->fy; c.twx = pl->twx; c.twy = pl->twy;
   c.spec = (cplx<T>*)pl->spec;
   c.cl = pl->cl;
-  c.th_cp = (const T*)pl->th_cp; c.th_iden = (const T*)pl->th_iden;
-  c.th_cp0 = pl->th_cp0; c.th_iden0 = pl->th_iden0;
+  c.th_cp = (const T*)pl->th_cp; c.th_last = (const T*)pl->th_last;
+  c.th_clT = (const T*)pl->th_clT; c.th_rcrT = (const T*)pl->th_rcrT;
+  c.th_cp0 = pl->th_cp0; c.th_iden0 = pl->th_iden0; c.th_rcr = pl->th_rcr;
   c.pairs = pl->pairs; c.kzy = pl->kzy;
   c.nkx = pl->nkx;
   c.jb = g->k1_jb > 0 ? g->k1_jb : 8;

@@ -23,8 +23,8 @@ struct EnsChan {
   const double2 *twx, *twy;
   cplx<T>* spec;
   const double* cl;
-  const T *th_cp, *th_iden;
-  const double *th_cp0, *th_iden0;
+  const T *th_cp, *th_last, *th_clT, *th_rcrT;
+  const double *th_cp0, *th_iden0, *th_rcr;
   int pairs, kzy, nkx;
   // launch shapes and scalars
   int jb, k5_blocks;

@@ -353,10 +353,14 @@ __device__ __forceinline__ void vstore_rows(E* __restrict__ p, const E (&x)[RM])
 // VEC: nz == L * RM (R == RM, every lane holds RM rows of the system) and every lane's first row
 // sits on a 16-byte boundary of each array, so a lane's rows move as 16-byte vectors instead of
 // RM strided element accesses.
+// The inverse pivots: 1/pivot_k = c'_k / c_k = c'_k * rcr[k] for k < nz - 1 and `last` for the
+// last row (c'_k = c_k / pivot_k, c_k > 0 above the last row); the gauge-fixed system (gauge) reads
+// its own float64 table ids instead (its pinned row 0 has c_0 = 0).
 template <typename U, typename T, typename TT, int RM, bool VEC>
 __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT* __restrict__ cps,
-                                            const TT* __restrict__ ids, const double* __restrict__ cl, int nz,
-                                            int R, int L, int g, bool gauge, bool store) {
+                                            const TT* __restrict__ ids, const TT* __restrict__ rcr, U last,
+                                            const TT* __restrict__ cl, int nz, int R, int L, int g,
+                                            bool gauge, bool store) {
   const int wl = lane_id();
   const int m0 = g * R;
   U A[RM], C[RM];
@@ -364,31 +368,33 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   cplx<T> dv[RM];
   // row m (mm = m clamped into the system) -> the forward map y -> A y + B and the backward
   // factor C; rows past nz are identity maps
-  auto row = [&](int r, cplx<T> d, TT idv, TT cpv, double clv) {
+  auto row = [&](int r, cplx<T> d, TT cpv, TT clv, TT rcv, TT idv) {
     const int m = m0 + r, mm = min(m, nz - 1);
     C[r] = (U)cpv;
-    const U be = (gauge && mm == 0) ? U(0) : (U)idv;
+    U be;
+    if (gauge) be = mm == 0 ? U(0) : (U)idv;
+    else be = mm == nz - 1 ? last : (U)cpv * (U)rcv;
     const U a = (mm == 0 || (gauge && mm == 1)) ? U(0) : (U)clv;
     const bool in = m < nz;
     A[r] = in ? -a * be : U(1);
     B[r] = in ? cplx<U>{be * (U)d.re, be * (U)d.im} : cplx<U>{U(0), U(0)};
   };
   if constexpr (VEC) {
-    TT idv[RM], cpv[RM];
-    double clv[RM];
+    TT cpv[RM], idv[RM], clv[RM], rcv[RM];
     vload_rows<cplx<T>, RM>(col + m0, dv);
-    vload_rows<TT, RM>(ids + m0, idv);
     vload_rows<TT, RM>(cps + m0, cpv);
-    vload_rows<double, RM>(cl + m0, clv);
+    vload_rows<TT, RM>(cl + m0, clv);
+    vload_rows<TT, RM>(rcr + m0, rcv);
+    if (gauge) vload_rows<TT, RM>(ids + m0, idv);
 #pragma unroll
-    for (int r = 0; r < RM; ++r) row(r, dv[r], idv[r], cpv[r], clv[r]);
+    for (int r = 0; r < RM; ++r) row(r, dv[r], cpv[r], clv[r], rcv[r], gauge ? idv[r] : TT(0));
   } else {
     // every load is issued up front and unconditionally (rows past nz read row nz - 1)
 #pragma unroll
     for (int r = 0; r < RM; ++r) {
       if (r < R) {
         const int mm = min(m0 + r, nz - 1);
-        row(r, col[mm], ids[mm], cps[mm], cl[mm]);
+        row(r, col[mm], cps[mm], cl[mm], rcr[mm], gauge ? ids[mm] : TT(0));
       }
     }
   }
@@ -486,35 +492,39 @@ __host__ __device__ __forceinline__ int thomas_rows_per_lane(int nz) {
 }
 
 // Block = sb systems of L lanes each; R <= RM rows per lane.
+// cl, rcr: float64 (the gauge-fixed system); clT, rcrT: the same in the working precision
 template <typename T, int RM, bool VEC = false>
 __device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int L, int sb, const double* __restrict__ cl,
-                                            const T* __restrict__ cp, const T* __restrict__ iden,
-                                            const double* __restrict__ cp0, const double* __restrict__ iden0,
-                                            cplx<T>* __restrict__ S, int blk) {
+                                            const double* __restrict__ rcr, const T* __restrict__ clT,
+                                            const T* __restrict__ rcrT, const T* __restrict__ cp,
+                                            const T* __restrict__ last, const double* __restrict__ cp0,
+                                            const double* __restrict__ iden0, cplx<T>* __restrict__ S, int blk) {
   const int lane = threadIdx.x & (L - 1);
   const int s = blk * sb + (int)(threadIdx.x / L);
   const bool live = s < nsys;
   const int sc = live ? s : nsys - 1;
   cplx<T>* col = S + (size_t)sc * nz;
   const size_t to = (size_t)sc * nz;
+  const T lv = last[sc];
   if constexpr (sizeof(T) == sizeof(double)) {
     const bool gauge = sc == 0;
-    thomas_rows<double, T, double, RM, VEC>(col, gauge ? cp0 : (const double*)cp + to,
-                                            gauge ? iden0 : (const double*)iden + to, cl, nz, R, L, lane, gauge,
-                                            live);
+    thomas_rows<double, T, double, RM, VEC>(col, gauge ? cp0 : (const double*)cp + to, iden0, rcr, (double)lv, cl,
+                                            nz, R, L, lane, gauge, live);
   } else {
-    if (sc == 0) thomas_rows<double, T, double, RM, VEC>(col, cp0, iden0, cl, nz, R, L, lane, true, live);
-    else thomas_rows<T, T, T, RM, VEC>(col, cp + to, iden + to, cl, nz, R, L, lane, false, live);
+    if (sc == 0) thomas_rows<double, T, double, RM, VEC>(col, cp0, iden0, rcr, 0.0, cl, nz, R, L, lane, true, live);
+    else thomas_rows<T, T, T, RM, VEC>(col, cp + to, nullptr, rcrT, lv, clT, nz, R, L, lane, false, live);
   }
 }
 
 template <typename T, int RM, bool VEC>
 __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
-                                                        const T* __restrict__ cp, const T* __restrict__ iden,
+                                                        const double* __restrict__ rcr, const T* __restrict__ clT,
+                                                        const T* __restrict__ rcrT, const T* __restrict__ cp,
+                                                        const T* __restrict__ last,
                                                         const double* __restrict__ cp0,
                                                         const double* __restrict__ iden0,
                                                         cplx<T>* __restrict__ S) {
-  thomas_body<T, RM, VEC>(nsys, nz, RM, thomas_lanes(nz), sb, cl, cp, iden, cp0, iden0, S,
+  thomas_body<T, RM, VEC>(nsys, nz, RM, thomas_lanes(nz), sb, cl, rcr, clT, rcrT, cp, last, cp0, iden0, S,
                           xcd_remap(blockIdx.x, gridDim.x));
 }
 
@@ -545,8 +555,8 @@ __global__ __launch_bounds__(256) void k_poisson_thomas_ens(const EnsChan<T>* __
   const int q = ens_chan(start, n, gb);
   const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
   const int L = thomas_lanes(c.d.nz);
-  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), L, TH_THREADS / L, c.cl, c.th_cp,
-                     c.th_iden, c.th_cp0, c.th_iden0, c.spec, gb - start[q]);
+  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), L, TH_THREADS / L, c.cl, c.th_rcr,
+                     c.th_clT, c.th_rcrT, c.th_cp, c.th_last, c.th_cp0, c.th_iden0, c.spec, gb - start[q]);
 }
 
 template <typename T>
@@ -758,8 +768,9 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   // Thomas factorisation of every (kx, ky) system of projection_fft.py:56-82 (rows built in
   // float64 exactly as the reference builds tri_a/b/c, factorised in long double), [system][k]
   const int nsys = nkx * ny;
-  std::vector<double> cp64((size_t)nz * nsys), id64((size_t)nz * nsys);
-  std::vector<double> cp0(nz), id0(nz);
+  std::vector<double> cp64((size_t)nz * nsys), last64(nsys);
+  std::vector<double> cp0(nz), id0(nz), rcr(nz);
+  for (int k = 0; k < nz; ++k) rcr[k] = k <= nz - 2 ? 1.0 / cr[k] : 0.0;
   for (int sy = 0; sy < nsys; ++sy) {
     const bool gauge = sy == 0;
     const double K2 = k2[sy];
@@ -774,7 +785,7 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
       const long double den = (long double)b - (long double)a * cprev;
       const long double cpk = (long double)c / den;
       cp64[(size_t)sy * nz + k] = (double)cpk;
-      id64[(size_t)sy * nz + k] = (double)(1.0L / den);
+      if (k == nz - 1) last64[sy] = (double)(1.0L / den);
       if (gauge) { cp0[k] = (double)cpk; id0[k] = (double)(1.0L / den); }
       cprev = cpk;
     }
@@ -782,19 +793,28 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   const size_t esz = g->dtype == TC_F64 ? 16 : 8;
   const size_t rsz = esz / 2;
   if (hipMalloc(&pl->th_cp, rsz * cp64.size()) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_iden, rsz * id64.size()) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_last, rsz * last64.size()) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_rcr, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_clT, rsz * nz) != hipSuccess) return TC_ERR_NOMEM;
+  if (hipMalloc(&pl->th_rcrT, rsz * nz) != hipSuccess) return TC_ERR_NOMEM;
   if (hipMalloc(&pl->th_cp0, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
   if (hipMalloc(&pl->th_iden0, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
   if (g->dtype == TC_F64) {
     hipMemcpy(pl->th_cp, cp64.data(), 8 * cp64.size(), hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_iden, id64.data(), 8 * id64.size(), hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_last, last64.data(), 8 * last64.size(), hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_clT, cl.data(), 8 * nz, hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_rcrT, rcr.data(), 8 * nz, hipMemcpyHostToDevice);
   } else {
-    std::vector<float> f1(cp64.begin(), cp64.end()), f2(id64.begin(), id64.end());
+    std::vector<float> f1(cp64.begin(), cp64.end()), f2(last64.begin(), last64.end());
     hipMemcpy(pl->th_cp, f1.data(), 4 * f1.size(), hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_iden, f2.data(), 4 * f2.size(), hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_last, f2.data(), 4 * f2.size(), hipMemcpyHostToDevice);
+    std::vector<float> f3(cl.begin(), cl.end()), f4(rcr.begin(), rcr.end());
+    hipMemcpy(pl->th_clT, f3.data(), 4 * nz, hipMemcpyHostToDevice);
+    hipMemcpy(pl->th_rcrT, f4.data(), 4 * nz, hipMemcpyHostToDevice);
   }
   hipMemcpy(pl->th_cp0, cp0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   hipMemcpy(pl->th_iden0, id0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
+  hipMemcpy(pl->th_rcr, rcr.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   if (hipMalloc(&pl->spec, esz * (size_t)nkx * ny * nz) != hipSuccess) return TC_ERR_NOMEM;
   // statistics workspace: 6 planes (u, v, w at the two sampled heights), float64
   if (hipMalloc(&pl->planes, sizeof(double) * (size_t)nx * ny * 6) != hipSuccess) return TC_ERR_NOMEM;
@@ -805,7 +825,8 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
 void poisson_free(Poisson* pl) {
   hipFree(pl->twx); hipFree(pl->twy); hipFree(pl->cl); hipFree(pl->cr); hipFree(pl->K2);
   hipFree(pl->spec); hipFree(pl->planes); hipFree(pl->plane_spec);
-  hipFree(pl->th_cp); hipFree(pl->th_iden); hipFree(pl->th_cp0); hipFree(pl->th_iden0);
+  hipFree(pl->th_cp); hipFree(pl->th_last); hipFree(pl->th_rcr); hipFree(pl->th_clT); hipFree(pl->th_rcrT);
+  hipFree(pl->th_cp0); hipFree(pl->th_iden0);
 }
 
 // pairs * nx must fit the budget the kernel is built for (TC_FFT_ELEMS, or TC_FFT_ELEMS_WIDE
@@ -873,7 +894,8 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
       switch (rm) {
 #define TC_TH(RM, V)                                                                                   \
   hipLaunchKernelGGL((k_poisson_thomas<T, RM, V>), dim3(blocks), dim3(sb * L), 0, st, nsys, d.nz, sb,  \
-                     pl->cl, (const T*)pl->th_cp, (const T*)pl->th_iden, pl->th_cp0, pl->th_iden0,    \
+                     pl->cl, pl->th_rcr, (const T*)pl->th_clT, (const T*)pl->th_rcrT,                 \
+                     (const T*)pl->th_cp, (const T*)pl->th_last, pl->th_cp0, pl->th_iden0,            \
                      (cplx<T>*)pl->spec)
         case 1: TC_TH(1, false); break;
         case 2:
