@@ -140,11 +140,14 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
     if (t < total) {
       const int bb = fdiv(t, mag_m), idx = t - bb * M;
       const int k = idx - fdiv(idx, mag_p) * p;
+      // the first pass (p = 1) has k = 0: every twiddle is 1
+      if (p > 1) {
 #pragma unroll
-      for (int mm = 1; mm < R; ++mm) {
-        const cplx<T> wv = twl[mm * k * tstride];
-        const cplx<T> wc = {wv.re, sign < 0 ? wv.im : -wv.im};
-        x[q][mm] = cmul(x[q][mm], wc);
+        for (int mm = 1; mm < R; ++mm) {
+          const cplx<T> wv = twl[mm * k * tstride];
+          const cplx<T> wc = {wv.re, sign < 0 ? wv.im : -wv.im};
+          x[q][mm] = cmul(x[q][mm], wc);
+        }
       }
       fft_butterfly<T, R>(x[q], sign);
       cplx<T>* base = buf + bb * ldb;
