@@ -194,8 +194,18 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
       const int qq = t & (pairs - 1), m = t >> lpr;
       const int ka = k0 + 2 * qq;
       const cplx<T>* col = S + ((size_t)m * ny + j) * nz;
-      va[q] = col[min(ka, nz - 1)];
-      vb[q] = col[min(ka + 1, nz - 1)];
+      if constexpr (sizeof(T) == 8) {
+        // the spectrum's last read of the step: non-temporal in float64 (measured: step -1 %;
+        // float32 keeps plain loads)
+        typedef T v2 __attribute__((ext_vector_type(2)));
+        const v2 a2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka, nz - 1)));
+        const v2 b2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka + 1, nz - 1)));
+        va[q] = {a2.x, a2.y};
+        vb[q] = {b2.x, b2.y};
+      } else {
+        va[q] = col[min(ka, nz - 1)];
+        vb[q] = col[min(ka + 1, nz - 1)];
+      }
     }
 #pragma unroll
     for (int q = 0; q < FFT_U; ++q) {

@@ -129,6 +129,33 @@ __device__ __forceinline__ void col_store(T* __restrict__ col, const T (&x)[R], 
   }
 }
 
+// The AB2 history (rhs) is written by one step's K1 and read by the next step's K1, with every
+// other kernel of the step in between: in float64 these columns move with non-temporal loads and
+// stores, so they do not push q*, p and the velocities out of the last-level cache (whole step
+// -5 % at 192x192x128); float32 keeps plain accesses (its working set fits the cache; the hint
+// measured 1 % slower).
+template <int R, typename T>
+__device__ __forceinline__ void col_load_hist(const T* __restrict__ col, T (&x)[R], int n, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    if constexpr (sizeof(T) == 8) x[r] = (int)k <= n ? __builtin_nontemporal_load(col + k) : T(0);
+    else x[r] = (int)k <= n ? col[k] : T(0);
+  }
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void col_store_hist(T* __restrict__ col, const T (&x)[R], int n, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const unsigned k = (unsigned)(r * TC_WAVE + lane + 1);
+    if ((int)k <= n) {
+      if constexpr (sizeof(T) == 8) __builtin_nontemporal_store(x[r], col + k);
+      else col[k] = x[r];
+    }
+  }
+}
+
 // dn = x one row below (row 0 takes `lo`), up = x one row above (row n-1 takes `hi`)
 template <int R, typename T>
 __device__ __forceinline__ void rows_pm(T* buf, const T (&x)[R], T (&dn)[R], T (&up)[R], T lo, T hi, int n,
@@ -232,7 +259,7 @@ __device__ __forceinline__ bool k1_rhs_ab2(const K1Ctx<T, R>& c, T (&rhs)[R], co
     rhs[r] = val;
   }
   if (c.mode & TC_K1_RHS) {
-    col_store<R, T>(rout, rhs, n, lane);
+    col_store_hist<R, T>(rout, rhs, n, lane);
     // wall-normal ghost entries of the rhs column carry only the forcing (solver.py:626)
     if (lane == 0) rout[0] = c.F + (bcol != nullptr ? bcol[0] : T(0));
     if (lane == 1) rout[Kc - 1] = c.F + (bcol != nullptr ? bcol[Kc - 1] : T(0));
@@ -415,7 +442,7 @@ __device__ __forceinline__ void k1_pencil_u(const K1Ctx<T, R>& c, int i, int j0,
     col_load<R, T>(ui + cu + d.syu, L.uN, n, lane);
     col_load<R, T>(uw + cu, L.uW, n, lane);
     col_load<R, T>(ue + cu, L.uE, n, lane);
-    if (ab2) col_load<R, T>(pu + oi + cu, L.rp, n, lane);
+    if (ab2) col_load_hist<R, T>(pu + oi + cu, L.rp, n, lane);
     if (adv) {
       const T* w0 = wi + (long long)j * d.syw;
       col_load<R, T>(vi + (long long)j * d.syv, L.v00, n, lane);
@@ -513,7 +540,7 @@ __device__ __forceinline__ void k1_pencil_v(const K1Ctx<T, R>& c, int i, int j0,
     col_load<R, T>(vn, L.vN, n, lane);
     col_load<R, T>(vi - d.sxv + cv, L.vW, n, lane);
     col_load<R, T>(vi + d.sxv + cv, L.vE, n, lane);
-    if (ab2) col_load<R, T>(pv + oi + cv, L.rp, n, lane);
+    if (ab2) col_load_hist<R, T>(pv + oi + cv, L.rp, n, lane);
     if (j < d.ny) {
       const T* w1 = wi + (long long)(j + 1) * d.syw;
       col_load<R, T>(ui + (long long)(j + 1) * d.syu, L.u01, n, lane);
@@ -613,7 +640,7 @@ __device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0,
     col_load<R, T>(u0, L.u00, n, lane);
     col_load<R, T>(u0 - d.sxu, L.um0, n, lane);
     col_load<R, T>(v0, L.v00, n, lane);
-    if (ab2) col_load<R, T>(pw + oi + cw, L.rp, n, lane);
+    if (ab2) col_load_hist<R, T>(pw + oi + cw, L.rp, n, lane);
     L.wb0 = wi[cw];
     L.wbn = wi[cw + n + 1];
     L.ubn = u0[n + 1];
