@@ -338,8 +338,8 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU only (the IMEX step of one channel on one "
-              "MI355X); multi-GPU runs are not part of it.", flush=True)
+        print("bench.py: measures one GPU only (the IMEX step of one channel on one MI355X); "
+              "runs on several GPUs are not supported.", flush=True)
         sys.exit(2)
     steps = max(10, (args.steps // 10) * 10)
     warmup = max(10, (args.warmup // 10) * 10)

@@ -1,7 +1,7 @@
 """Parity of each HIP operator with the CPU oracle (float64), on seeded inputs.
 
-Bar: <= 1e-12 relative (max|a-b| / max|b|) for every field, as BASELINE.json states for the
-plain FP64 path; p is compared modulo its mean (the reference's p is defined up to a constant).
+Bar: <= 1e-12 relative (max|a-b| / max|b|) for every field of the plain FP64 path; p is compared
+modulo its mean (the reference's p is defined up to a constant).
 """
 import math
 import os

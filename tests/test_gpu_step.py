@@ -131,7 +131,7 @@ def test_three_steps_match_reference_trajectory(tc, golden_dir):
 
 
 def test_c1_200_steps_match_reference(tc, golden_dir):
-    """BASELINE config 1 (64x64x48, dt = 0.005, 200 steps, CPU branch) against the reference's
+    """The CPU-runnable case (64x64x48, dt = 0.005, 200 steps, CPU branch) against the reference's
     own run: bulk velocity every step, mean and RMS profiles at the end (<= 1e-9)."""
     g = np.load(os.path.join(golden_dir, "c1_run.npz"))
     nx, ny, nz = 64, 64, 48

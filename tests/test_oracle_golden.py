@@ -96,7 +96,7 @@ def test_three_step_trajectory_matches_reference(golden_dir):
 
 @pytest.mark.slow_cpu
 def test_c1_200_steps_matches_reference(golden_dir):
-    """64x64x48 plain method, dt=0.005, 200 steps on the CPU branch."""
+    """64x64x48 plain method, dt=0.005, 200 steps on the CPU branch (the CPU-runnable case)."""
     g = np.load(os.path.join(golden_dir, "c1_run.npz"))
     nx, ny, nz = 64, 64, 48
     ch = O.Channel(nx, ny, nz, 4 * math.pi, 4 * math.pi / 3, seed=0, branch="cpu")

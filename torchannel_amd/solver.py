@@ -1,6 +1,6 @@
-"""ChannelFlow driver and apply_bc_all (solver.py) on the HIP library.
+"""ChannelFlow and apply_bc_all (solver.py) on the HIP library.
 
-The driver keeps the reference's configuration keys, attributes and step semantics; every step
+ChannelFlow keeps the reference's configuration keys, attributes and step semantics; every step
 runs the fused HIP path:
 
   k_cn_tables  -> CN/PCR coefficient tables for the current dt (device scalar)
