@@ -220,7 +220,8 @@ def test_bc_div_project_bulk_cfl_utau(tc, shape):
 
 
 POISSON = [(16, 12, 16), (8, 6, 10), (64, 48, 40), (48, 96, 32), (192, 64, 20), (24, 18, 130), (10, 8, 129),
-           (16, 12, 200), (8, 6, 256), (320, 6, 18), (16, 12, 128), (12, 10, 64), (8, 6, 32)]
+           (16, 12, 200), (8, 6, 256), (320, 6, 18), (16, 12, 128), (12, 10, 64), (8, 6, 32),
+           (4, 4, 4), (1536, 4, 6), (4, 1536, 6)]
 
 
 @pytest.mark.parametrize("shape", POISSON)

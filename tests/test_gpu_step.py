@@ -42,10 +42,11 @@ def tc():
     return torchannel_amd
 
 
-@pytest.mark.parametrize("shape", [(14, 12, 16), (16, 15, 16), (16, 12, 300), (16, 12, 3)])
+@pytest.mark.parametrize("shape", [(14, 12, 16), (16, 15, 16), (16, 12, 300), (16, 12, 3), (2, 12, 16),
+                                   (1600, 4, 8)])
 def test_unsupported_grid_is_rejected(tc, shape):
-    """x/y sizes must be even with prime factors 2, 3, 5 only, and 4 <= nz <= 256: anything else
-    raises (with the library's message) before any step runs."""
+    """x/y sizes must be even, in [4, 1536], with prime factors 2, 3, 5 only, and 4 <= nz <= 256:
+    anything else raises (with the library's message) before any step runs."""
     print(f"[reject] shape={shape}", flush=True)
     with pytest.raises(RuntimeError, match="unsupported grid size"):
         tc.ChannelFlow(config=cfg(*shape))
@@ -56,7 +57,8 @@ def _pmod(p):
     return p - p.mean()
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 16), (32, 24, 48), (24, 18, 130), (16, 12, 256), (288, 6, 12)])
+@pytest.mark.parametrize("shape", [(4, 4, 4), (16, 12, 16), (32, 24, 48), (24, 18, 130), (16, 12, 256),
+                                   (288, 6, 12)])
 @pytest.mark.parametrize("top", ["dirichlet", "neumann"])
 def test_steps_match_oracle(tc, shape, top):
     nx, ny, nz = shape

@@ -64,7 +64,8 @@ const char* tc_strerror(int code) {
   switch (code) {
     case TC_OK: return "success";
     case TC_ERR_ARG: return "torchannel: invalid argument";
-    case TC_ERR_SHAPE: return "torchannel: unsupported grid size (nx, ny even and 2^a*3^b*5^c; 4 <= nz <= 256)";
+    case TC_ERR_SHAPE:
+      return "torchannel: unsupported grid size (nx, ny even, of the form 2^a*3^b*5^c, in [4, 1536]; 4 <= nz <= 256)";
     case TC_ERR_DTYPE: return "torchannel: unsupported dtype (float64 or float32)";
     case TC_ERR_UNSUPPORTED: return "torchannel: size outside the compiled kernel range";
     case TC_ERR_NOMEM: return "torchannel: device workspace allocation failed";
