@@ -54,19 +54,23 @@ def test_diagnostics_row_matches_host_functions(tc, tmp_path):
     assert r[6] == float(div.abs().max())
 
 
-def test_run_simulation_matches_eager_loop(tc, tmp_path):
+@pytest.mark.parametrize("n_steps,n_out,n_save,dt_interval", [(40, 10, 20, 10), (30, 5, 15, 5)])
+def test_run_simulation_matches_eager_loop(tc, tmp_path, n_steps, n_out, n_save, dt_interval):
+    """n_out = 10 runs graph chunks between the events, n_out = 5 (odd) eager steps."""
     folder = str(tmp_path / "run")
-    print("[runloop] run_simulation 40 steps, n_out 10, n_save 20, dt every 10", flush=True)
-    ch = tc.ChannelFlow(config=cfg(folder))
-    assert ch.run_simulation() == 40
+    print(f"[runloop] run_simulation {n_steps} steps, n_out {n_out}, n_save {n_save}, dt every {dt_interval}",
+          flush=True)
+    kw = dict(n_steps=n_steps, n_out=n_out, n_save=n_save, dt_interval=dt_interval)
+    ch = tc.ChannelFlow(config=cfg(folder, **kw))
+    assert ch.run_simulation() == n_steps
     # the same cadence, eagerly, recording what the reference loop records
-    ref = tc.ChannelFlow(config=cfg(str(tmp_path / "eager")))
+    ref = tc.ChannelFlow(config=cfg(str(tmp_path / "eager"), **kw))
     want = []
-    for step in range(1, 41):
-        if step % 10 == 0:
+    for step in range(1, n_steps + 1):
+        if step % dt_interval == 0:
             ref.update_dt_device()
         ub, f = ref.step_imex(None)
-        if step % 10 == 0:
+        if step % n_out == 0:
             ut = tc.compute_u_tau(ref.u, ref.z_c, ref.nu, "dirichlet")
             want.append((step, float(ref._scal[3]), float(ub), float(ut), float(f)))
     for a, b in ((ch.u, ref.u), (ch.v, ref.v), (ch.w, ref.w), (ch.p, ref.p)):
@@ -77,5 +81,5 @@ def test_run_simulation_matches_eager_loop(tc, tmp_path):
         assert np.array_equal(ts[key], np.array([w[col] for w in want])), key
     assert np.allclose(ts["u_tau"], [w[3] for w in want], rtol=1e-14, atol=0)
     fields = np.load(os.path.join(folder, "fields.npz"))
-    assert int(fields["step"]) == 40
+    assert int(fields["step"]) == n_steps
     assert np.array_equal(fields["u"], ch.u.cpu().numpy())
