@@ -115,9 +115,11 @@ __device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
   }
 }
 
-template <typename T, int R, int ELEMS>
+// TW: the twiddle table's type (T itself, or float for T = a float2 vector: two sequences per
+// element, one twiddle for both)
+template <typename T, int R, int ELEMS, typename TW = T>
 __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N, int p, unsigned mag_m,
-                                         unsigned mag_p, const cplx<T>* __restrict__ twl, int sign) {
+                                         unsigned mag_p, const cplx<TW>* __restrict__ twl, int sign) {
   constexpr int MB = (ELEMS / R + 255) / 256;
   const int M = N / R;
   const int total = batch * M;
@@ -144,8 +146,8 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
       if (p > 1) {
 #pragma unroll
         for (int mm = 1; mm < R; ++mm) {
-          const cplx<T> wv = twl[mm * k * tstride];
-          const cplx<T> wc = {wv.re, sign < 0 ? wv.im : -wv.im};
+          const cplx<TW> wv = twl[mm * k * tstride];
+          const cplx<T> wc = {T(wv.re), sign < 0 ? T(wv.im) : T(-wv.im)};
           x[q][mm] = cmul(x[q][mm], wc);
         }
       }
@@ -170,17 +172,17 @@ __device__ __forceinline__ void fft_stage_twiddles(cplx<T>* twl, const double2* 
 }
 
 // batch * f.n <= ELEMS
-template <typename T, int ELEMS = TC_FFT_ELEMS>
-__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const cplx<T>* __restrict__ twl,
+template <typename T, int ELEMS = TC_FFT_ELEMS, typename TW = T>
+__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const cplx<TW>* __restrict__ twl,
                         int sign) {
   int p = 1;
   for (int ps = 0; ps < f.npass; ++ps) {
     const int R = f.radix[ps];
-    if (R == 8) fft_pass<T, 8, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 4) fft_pass<T, 4, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 2) fft_pass<T, 2, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 3) fft_pass<T, 3, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else fft_pass<T, 5, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    if (R == 8) fft_pass<T, 8, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 4) fft_pass<T, 4, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 2) fft_pass<T, 2, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 3) fft_pass<T, 3, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else fft_pass<T, 5, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     p *= R;
   }
 }
