@@ -161,6 +161,103 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
   __syncthreads();
 }
 
+// First pass (span p = 1: no twiddles) of a batch of 1 << lb sequences, fed from registers:
+// ld(b, x) returns input x of sequence b.  Threads take the sequence index fastest, so a caller
+// whose sequences are adjacent in memory reads contiguous segments.  Its outputs go to LDS in the
+// layout of fft_pass; the closing barrier also publishes whatever the block staged before.
+template <typename T, int R, int ELEMS, typename LD>
+__device__ __forceinline__ void fft_pass_first(cplx<T>* buf, int ldb, int lb, int N, LD ld, int sign) {
+  constexpr int MB = (ELEMS / R + 255) / 256;
+  const int M = N / R;
+  const int total = M << lb;
+  cplx<T> x[MB][R];
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < total) {
+      const int b = t & ((1 << lb) - 1), idx = t >> lb;
+#pragma unroll
+      for (int mm = 0; mm < R; ++mm) x[q][mm] = ld(b, idx + mm * M);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < total) {
+      const int b = t & ((1 << lb) - 1), idx = t >> lb;
+      fft_butterfly<T, R>(x[q], sign);
+      cplx<T>* base = buf + b * ldb;
+#pragma unroll
+      for (int mm = 0; mm < R; ++mm) base[fpad<T>(idx * R + mm)] = x[q][mm];
+    }
+  }
+  __syncthreads();
+}
+
+// Last pass (span p = N / R: k = idx, twiddle stride 1, outputs at idx + r*p), its outputs handed
+// to st(b, x, value) instead of LDS; sequence index fastest, as in fft_pass_first.  Nothing is
+// written to LDS, so no barrier follows.
+template <typename T, int R, int ELEMS, typename TW, typename ST>
+__device__ __forceinline__ void fft_pass_last(const cplx<T>* buf, int ldb, int lb, int N,
+                                              const cplx<TW>* __restrict__ twl, ST st, int sign) {
+  constexpr int MB = (ELEMS / R + 255) / 256;
+  const int M = N / R;
+  const int total = M << lb;
+#pragma unroll
+  for (int q = 0; q < MB; ++q) {
+    const int t = threadIdx.x + q * 256;
+    if (t < total) {
+      const int b = t & ((1 << lb) - 1), idx = t >> lb;
+      const cplx<T>* base = buf + b * ldb;
+      cplx<T> x[R];
+#pragma unroll
+      for (int mm = 0; mm < R; ++mm) x[mm] = base[fpad<T>(idx + mm * M)];
+#pragma unroll
+      for (int mm = 1; mm < R; ++mm) {
+        const cplx<TW> wv = twl[mm * idx];
+        const cplx<T> wc = {T(wv.re), sign < 0 ? T(wv.im) : T(-wv.im)};
+        x[mm] = cmul(x[mm], wc);
+      }
+      fft_butterfly<T, R>(x, sign);
+#pragma unroll
+      for (int mm = 0; mm < R; ++mm) st(b, idx + mm * M, x[mm]);
+    }
+  }
+}
+
+// Whole transform of a batch of 1 << lb sequences (f.npass >= 2) with the first pass reading
+// through ld and the last writing through st, so the inputs and outputs never pass through LDS
+// on their own: two LDS round trips and four barriers fewer than staging + lds_fft + unstaging.
+template <typename T, int ELEMS, typename TW, typename LD, typename ST>
+__device__ void fft_through(cplx<T>* buf, int ldb, int lb, const FftDesc& f, const cplx<TW>* __restrict__ twl,
+                            LD ld, ST st, int sign) {
+  const int batch = 1 << lb;
+  switch (f.radix[0]) {
+    case 8: fft_pass_first<T, 8, ELEMS>(buf, ldb, lb, f.n, ld, sign); break;
+    case 4: fft_pass_first<T, 4, ELEMS>(buf, ldb, lb, f.n, ld, sign); break;
+    case 2: fft_pass_first<T, 2, ELEMS>(buf, ldb, lb, f.n, ld, sign); break;
+    case 3: fft_pass_first<T, 3, ELEMS>(buf, ldb, lb, f.n, ld, sign); break;
+    default: fft_pass_first<T, 5, ELEMS>(buf, ldb, lb, f.n, ld, sign); break;
+  }
+  int p = f.radix[0];
+  for (int ps = 1; ps + 1 < f.npass; ++ps) {
+    const int R = f.radix[ps];
+    if (R == 8) fft_pass<T, 8, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 4) fft_pass<T, 4, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 2) fft_pass<T, 2, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 3) fft_pass<T, 3, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else fft_pass<T, 5, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    p *= R;
+  }
+  switch (f.radix[f.npass - 1]) {
+    case 8: fft_pass_last<T, 8, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 4: fft_pass_last<T, 4, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 2: fft_pass_last<T, 2, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 3: fft_pass_last<T, 3, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
+    default: fft_pass_last<T, 5, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
+  }
+}
+
 // Twiddles exp(-2*pi*i*t/N), t in [0, N), copied from the float64 table into LDS in the
 // working type (every thread of the block takes part; the caller's next barrier publishes them).
 template <typename T>
