@@ -115,11 +115,9 @@ __device__ __forceinline__ void fft_butterfly(cplx<T> (&x)[R], int sign) {
   }
 }
 
-// TW: the twiddle table's type (T itself, or float for T = a float2 vector: two sequences per
-// element, one twiddle for both)
-template <typename T, int R, int ELEMS, typename TW = T>
+template <typename T, int R, int ELEMS>
 __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N, int p, unsigned mag_m,
-                                         unsigned mag_p, const cplx<TW>* __restrict__ twl, int sign) {
+                                         unsigned mag_p, const cplx<T>* __restrict__ twl, int sign) {
   constexpr int MB = (ELEMS / R + 255) / 256;
   const int M = N / R;
   const int total = batch * M;
@@ -146,8 +144,8 @@ __device__ __forceinline__ void fft_pass(cplx<T>* buf, int ldb, int batch, int N
       if (p > 1) {
 #pragma unroll
         for (int mm = 1; mm < R; ++mm) {
-          const cplx<TW> wv = twl[mm * k * tstride];
-          const cplx<T> wc = {T(wv.re), sign < 0 ? T(wv.im) : T(-wv.im)};
+          const cplx<T> wv = twl[mm * k * tstride];
+          const cplx<T> wc = {wv.re, sign < 0 ? wv.im : -wv.im};
           x[q][mm] = cmul(x[q][mm], wc);
         }
       }
@@ -197,9 +195,9 @@ __device__ __forceinline__ void fft_pass_first(cplx<T>* buf, int ldb, int lb, in
 // Last pass (span p = N / R: k = idx, twiddle stride 1, outputs at idx + r*p), its outputs handed
 // to st(b, x, value) instead of LDS; sequence index fastest, as in fft_pass_first.  Nothing is
 // written to LDS, so no barrier follows.
-template <typename T, int R, int ELEMS, typename TW, typename ST>
+template <typename T, int R, int ELEMS, typename ST>
 __device__ __forceinline__ void fft_pass_last(const cplx<T>* buf, int ldb, int lb, int N,
-                                              const cplx<TW>* __restrict__ twl, ST st, int sign) {
+                                              const cplx<T>* __restrict__ twl, ST st, int sign) {
   constexpr int MB = (ELEMS / R + 255) / 256;
   const int M = N / R;
   const int total = M << lb;
@@ -214,8 +212,8 @@ __device__ __forceinline__ void fft_pass_last(const cplx<T>* buf, int ldb, int l
       for (int mm = 0; mm < R; ++mm) x[mm] = base[fpad<T>(idx + mm * M)];
 #pragma unroll
       for (int mm = 1; mm < R; ++mm) {
-        const cplx<TW> wv = twl[mm * idx];
-        const cplx<T> wc = {T(wv.re), sign < 0 ? T(wv.im) : T(-wv.im)};
+        const cplx<T> wv = twl[mm * idx];
+        const cplx<T> wc = {wv.re, sign < 0 ? wv.im : -wv.im};
         x[mm] = cmul(x[mm], wc);
       }
       fft_butterfly<T, R>(x, sign);
@@ -228,8 +226,8 @@ __device__ __forceinline__ void fft_pass_last(const cplx<T>* buf, int ldb, int l
 // Whole transform of a batch of 1 << lb sequences (f.npass >= 2) with the first pass reading
 // through ld and the last writing through st, so the inputs and outputs never pass through LDS
 // on their own: two LDS round trips and four barriers fewer than staging + lds_fft + unstaging.
-template <typename T, int ELEMS, typename TW, typename LD, typename ST>
-__device__ void fft_through(cplx<T>* buf, int ldb, int lb, const FftDesc& f, const cplx<TW>* __restrict__ twl,
+template <typename T, int ELEMS, typename LD, typename ST>
+__device__ void fft_through(cplx<T>* buf, int ldb, int lb, const FftDesc& f, const cplx<T>* __restrict__ twl,
                             LD ld, ST st, int sign) {
   const int batch = 1 << lb;
   switch (f.radix[0]) {
@@ -242,19 +240,19 @@ __device__ void fft_through(cplx<T>* buf, int ldb, int lb, const FftDesc& f, con
   int p = f.radix[0];
   for (int ps = 1; ps + 1 < f.npass; ++ps) {
     const int R = f.radix[ps];
-    if (R == 8) fft_pass<T, 8, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 4) fft_pass<T, 4, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 2) fft_pass<T, 2, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 3) fft_pass<T, 3, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else fft_pass<T, 5, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    if (R == 8) fft_pass<T, 8, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 4) fft_pass<T, 4, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 2) fft_pass<T, 2, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 3) fft_pass<T, 3, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else fft_pass<T, 5, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     p *= R;
   }
   switch (f.radix[f.npass - 1]) {
-    case 8: fft_pass_last<T, 8, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
-    case 4: fft_pass_last<T, 4, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
-    case 2: fft_pass_last<T, 2, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
-    case 3: fft_pass_last<T, 3, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
-    default: fft_pass_last<T, 5, ELEMS, TW>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 8: fft_pass_last<T, 8, ELEMS>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 4: fft_pass_last<T, 4, ELEMS>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 2: fft_pass_last<T, 2, ELEMS>(buf, ldb, lb, f.n, twl, st, sign); break;
+    case 3: fft_pass_last<T, 3, ELEMS>(buf, ldb, lb, f.n, twl, st, sign); break;
+    default: fft_pass_last<T, 5, ELEMS>(buf, ldb, lb, f.n, twl, st, sign); break;
   }
 }
 
@@ -269,17 +267,17 @@ __device__ __forceinline__ void fft_stage_twiddles(cplx<T>* twl, const double2* 
 }
 
 // batch * f.n <= ELEMS
-template <typename T, int ELEMS = TC_FFT_ELEMS, typename TW = T>
-__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const cplx<TW>* __restrict__ twl,
+template <typename T, int ELEMS = TC_FFT_ELEMS>
+__device__ void lds_fft(cplx<T>* buf, int ldb, int batch, const FftDesc& f, const cplx<T>* __restrict__ twl,
                         int sign) {
   int p = 1;
   for (int ps = 0; ps < f.npass; ++ps) {
     const int R = f.radix[ps];
-    if (R == 8) fft_pass<T, 8, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 4) fft_pass<T, 4, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 2) fft_pass<T, 2, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else if (R == 3) fft_pass<T, 3, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
-    else fft_pass<T, 5, ELEMS, TW>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    if (R == 8) fft_pass<T, 8, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 4) fft_pass<T, 4, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 2) fft_pass<T, 2, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else if (R == 3) fft_pass<T, 3, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
+    else fft_pass<T, 5, ELEMS>(buf, ldb, batch, f.n, p, f.mag_m[ps], f.mag_p[ps], twl, sign);
     p *= R;
   }
 }

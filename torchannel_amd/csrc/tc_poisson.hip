@@ -134,7 +134,7 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const Fft
   const int nk = min(KZ, nz_in - k0);
   if (fy.npass >= 2) {
     // first pass straight from the spectrum, last pass straight back to it
-    fft_through<T, TC_FFT_ELEMS, T>(
+    fft_through<T, TC_FFT_ELEMS>(
         Y, ldb, lkz, fy, twl,
         [&](int b, int x) { return b < nk ? Sm[(size_t)x * nz_in + b] : cplx<T>{T(0), T(0)}; },
         [&](int b, int x, const cplx<T>& val) {
@@ -168,71 +168,6 @@ __device__ __forceinline__ void fft_y_body(int nkx, int ny, int nz_in, const Fft
     const int k = k0 + kk;
     if (k < nz_in) S[((size_t)m * ny + j) * nz_in + k] = Y[kk * ldb + fpad<T>(j)];
   }
-}
-
-// P2 / P4 in float32 with two z rows per LDS element: the transforms of z rows 2q and 2q+1 run
-// together on float2 components, so every LDS access of the passes moves 16 bytes and the
-// index arithmetic is shared.  LDS: KZ/2 padded rows of 16-byte elements, one spare element,
-// then the ny float twiddles.
-typedef float f2v __attribute__((ext_vector_type(2)));
-
-__host__ __device__ __forceinline__ size_t fft_lds_f2(int kz, int n) {
-  return 16 * ((size_t)(kz / 2) * fft_ld<f2v>(n) + 1) + 8 * (size_t)n;
-}
-
-__device__ __forceinline__ void fft_y_body_f2(int nkx, int ny, int nz_in, const FftDesc& fy,
-                                              const double2* __restrict__ twy, cplx<float>* __restrict__ S,
-                                              int KZ, int sign, int blk) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<f2v>* Y = reinterpret_cast<cplx<f2v>*>(smem_raw);
-  const int nch = (nz_in + KZ - 1) / KZ;
-  const int m = blk / nch, k0 = (blk - m * nch) * KZ;
-  const int KP = KZ / 2;
-  const int ldb = fft_ld<f2v>(ny);
-  const int lkp = 31 - __clz(KP);   // KZ is a power of two >= 2 (checked by the launcher)
-  const int spare = KP * ldb;
-  cplx<float>* twl = reinterpret_cast<cplx<float>*>(Y + KP * ldb + 1);
-  fft_stage_twiddles<float>(twl, twy, ny);
-  const int tot = ny * KP;
-  // staged, unlike fft_y_body: reading the first pass's inputs straight from the spectrum
-  // measured no faster here (float32 step +0.6 % at 192x192x128)
-  const cplx<float>* Sm = S + (size_t)m * ny * nz_in;
-  for (int t0 = threadIdx.x; t0 < tot; t0 += 256 * FFT_U) {
-    cplx<float> va[FFT_U], vb[FFT_U];
-#pragma unroll
-    for (int q = 0; q < FFT_U; ++q) {
-      const int t = min(t0 + 256 * q, tot - 1);
-      const int pp = t & (KP - 1), jj = t >> lkp;
-      const int ka = k0 + 2 * pp;
-      va[q] = Sm[(size_t)jj * nz_in + min(ka, nz_in - 1)];
-      vb[q] = Sm[(size_t)jj * nz_in + min(ka + 1, nz_in - 1)];
-    }
-#pragma unroll
-    for (int q = 0; q < FFT_U; ++q) {
-      const int t = t0 + 256 * q;
-      const int pp = t & (KP - 1), jj = t >> lkp;
-      const int ka = k0 + 2 * pp;
-      const cplx<float> a = ka < nz_in ? va[q] : cplx<float>{0.f, 0.f};
-      const cplx<float> b = ka + 1 < nz_in ? vb[q] : cplx<float>{0.f, 0.f};
-      Y[t < tot ? pp * ldb + fpad<f2v>(jj) : spare] = cplx<f2v>{f2v{a.re, b.re}, f2v{a.im, b.im}};
-    }
-  }
-  __syncthreads();
-  lds_fft<f2v, TC_FFT_ELEMS, float>(Y, ldb, KP, fy, twl, sign);
-  for (int t = threadIdx.x; t < tot; t += 256) {
-    const int pp = t & (KP - 1), j = t >> lkp;
-    const int ka = k0 + 2 * pp;
-    const cplx<f2v> y = Y[pp * ldb + fpad<f2v>(j)];
-    cplx<float>* o = S + ((size_t)m * ny + j) * nz_in;
-    if (ka < nz_in) o[ka] = {y.re.x, y.im.x};
-    if (ka + 1 < nz_in) o[ka + 1] = {y.re.y, y.im.y};
-  }
-}
-
-__global__ __launch_bounds__(256) void k_fft_y_f2(int nkx, int ny, int nz_in, FftDesc fy,
-                                                  const double2* __restrict__ twy, cplx<float>* __restrict__ S,
-                                                  int KZ, int sign) {
-  fft_y_body_f2(nkx, ny, nz_in, fy, twy, S, KZ, sign, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 template <typename T>
@@ -952,14 +887,6 @@ int launch_fft_y_t(const Poisson* pl, void* spec, int nz_in, int kz, int sign, h
   const Dims& d = pl->g->d;
   if (kz < 1 || (kz & (kz - 1)) != 0) return TC_ERR_ARG;
   const int blocks = pl->nkx * ((nz_in + kz - 1) / kz);
-  if constexpr (sizeof(T) == 4) {
-    if (kz >= 2) {
-      hipLaunchKernelGGL(k_fft_y_f2, dim3(blocks), dim3(256), fft_lds_f2(kz, d.ny), st, pl->nkx, d.ny, nz_in,
-                         pl->fy, pl->twy, (cplx<float>*)spec, kz, sign);
-      TC_CHECK_LAUNCH();
-      return TC_OK;
-    }
-  }
   const size_t lds = fft_lds<T>(kz, d.ny);
   hipLaunchKernelGGL((k_fft_y<T>), dim3(blocks), dim3(256), lds, st, pl->nkx, d.ny, nz_in, pl->fy,
                      pl->twy, (cplx<T>*)spec, kz, sign);
