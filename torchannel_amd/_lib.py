@@ -210,10 +210,18 @@ def _metric_host(t, n):
     return [float(x) for x in t.detach().to("cpu", torch.float64).reshape(-1)[:n].tolist()]
 
 
+def device_key(device) -> str:
+    """'cuda' and 'cuda:<current>' name the same device: one spelling for cache keys."""
+    d = torch.device(device)
+    if d.type == "cuda" and d.index is None:
+        d = torch.device("cuda", torch.cuda.current_device())
+    return str(d)
+
+
 def get_grid(nx, ny, nz, dx, dy, nu, top, dtype, device, dz_f=None, dz_c=None) -> Grid:
     """Cached Grid for these arguments.  The metric tensors are identified by object (weak
     reference) and version counter, so an in-place change or a new tensor rebuilds the grid."""
-    key = (int(nx), int(ny), int(nz), float(dx), float(dy), float(nu), int(top), dtype, str(device),
+    key = (int(nx), int(ny), int(nz), float(dx), float(dy), float(nu), int(top), dtype, device_key(device),
            id(dz_f), getattr(dz_f, "_version", -1), id(dz_c), getattr(dz_c, "_version", -1))
     ent = _grid_cache.get(key)
     if ent is not None:

@@ -160,6 +160,8 @@ class ChannelFlow:
                 self.nx, self.ny, self.nz, self.Lx, self.Ly, self.Lz, self.z_c, self.z_f, self.dz_c,
                 self.dz_f, self.dx, self.dy, self.nu, self.Re_tau,
                 z_plus_target=st.get("z_plus_target", 15.0), device=self.device)
+            self.turbulence_stats.use_grid(self._grid, self.wdtype, self.device)
+            self.turbulence_stats._plan(self.wdtype, self.device)      # built here, not in the time loop
             if st.get("restart_state_file") is not None:
                 self.turbulence_stats.load_state(st["restart_state_file"])
 

@@ -31,8 +31,13 @@ class TurbulenceStats:
         self.ky = 2 * np.pi * np.fft.rfftfreq(ny, d=dy)[1:]
         self._grids = {}
 
+    def use_grid(self, grid, dtype, device):
+        """Take the plane transforms of an existing grid of the same nx, ny, dx, dy (the solver's),
+        so that the first sample builds no plan of its own inside the time loop."""
+        self._grids[(dtype, L.device_key(device))] = grid
+
     def _plan(self, dt, device):
-        key = (dt, str(device))
+        key = (dt, L.device_key(device))
         if key not in self._grids:
             g = L.get_grid(self.nx, self.ny, self.nz, self.dx, self.dy, self.nu, L.TC_TOP_DIRICHLET, dt,
                            device, dz_f=self.dz_f, dz_c=self.dz_c)
