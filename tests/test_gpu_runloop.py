@@ -83,3 +83,18 @@ def test_run_simulation_matches_eager_loop(tc, tmp_path, n_steps, n_out, n_save,
     fields = np.load(os.path.join(folder, "fields.npz"))
     assert int(fields["step"]) == n_steps
     assert np.array_equal(fields["u"], ch.u.cpu().numpy())
+
+
+def test_statistics_share_the_solver_plan(tc, tmp_path):
+    """The statistics sample on the solver's own grid and plan ('cuda' and 'cuda:0' are one cache
+    key), so no second plan is built at the first sample inside the time loop."""
+    from torchannel_amd import _lib as L
+    c = cfg(str(tmp_path))
+    c["statistics"] = {"n_stats": 10, "t_stats": 0.0, "z_plus_target": 15.0}
+    ch = tc.ChannelFlow(config=c)
+    assert L.device_key("cuda") == L.device_key(ch.u.device)
+    assert ch.turbulence_stats._plan(ch.u.dtype, ch.u.device).value == ch._grid.poisson().value
+    n = len(L._grid_cache)
+    ch.step_imex(0.005)
+    ch.turbulence_stats.accumulate_statistics(ch.u, ch.v, ch.w)
+    assert len(L._grid_cache) == n and ch.turbulence_stats.n_samples == 1
