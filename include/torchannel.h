@@ -61,7 +61,8 @@ int tc_grid_destroy(tc_grid* g);
 
 /* Launch-shape knobs of a grid's kernels (no counterpart in the reference: its kernels are
  * TorchScript-fused).  Results do not depend on them; torchannel_amd/autotune.py picks them by
- * timing.  TC_TUNE_K1_JB: j-columns per explicit/CN pencil, 1..ny+2 (0 restores the default).
+ * timing.  TC_TUNE_K1_JB: j-columns per explicit/CN pencil, 1..ny+2 (0 restores the default,
+ * about ny/24 in steps of 4 within [8, 16]).
  * TC_TUNE_THOMAS_SB: (kx, ky) systems per block of the wall-normal Poisson solve (16 lanes each,
  * 32 above 128 rows): 4, 8 or 16, at most 256 threads per block (0 restores the default, 16). */
 #define TC_TUNE_K1_JB 1

@@ -72,3 +72,13 @@ def test_bad_knob_rejected(tc):
         ch._grid.set_tuning(L.TUNE_THOMAS_SB, 17)
     with pytest.raises(RuntimeError):
         ch._grid.set_tuning(99, 1)
+
+
+def test_default_pencil_length_follows_ny(tc):
+    """Untuned K1 pencils: about ny/24 j-columns in steps of 4, within [8, 16]."""
+    from torchannel_amd import _lib as L
+    dzf = torch.full((8,), 0.25, dtype=torch.float64, device="cuda")
+    dzc = torch.full((9,), 0.25, dtype=torch.float64, device="cuda")
+    for ny, jb in ((12, 8), (192, 8), (288, 12), (384, 16), (768, 16)):
+        g = L.get_grid(4, ny, 8, 0.1, 0.1, 1e-3, L.TC_TOP_DIRICHLET, torch.float64, "cuda", dz_f=dzf, dz_c=dzc)
+        assert g.get_tuning(L.TUNE_K1_JB) == jb

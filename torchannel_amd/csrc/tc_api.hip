@@ -169,7 +169,7 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
   const Grid& g = h->g;
   switch (key) {
     case TC_TUNE_K1_JB:
-      *value = g.k1_jb > 0 ? g.k1_jb : 8;
+      *value = g.k1_pencil();
       return TC_OK;
     case TC_TUNE_THOMAS_SB:
       *value = g.thomas_sb > 0 ? g.thomas_sb : 16;
@@ -685,7 +685,7 @@ static void fill_chan(EnsChan<T>& c, tc_solver* s) {
   c.th_cp0 = pl->th_cp0; c.th_iden0 = pl->th_iden0; c.th_rcr = pl->th_rcr;
   c.pairs = pl->pairs; c.kzy = pl->kzy;
   c.nkx = pl->nkx;
-  c.jb = g->k1_jb > 0 ? g->k1_jb : 8;
+  c.jb = g->k1_pencil();
   c.k5_blocks = k5_blocks(g);
   c.inv_volume = s->inv_volume;
   c.nu = g->nu;

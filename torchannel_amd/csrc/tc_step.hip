@@ -1010,7 +1010,7 @@ static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void*
                        const double* scal, const void* body, int mode, int branch, double theta,
                        hipStream_t st) {
   const Dims& d = gr->d;
-  const int jb = gr->k1_jb > 0 ? gr->k1_jb : 8;
+  const int jb = gr->k1_pencil();
   const int blocks = 3 * k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
@@ -1196,7 +1196,7 @@ int ens_launch_finalize(int dtype, const void* dch, int n, int flags, hipStream_
 }
 
 int k1_blocks(const Grid* g) {
-  const int jb = g->k1_jb > 0 ? g->k1_jb : 8;
+  const int jb = g->k1_pencil();
   return 3 * k1_bands_i(g->d.nx) * k1_bands_j(g->d.ny, jb);
 }
 
