@@ -4,9 +4,11 @@ Workload (the Re_tau~180 production case): 192x192x128, Lx=4pi, Ly=4pi/3, Lz=2, 
 Re_bulk=2870, constant-flow-rate forcing, adaptive dt at CFL 0.25 (dt update every 10 steps, on
 the device), turbulence statistics every 200 steps; seeded perturbed parabola (synthetic).
 `value` is the plain float64 path; the float32 path (float64 gauge mode, bulk/forcing and
-statistic sums) is reported beside it.  A "step" is one full IMEX step of the whole grid; steps
-run as HIP-graph chunks of 10 (the dt update sits inside the chunk), statistics run between
-chunks, all inside the timed region.
+statistic sums) is reported beside it.  A "step" is one full IMEX step of the whole grid.  After
+W eager warm-up steps, exactly K steps are timed: eager steps up to the next multiple of 10, then
+HIP-graph chunks of 10 (the dt update sits inside the chunk, at every 10th step counted from the
+start), then eager steps for the rest; statistics run every 200th step, all inside the timed
+region.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 20] [--no-fp32] [--no-cpu]
 """
@@ -39,6 +41,7 @@ def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
                  "dt_max": 0.01, "dt_min": 1e-4},
         "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": 0},
         "compute": {"device": "cuda", "precision": precision},
+        "output": {"save_init": False},
         "statistics": {"n_stats": 200, "t_stats": 0.0, "z_plus_target": 15.0},
     }
 
@@ -83,6 +86,32 @@ def phase_times(ch, reps):
             for q, i in enumerate(order)}
 
 
+def _eager(ch, n, interval):
+    """n eager steps with the reference cadence: the dt update before every interval-th step and
+    statistics after every n_stats-th step, both counted from the start of the run."""
+    for _ in range(n):
+        ch.current_step += 1
+        if ch.current_step % interval == 0:
+            ch.update_dt_device()
+        ch.step_imex(None)
+        if ch.current_step % ch.n_stats == 0:
+            ch.turbulence_stats.accumulate_statistics(ch.u, ch.v, ch.w)
+
+
+def _precapture(ch, chunk, interval, newest):
+    """Record the chunk graph for the AB2 parity the chunks will start from (capture launches
+    nothing, so the state is untouched)."""
+    from torchannel_amd import _lib as L
+    if newest < 0:
+        return
+    saved = ch._newest
+    L.call("tc_solver_set_newest", ch._solver, newest)
+    ch._newest = newest
+    ch.run_graph(0, chunk, dt_update_interval=interval)
+    L.call("tc_solver_set_newest", ch._solver, saved)
+    ch._newest = saved
+
+
 def run_gpu(precision, steps, warmup, phase_reps=20):
     import torch
     import torchannel_amd as tc
@@ -90,38 +119,42 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
 
     ch = tc.ChannelFlow(config=config(precision, steps=steps))
     interval = ch.dt_update_interval
-    # warm-up: forward-Euler start then AB2, eager, with the reference cadence
-    step = 0
-    for _ in range(warmup):
-        step += 1
-        if step % interval == 0:
-            ch.update_dt_device()
-        ch.step_imex(None)
-    ch.current_step = step
     chunk = 10
-    n_chunks = steps // chunk
-    ch.run_graph(0, chunk, dt_update_interval=interval)      # capture only
+    # warm-up: forward-Euler start then AB2, eager, with the reference cadence
+    _eager(ch, warmup, interval)
+    # the timed K steps: eager up to a multiple of the chunk, chunks, eager rest
+    # (with no warm-up the first chunk's worth is eager: a graph needs the AB2 history)
+    lead = min(steps, (-warmup) % chunk if warmup > 0 else chunk)
+    n_chunks = (steps - lead) // chunk
+    tail = steps - lead - n_chunks * chunk
+    if n_chunks:
+        newest = ch._newest if lead % 2 == 0 or ch._newest < 0 else 1 - ch._newest
+        if ch._newest < 0 and lead > 0:
+            newest = (lead - 1) % 2        # the forward-Euler start fills set 0, then they alternate
+        _precapture(ch, chunk, interval, newest)
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
     t0.record()
-    done = 0
+    _eager(ch, lead, interval)
     for _ in range(n_chunks):
         ch.run_graph(1, chunk, dt_update_interval=interval)
-        done += chunk
-        if (warmup + done) % ch.n_stats == 0:
+        if ch.current_step % ch.n_stats == 0:
             ch.turbulence_stats.accumulate_statistics(ch.u, ch.v, ch.w)
+    _eager(ch, tail, interval)
     t1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     ms = t0.elapsed_time(t1)
+    done = steps
     n_cells = NX * NY * NZ
-    # fixed-dt kernel throughput: the same number of steps with no CFL evaluation in the chunks
+    # fixed-dt kernel throughput: graph chunks with no CFL evaluation (at least one chunk)
+    fixed_chunks = max(1, steps // chunk)
     ch.run_graph(0, chunk, dt_update_interval=0)
     torch.cuda.synchronize()
     t0.record()
-    ch.run_graph(n_chunks, chunk, dt_update_interval=0)
+    ch.run_graph(fixed_chunks, chunk, dt_update_interval=0)
     t1.record()
     torch.cuda.synchronize()
     ms_fixed = t0.elapsed_time(t1)
@@ -130,7 +163,7 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
         "ms_per_step": ms / done,
         "wall_s": wall,
         "steps": done,
-        "fixed_dt_ms_per_step": ms_fixed / done,
+        "fixed_dt_ms_per_step": ms_fixed / (fixed_chunks * chunk),
         "dt_final": ch.device_dt(),
         "u_bulk": float(ch._scal[L.S_UBULK]),
         "stats_samples": ch.turbulence_stats.n_samples,
@@ -297,25 +330,49 @@ def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
             "speedup_vs_loop_graph": round(ms_loop_graph / ms_step, 2)}
 
 
-def cpu_baseline(budget_s=20.0):
-    """The float64 CPU oracle (a port of the reference op sequence, GPU-branch RHS) on the
-    host cores, on a bounded sample of the same 192x192x128 workload at fixed dt."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=15.0, c1_steps=200):
+    """The float64 CPU oracle (a port of the reference op sequence) on the host cores, following
+    the reference's CPU-device branch (separate RHS kernels, solver.py:601-622): a bounded sample
+    of the same 192x192x128 workload at fixed dt (after 2 warm-up steps), and the CPU-runnable
+    64x64x48 case (200 steps at dt = 0.005) in full.  Threads: the box's CPU share
+    (OMP_NUM_THREADS), not every core the host reports."""
     import torch
     from oracle import channel_oracle as O
 
-    torch.set_num_threads(os.cpu_count() or 1)
-    ch = O.Channel(NX, NY, NZ, 4 * math.pi, 4 * math.pi / 3, seed=0, gauge_zero_mode=True)
-    ch.step_imex(0.005)                       # warm-up (forward-Euler start)
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    torch.set_num_threads(threads)
+    ch = O.Channel(NX, NY, NZ, 4 * math.pi, 4 * math.pi / 3, seed=0, branch="cpu")
+    for _ in range(2):                        # warm-up (forward-Euler start, first AB2 step)
+        ch.step_imex(0.005)
     n = 0
     t0 = time.perf_counter()
-    while n < 10 and (time.perf_counter() - t0) < budget_s:
+    while n < 20 and (time.perf_counter() - t0) < budget_s:
         ch.step_imex(0.005)
         n += 1
     dt = time.perf_counter() - t0
+    del ch
+    c1 = O.Channel(64, 64, 48, 4 * math.pi, 4 * math.pi / 3, seed=0, branch="cpu")
+    t1 = time.perf_counter()
+    for _ in range(c1_steps):
+        c1.step_imex(0.005)
+    dt1 = time.perf_counter() - t1
     return {"value": NX * NY * NZ * n / dt / 1e6, "unit": "Mcell-steps/s", "cores": torch.get_num_threads(),
-            "kind": "port",
-            "sample": f"{n} float64 IMEX steps of the 192x192x128 channel (after 1 warm-up step), "
-                      f"fixed dt=0.005, torch CPU, {dt:.1f} s"}
+            "kind": "port", "cpu_model": _cpu_model(),
+            "sample": f"{n} float64 IMEX steps of the 192x192x128 channel (after 2 warm-up steps), CPU-branch "
+                      f"right-hand side, fixed dt=0.005, torch CPU, {dt:.1f} s",
+            "c1_64x64x48_200_steps": {"seconds": round(dt1, 2), "ms_per_step": round(dt1 * 1e3 / c1_steps, 2),
+                                      "value": round(64 * 64 * 48 * c1_steps / dt1 / 1e6, 3)}}
 
 
 def main():
@@ -341,8 +398,10 @@ def main():
         print("bench.py: measures one GPU only (the IMEX step of one channel on one MI355X); "
               "runs on several GPUs are not supported.", flush=True)
         sys.exit(2)
-    steps = max(10, (args.steps // 10) * 10)
-    warmup = max(10, (args.warmup // 10) * 10)
+    if args.steps < 1 or args.warmup < 0:
+        print("bench.py: --steps must be >= 1 and --warmup >= 0", flush=True)
+        sys.exit(2)
+    steps, warmup = args.steps, args.warmup
     print(f"[bench] {args.precision} {NX}x{NY}x{NZ} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
     if args.copy_calib_mb > 0:
         import torch

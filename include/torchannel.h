@@ -30,6 +30,7 @@ extern "C" {
 #define TC_ERR_DTYPE (-1002)        /* unsupported dtype */
 #define TC_ERR_UNSUPPORTED (-1003)  /* size outside the compiled kernel range */
 #define TC_ERR_NOMEM (-1004)        /* library workspace allocation failed */
+#define TC_ERR_STATE (-1005)        /* AB2 history parity differs from the captured graph's */
 
 #define TC_TOP_DIRICHLET 0          /* no-slip top wall  (solver.py:53-54) */
 #define TC_TOP_NEUMANN 1            /* free-slip top wall (solver.py:51-52) */
@@ -168,6 +169,10 @@ int tc_solver_step(tc_solver* s, double dt, int first, const void* body, int* ne
  * the target bulk velocity (skipped on restart, solver.py:271); then div -> Poisson with dt = 1
  * -> project -> boundary conditions. */
 int tc_solver_project_initial(tc_solver* s, int rescale, void* stream);
+/* The rescale alone (solver.py:271-276): u *= U_bulk / u_b when |u_b| > 1e-9, so that a caller
+ * can save the rescaled, not yet projected state (fields_init.npz, solver.py:337-340) before
+ * tc_solver_project_initial(s, 0, ..). */
+int tc_solver_rescale(tc_solver* s, void* stream);
 /* Adaptive dt on the device (solver.py:465-496 and the 5% rule of 807-812): reads the state,
  * writes the new dt into scal when it changed by more than 5%. */
 int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double dt_max, void* stream);
@@ -180,7 +185,8 @@ int tc_solver_update_dt(tc_solver* s, double cfl_target, double dt_min, double d
 int tc_solver_diagnostics(tc_solver* s, double z_c1, double* rows, int capacity, void* stream);
 /* Capture `steps` IMEX steps (even; AB2 history must exist) into a HIP graph, with the adaptive
  * dt update before every `dt_interval`-th step of the chunk when dt_interval > 0.  Replay
- * advances the state by that many steps on `stream`. */
+ * advances the state by that many steps on `stream`; it returns TC_ERR_STATE when the newest rhs
+ * set is not the one it was at capture (an odd number of eager steps in between). */
 int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_target, double dt_min,
                       double dt_max, void* stream);
 int tc_solver_replay(tc_solver* s, void* stream);
@@ -225,7 +231,10 @@ typedef struct tc_ensemble tc_ensemble;
 int tc_ensemble_create(tc_solver* const* solvers, int n, tc_ensemble** out);
 int tc_ensemble_destroy(tc_ensemble* e);
 int tc_ensemble_step(tc_ensemble* e, double dt, int first, void* stream);
-/* `steps` (even) ensemble steps captured into one HIP graph; replay advances every channel. */
+/* `steps` (even) ensemble steps captured into one HIP graph; replay advances every channel.
+ * Step and replay return TC_ERR_STATE when a channel's AB2 history is no longer in step with
+ * the ensemble's (a channel stepped on its own in between), or, for replay, when the newest rhs
+ * set differs from the one at capture. */
 int tc_ensemble_capture(tc_ensemble* e, int steps, void* stream);
 int tc_ensemble_replay(tc_ensemble* e, void* stream);
 

@@ -660,7 +660,11 @@ class Channel:
 
     def __init__(self, nx, ny, nz, Lx, Ly, Lz=2.0, gamma=2.6, Re=2870.0, U_bulk=1.0,
                  perturbation=0.1, seed=0, top_wall_bc_type="dirichlet", branch="gpu",
-                 gauge_zero_mode=False, grid=None):
+                 gauge_zero_mode=False, grid=None, restart=None):
+        """``restart`` = (u, v, w, step, time) starts from saved fields as initflow.py:106-167 and
+        solver.py:246-249 do: wall-normal ghosts re-applied (no-slip reflections, zero wall w,
+        whatever the top wall), no rescale (solver.py:271), step and time kept, forcing and AB2
+        history from zero; the initial projection runs either way."""
         self.nx, self.ny, self.nz = nx, ny, nz
         self.Lx, self.Ly, self.Lz = Lx, Ly, Lz
         self.nu = 1.0 / Re
@@ -673,11 +677,25 @@ class Channel:
         self.dx, self.dy = Lx / nx, Ly / ny
         self.cell_vol = (self.dx * self.dy * self.dz_f.view(1, 1, -1)).expand(nx, ny, nz)
         self.total_volume = Lx * Ly * Lz
-        self.u, self.v, self.w, self.p = initialize_flow(nx, ny, nz, self.z_c, Lz, U_bulk,
-                                                         perturbation, top_wall_bc_type, seed)
-        ub = compute_bulk_velocity(self.u, self.cell_vol, self.total_volume)  # solver.py:271-276
-        if abs(ub) > 1e-9:
-            self.u *= (U_bulk / ub)
+        step0, time0 = 0, 0.0
+        if restart is not None:
+            u, v, w, step0, time0 = restart
+            self.u, self.v, self.w = (torch.as_tensor(q, dtype=F64).clone() for q in (u, v, w))
+            self.p = torch.zeros(nx + 2, ny + 2, nz + 2, dtype=F64)
+            self.u[:, :, 0] = -self.u[:, :, 1]                            # initflow.py:156-165
+            self.u[:, :, -1] = -self.u[:, :, -2]
+            self.v[:, :, 0] = -self.v[:, :, 1]
+            self.v[:, :, -1] = -self.v[:, :, -2]
+            self.w[:, :, 0] = 0.0
+            self.w[:, :, -1] = 0.0
+        else:
+            self.u, self.v, self.w, self.p = initialize_flow(nx, ny, nz, self.z_c, Lz, U_bulk,
+                                                             perturbation, top_wall_bc_type, seed)
+            ub = compute_bulk_velocity(self.u, self.cell_vol, self.total_volume)  # solver.py:271-276
+            if abs(ub) > 1e-9:
+                self.u *= (U_bulk / ub)
+        # the state fields_init.npz holds (solver.py:337-340): before the initial projection
+        self.init_state = (self.u.clone(), self.v.clone(), self.w.clone(), self.p.clone())
         self.forcing = 0.0
         self.fft_data = initialize_fft_solver(nx, ny, nz, self.dx, self.dy, self.dz_c, self.dz_f,
                                               top_wall_bc_type)
@@ -688,8 +706,8 @@ class Channel:
         apply_bc_all(self.u, self.v, self.w, self.top)
         self.rhs_prev = None
         self.rhs_curr = None
-        self.time = 0.0
-        self.current_step = 0
+        self.time = time0
+        self.current_step = step0
 
     def rhs(self):
         f = rhs_fused_imex if self.branch == "gpu" else rhs_separate
@@ -739,21 +757,28 @@ class Channel:
         return cfl_dt(rate, cfl_target, dt_min, dt_max)
 
     def run(self, n_steps, dt, dt_update_interval=0, cfl_target=0.25, dt_min=1e-4, dt_max=0.01,
-            stats=None, n_stats=0, t_stats=0.0):
-        """The stepping part of run_simulation (solver.py:793-859): adaptive-dt cadence with the
-        5% rule, then step, then time advance, then statistics every n_stats steps."""
-        for _ in range(n_steps):
+            stats=None, n_stats=0, t_stats=0.0, t_max=math.inf, n_out=0):
+        """The stepping part of run_simulation (solver.py:793-859), up to step ``n_steps`` (an
+        absolute step count, as the reference's n_steps) while time < t_max: adaptive-dt
+        cadence with the 5% rule, then step, then time advance, then the n_out diagnostics row
+        (step, time, u_bulk, u_tau, forcing; solver.py:823-842) and statistics every n_stats
+        steps once time >= t_stats.  Returns (dt, rows)."""
+        rows = []
+        while self.current_step < n_steps and self.time < t_max:
             self.current_step += 1
             step = self.current_step
             if dt_update_interval > 0 and step % dt_update_interval == 0:
                 new = self.cfl_dt(cfl_target, dt_min, dt_max)
                 if abs(new - dt) / dt > 0.05:
                     dt = new
-            self.step_imex(dt)
+            ub, f = self.step_imex(dt)
             self.time += dt
+            if n_out > 0 and step % n_out == 0:
+                rows.append((step, self.time, float(ub), float(compute_u_tau(self.u, self.z_c, self.nu, self.top)),
+                             float(f)))
             if stats is not None and n_stats > 0 and self.time >= t_stats and step % n_stats == 0:
                 stats.accumulate_statistics(self.u, self.v, self.w)
-        return dt
+        return dt, rows
 
 
 PINS = {
@@ -770,13 +795,18 @@ PINS = {
     "solve_implicit_diffusion_w": "golden fixture (reference, shifted explicit half)",
     "compute_divergence": "brute-force loop + closed form on linear fields",
     "compute_bulk_velocity": "bulk of 1 = 1; forcing hits target (test_forcing_indexing.py:49,78)",
-    "compute_u_tau": "parabola closed form",
+    "compute_u_tau": "golden fixture (reference, both top walls, one-sided grid, distinct ghost/face-0/wall-plane means) "
+                     "+ planted mistakes (interior mean, ghost plane, top-wall distance, no abs) fail it + parabola",
     "project_velocity": "divergence <= 1e-12 after projection + golden trajectory",
     "initialize_fft_solver": "dense 7-point operator (projection.py:3-84) restated as a matrix",
     "solve_poisson_fft": "dense lstsq on the 7-point matrix (test_fft_vs_direct.py:97) + numpy.fft",
     "build_poisson_matrix": "brute-force loop over the stencil; constant and linear-in-x/y null behaviour",
     "solve_poisson": "residual of the dense system; FFT solve modulo the mean (unpinned matrix)",
-    "compute_cfl_fused": "brute-force loop",
+    "compute_cfl_fused": "golden fixture (reference, random signed fields, wall and ghost-row spikes, two grids) "
+                         "+ swapped dz_f/dz_c fails it + brute-force loop",
     "TurbulenceStats": "single-mode fields with known spectra + golden fixture from the reference",
     "Channel.step_imex": "golden 3-step trajectory from the reference ChannelFlow (cpu branch)",
+    "Channel.run": "golden run_simulation from the reference: adaptive dt, statistics, t_max between saves, restart "
+                   "(time series, final fields and dt, statistics)",
+    "Channel(restart=...)": "golden restart leg from the reference (fields_init, time series, final fields)",
 }

@@ -11,9 +11,16 @@ vectors for the parts of the IMEX path that closed forms and invariants leave op
   v-face ny behaviour), solve_implicit_diffusion_w (shifted explicit half), and
   TurbulenceStats.accumulate_statistics on one random field.
 * ``ops_v2.npz`` -- compute_momentum_rhs_fused_v2 (FE-scheme RHS) on random fields.
+* ``utau_cfl.npz`` -- compute_u_tau (both top walls) on fields whose ghost rows, face 0 and two
+  wall planes all have different means, over the one-sided ('bottom') grid where z_c[1] differs
+  from Lz - z_c[nz]; compute_cfl_fused on random signed fields with and without a corner spike.
+* ``runloop.npz`` -- run_simulation (solver.py:738-970) on a 16x12x16 channel (CPU branch) with
+  the adaptive-dt rule changing dt, statistics past t_stats and t_max reached between two saves:
+  every file it writes (fields_init/fields/fields_final, timeseries, statistics and their state);
+  then a restart from that fields.npz and statistics state, run to a fixed step count.
 
 Only data is written: inputs and the reference's outputs.  Usage:
-    PYTORCH_JIT=0 MPLBACKEND=Agg python tests/golden/make_golden.py [ops] [trajectory] [c1] [v2]
+    PYTORCH_JIT=0 MPLBACKEND=Agg python tests/golden/make_golden.py [ops] [trajectory] [c1] [v2] [utau_cfl] [runloop]
 """
 import os
 import sys
@@ -137,6 +144,110 @@ def make_v2():
                         u=np64(u), v=np64(v), w=np64(w), rhs_u=np64(ru), rhs_v=np64(rv), rhs_w=np64(rw))
 
 
+def make_utau_cfl():
+    """utau_cfl.npz: compute_u_tau and compute_cfl_fused on random fields (reference functions)."""
+    from utils import compute_u_tau
+    g = torch.Generator().manual_seed(2468)
+    nx, ny, nz = 12, 10, 14
+    Lx, Ly, Lz, nu = 2.0, 1.5, 2.0, 1.0 / 2870.0
+    out = {"nx": nx, "ny": ny, "nz": nz, "Lx": Lx, "Ly": Ly, "Lz": Lz, "nu": nu}
+    for grid_kind in ("symmetric", "bottom"):
+        z_f, z_c, dz_f, dz_c = generate_grid(2.6, nz, Lz, stretching_type=grid_kind)
+        u = torch.rand(nx + 1, ny + 2, nz + 2, generator=g) + 0.05 * torch.arange(nz + 2).view(1, 1, -1)
+        u[0] += 1.0                 # the duplicate periodic face
+        u[:, 0] += 2.0              # y ghost rows
+        u[:, ny + 1] -= 1.5
+        u[:, :, nz] -= 3.0          # top-wall plane: negative mean (abs inside the square root)
+        out[f"{grid_kind}_u"] = np64(u)
+        out[f"{grid_kind}_z_c"] = np64(z_c)
+        for top in ("dirichlet", "neumann"):
+            out[f"{grid_kind}_utau_{top}"] = float(compute_u_tau(u, z_c, nu, top_wall_bc_type=top))
+        for case in range(3):
+            u = (torch.rand(nx + 1, ny + 2, nz + 2, generator=g) - 0.3) * 2.0
+            v = (torch.rand(nx + 2, ny + 1, nz + 2, generator=g) - 0.5) * 0.7
+            w = (torch.rand(nx + 2, ny + 2, nz + 1, generator=g) - 0.5) * 0.4
+            if case == 1:
+                w[nx, ny, nz - 1] = 25.0           # spike next to the top wall in the last column
+            if case == 2:
+                v[1, 0, 1] = -40.0                 # spike in a y ghost row next to the bottom wall
+            dx, dy = Lx / nx, Ly / ny
+            rate = ref_ops.compute_cfl_fused(u, v, w, nx, ny, nz, dx, dy, dz_f, dz_c)
+            key = f"{grid_kind}_cfl{case}"
+            out[key + "_u"], out[key + "_v"], out[key + "_w"] = np64(u), np64(v), np64(w)
+            out[key] = float(rate)
+    np.savez_compressed(os.path.join(HERE, "utau_cfl.npz"), **out)
+
+
+RUNLOOP = {
+    "nx": 16, "ny": 12, "nz": 16, "seed": 0, "dt": 0.005, "dt_update_interval": 5, "CFL_target": 0.01,
+    "n_out": 4, "n_save": 8, "n_stats": 4, "t_stats": 0.05, "t_max": 0.15, "n_steps": 1000,
+    "restart_n_steps": 44,
+}
+
+
+def runloop_config(res, n_steps, t_max, field_file=None, stats_file=None):
+    """The reference YAML of the run-loop fixture (also rebuilt by the tests for this package)."""
+    r = RUNLOOP
+    cfg = {
+        "grid": {"nx": r["nx"], "ny": r["ny"], "nz": r["nz"]},
+        "domain": {"Lx": 4 * np.pi, "Ly": 4 * np.pi / 3, "Lz": 2.0},
+        "flow": {"Re": 2870.0, "Re_tau": 180.0, "U_bulk": 1.0, "gamma": 2.6},
+        "time": {"dt": r["dt"], "n_steps": n_steps, "t_max": t_max, "CFL_target": r["CFL_target"],
+                 "dt_update_interval": r["dt_update_interval"]},
+        "initialization": {"type": "parabolic", "perturbation_intensity": 0.1},
+        "output": {"results_folder": res, "n_out": r["n_out"], "n_save": r["n_save"]},
+        "compute": {"device": "cpu"},
+        "statistics": {"n_stats": r["n_stats"], "t_stats": r["t_stats"], "z_plus_target": 15.0},
+    }
+    if field_file is not None:
+        cfg["initialization"]["field_file"] = field_file
+    if stats_file is not None:
+        cfg["statistics"]["restart_state_file"] = stats_file
+    return cfg
+
+
+def _npz(path):
+    d = np.load(path, allow_pickle=False)
+    return {k: d[k] for k in d.files}
+
+
+def make_runloop(tmp):
+    """runloop.npz: every file run_simulation writes, for a fresh run and for a restart."""
+    r = RUNLOOP
+    out = {}
+    for leg in ("a", "b"):
+        res = os.path.join(tmp, f"run_{leg}")
+        if leg == "a":
+            cfg = runloop_config(res, r["n_steps"], r["t_max"])
+        else:
+            cfg = runloop_config(res, r["restart_n_steps"], 1000.0,
+                                 field_file=os.path.join(tmp, "a_fields.npz"),
+                                 stats_file=os.path.join(tmp, "a_stats_state.npz"))
+        path = os.path.join(tmp, f"cfg_{leg}.yaml")
+        with open(path, "w") as f:
+            yaml.safe_dump(cfg, f)
+        torch.manual_seed(r["seed"])
+        cf = ref_solver.ChannelFlow(path)
+        cf.run_simulation()
+        out[f"{leg}_final_dt"] = float(cf.dt)
+        out[f"{leg}_final_time"] = float(cf.time)
+        out[f"{leg}_files"] = np.array(sorted(f for f in os.listdir(res) if f.endswith(".npz")))
+        for name in ("fields_init", "fields", "fields_final", "timeseries", "turbulence_stats",
+                     "turbulence_stats_state"):
+            fp = os.path.join(res, name + ".npz")
+            if os.path.exists(fp):
+                for k, val in _npz(fp).items():
+                    out[f"{leg}_{name}__{k}"] = val
+        if leg == "a":
+            # the restart inputs: this run's last checkpoint and statistics state
+            import shutil
+            shutil.copy(os.path.join(res, "fields.npz"), os.path.join(tmp, "a_fields.npz"))
+            shutil.copy(os.path.join(res, "turbulence_stats_state.npz"), os.path.join(tmp, "a_stats_state.npz"))
+    for k, val in r.items():
+        out[f"cfg_{k}"] = val
+    np.savez_compressed(os.path.join(HERE, "runloop.npz"), **out)
+
+
 if __name__ == "__main__":
     only = sys.argv[1:]
     with tempfile.TemporaryDirectory() as tmp:
@@ -148,4 +259,8 @@ if __name__ == "__main__":
             make_c1(tmp)
         if not only or "v2" in only:
             make_v2()
+        if not only or "utau_cfl" in only:
+            make_utau_cfl()
+        if not only or "runloop" in only:
+            make_runloop(tmp)
     print("fixtures written to", HERE)

@@ -18,6 +18,7 @@ def cfg(nx, ny, nz, top="dirichlet"):
         "time": {"dt": 0.005, "n_steps": 10},
         "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": 0},
         "compute": {"device": "cuda", "precision": "fp64", "rhs_branch": "gpu"},
+        "output": {"save_init": False},
         "boundary_conditions": {"top_wall": {"type": top}},
     }
 

@@ -16,6 +16,7 @@ def cfg(nx, ny, nz, re=2870.0, gamma=2.6, seed=0, precision="fp64", private=True
         "time": {"dt": 0.005, "n_steps": 10},
         "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": seed},
         "compute": {"device": "cuda", "precision": precision, "private_plan": private},
+        "output": {"save_init": False},
         "boundary_conditions": {"top_wall": {"type": top}},
     }
 

@@ -46,7 +46,8 @@ def tc():
     return torchannel_amd
 
 
-SHAPES = [(12, 10, 14), (16, 12, 16), (64, 48, 40), (24, 18, 130)]
+# nz in 65-128 runs the two-rows-per-lane instances of the column kernels
+SHAPES = [(12, 10, 14), (16, 12, 16), (64, 48, 40), (32, 24, 96), (24, 18, 128), (24, 18, 130)]
 
 
 @pytest.mark.parametrize("shape", SHAPES)

@@ -15,6 +15,7 @@ def cfg(nx, ny, nz, precision="fp64", autotune=False):
         "time": {"dt": 0.005, "n_steps": 10},
         "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": 0},
         "compute": {"device": "cuda", "precision": precision, "autotune": autotune},
+        "output": {"save_init": False},
     }
 
 

@@ -65,6 +65,7 @@ SIGNATURES = {
     "tc_solver_destroy": [_vp],
     "tc_solver_step": [_vp, _d, _i, _vp, ctypes.POINTER(_i), _vp],
     "tc_solver_project_initial": [_vp, _i, _vp],
+    "tc_solver_rescale": [_vp, _vp],
     "tc_solver_update_dt": [_vp, _d, _d, _d, _vp],
     "tc_solver_diagnostics": [_vp, _d, _vp, _i, _vp],
     "tc_solver_capture": [_vp, _i, _i, _d, _d, _d, _vp],

@@ -56,6 +56,7 @@ struct tc_solver {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_steps = 0;
+  int graph_newest = -1;                                 // rhs set holding the newest rhs at capture
 };
 
 extern "C" {
@@ -69,6 +70,8 @@ const char* tc_strerror(int code) {
     case TC_ERR_DTYPE: return "torchannel: unsupported dtype (float64 or float32)";
     case TC_ERR_UNSUPPORTED: return "torchannel: size outside the compiled kernel range";
     case TC_ERR_NOMEM: return "torchannel: device workspace allocation failed";
+    case TC_ERR_STATE:
+      return "torchannel: the AB2 history is not in the state the captured graph was recorded for (capture again)";
     default: break;
   }
   if (code < 0) return hipGetErrorString((hipError_t)(-code));
@@ -403,6 +406,22 @@ __global__ void k_rescale_u(double* __restrict__ u64, float* __restrict__ u32, s
   }
 }
 
+int tc_solver_rescale(tc_solver* s, void* stream) {
+  if (!s) return TC_ERR_ARG;
+  const Grid* g = s->g;
+  const Dims& d = g->d;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nu_ = (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
+  // u_b of the initial field into scal[TC_S_UBULK] (cell volume dx*dy*dz_f expanded, solver.py:231)
+  TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
+                    kMaxRedBlocks, st));
+  hipLaunchKernelGGL(k_rescale_u, dim3(1024), dim3(256), 0, st,
+                     g->dtype == TC_F64 ? (double*)s->u : nullptr, g->dtype == TC_F64 ? nullptr : (float*)s->u,
+                     nu_, s->scal);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
 int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
   if (!s) return TC_ERR_ARG;
   const Grid* g = s->g;
@@ -412,15 +431,7 @@ int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
   const size_t nv_ = (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
   const size_t nw_ = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
   const size_t esz = g->dtype == TC_F64 ? 8 : 4;
-  // u_b of the initial field into scal[TC_S_UBULK] (cell volume dx*dy*dz_f expanded, solver.py:231)
-  if (rescale) {
-    TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
-                      kMaxRedBlocks, st));
-    hipLaunchKernelGGL(k_rescale_u, dim3(1024), dim3(256), 0, st,
-                       g->dtype == TC_F64 ? (double*)s->u : nullptr, g->dtype == TC_F64 ? nullptr : (float*)s->u,
-                       nu_, s->scal);
-    TC_CHECK_LAUNCH();
-  }
+  if (rescale) TC_RC(tc_solver_rescale(s, stream));
   TC_HIP(hipMemcpyAsync(s->su, s->u, esz * nu_, hipMemcpyDeviceToDevice, st));
   TC_HIP(hipMemcpyAsync(s->sv, s->v, esz * nv_, hipMemcpyDeviceToDevice, st));
   TC_HIP(hipMemcpyAsync(s->sw, s->w, esz * nw_, hipMemcpyDeviceToDevice, st));
@@ -475,6 +486,7 @@ int tc_solver_capture(tc_solver* s, int steps, int dt_interval, double cfl_targe
   s->graph = graph;
   TC_HIP(hipGraphInstantiate(&s->gexec, graph, nullptr, nullptr, 0));
   s->graph_steps = steps;
+  s->graph_newest = newest0;
   return TC_OK;
 }
 
@@ -513,6 +525,9 @@ int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream) {
 
 int tc_solver_replay(tc_solver* s, void* stream) {
   if (!s || !s->gexec) return TC_ERR_ARG;
+  // the graph reads R_{n-1} from the rhs set that was newest at capture; after an odd number of
+  // eager steps the other set holds it, and a replay would pair the wrong history
+  if (s->newest != s->graph_newest) return TC_ERR_STATE;
   TC_HIP(hipGraphLaunch(s->gexec, (hipStream_t)stream));
   return TC_OK;
 }
@@ -656,6 +671,7 @@ struct tc_ensemble {
   size_t plds[4] = {0, 0, 0, 0};
   int th_rm = 1;                            // rows per lane of the widest wall-normal Poisson solve
   int newest = -1;
+  int graph_newest = -1;                    // newest rhs set at capture
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -804,8 +820,17 @@ int tc_ensemble_destroy(tc_ensemble* e) {
 
 // tables: as in solver_step_impl (the ensemble steps at a fixed dt: a captured chunk rebuilds
 // them at its first step only)
+// Every channel's AB2 history must be where the ensemble left it (a channel stepped on its own
+// in between has swapped its rhs sets).
+static bool ensemble_in_step(const tc_ensemble* e, int newest) {
+  for (const tc_solver* s : e->solvers)
+    if (s->newest != newest) return false;
+  return true;
+}
+
 static int ensemble_step_impl(tc_ensemble* e, double dt, int first, hipStream_t st, bool tables = true) {
   if (!first && e->newest < 0) return TC_ERR_ARG;
+  if (!first && !ensemble_in_step(e, e->newest)) return TC_ERR_STATE;
   const int cur = first ? 0 : 1 - e->newest;
   const int prev = first ? -1 : e->newest;
   if (tables || dt > 0.0) TC_RC(ens_launch_tables(e->dtype, e->dch, e->n, dt, st));
@@ -848,11 +873,13 @@ int tc_ensemble_capture(tc_ensemble* e, int steps, void* stream) {
   if (err != hipSuccess) return -(int)err;
   e->graph = graph;
   TC_HIP(hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0));
+  e->graph_newest = newest0;
   return TC_OK;
 }
 
 int tc_ensemble_replay(tc_ensemble* e, void* stream) {
   if (!e || !e->gexec) return TC_ERR_ARG;
+  if (e->newest != e->graph_newest || !ensemble_in_step(e, e->newest)) return TC_ERR_STATE;
   TC_HIP(hipGraphLaunch(e->gexec, (hipStream_t)stream));
   return TC_OK;
 }

@@ -63,9 +63,11 @@ class ChannelEnsemble:
         """Replay `n_chunks` graph chunks of `chunk_steps` (even) steps of every channel."""
         if self.channels[0]._newest < 0:
             raise RuntimeError("run_graph needs one eager step first (forward-Euler start)")
-        if self._graph_steps != chunk_steps:
+        # a graph is recorded for one parity of the AB2 history (which rhs set is newest)
+        key = (chunk_steps, self.channels[0]._newest)
+        if self._graph_steps != key:
             L.call("tc_ensemble_capture", self._h, int(chunk_steps), self._stream())
-            self._graph_steps = chunk_steps
+            self._graph_steps = key
         for _ in range(n_chunks):
             L.call("tc_ensemble_replay", self._h, self._stream())
         for ch in self.channels:
@@ -91,6 +93,7 @@ def ensemble_configs(n=48, seed=2026, precision="fp32", nz_choices=(32, 48, 64, 
             "time": {"dt": dt, "n_steps": 300},
             "initialization": {"type": "parabolic", "perturbation_intensity": perturbation, "seed": i},
             "compute": {"device": "cuda", "precision": precision, "private_plan": True},
+            "output": {"save_init": False},
         })
     return out
 

@@ -45,7 +45,9 @@ class ChannelFlow:
     ``compute.rhs_branch`` ('gpu' = the fused RHS the reference runs on any HIP/CUDA device,
     'cpu' = the separate kernels of its CPU branch, solver.py:593-622), ``compute.autotune``
     (pick the kernels' launch shapes by timing after the first step; results do not change) and
-    ``compute.private_plan`` (a Poisson workspace of its own, for channels stepped together).
+    ``compute.private_plan`` (a Poisson workspace of its own, for channels stepped together) and
+    ``output.save_init`` (default true: write fields_init.npz at construction, as the reference
+    does; callers that only step, such as benchmarks, turn it off).
     ``config`` may be given as a dict instead of a file path.
     """
 
@@ -104,6 +106,7 @@ class ChannelFlow:
         self.results_folder = out.get("results_folder", "results")
         self.n_out = out.get("n_out", out.get("print_interval", 10))
         self.n_save = out.get("n_save", out.get("slice_interval", 100))
+        self.save_init = bool(out.get("save_init", True))
 
         # grid (utils.py:5-113), built on the host in float64 and moved to the device; the hybrid
         # grid sets nz from its two regions (solver.py:192-211)
@@ -182,9 +185,17 @@ class ChannelFlow:
         self._solver = h
         self._graph_key = None
 
-        # rescale to U_bulk unless restarting (solver.py:268-276) and the initial projection
-        # (solver.py:342-369); forcing and AB2 history start from zero (solver.py:249, 377-383)
-        L.call("tc_solver_project_initial", self._solver, 0 if field_file is not None else 1, self._stream())
+        # rescale to U_bulk unless restarting (solver.py:268-276), fields_init.npz of the state
+        # before the projection (solver.py:337-340: step 0, time 0, forcing 0 also on restart),
+        # and the initial projection (solver.py:342-369); forcing and AB2 history start from zero
+        # (solver.py:249, 377-383)
+        if field_file is None:
+            L.call("tc_solver_rescale", self._solver, self._stream())
+        if self.save_init:
+            ut0 = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
+            save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, 0, 0.0, ut0,
+                             0.0, self.results_folder, "fields_init.npz")
+        L.call("tc_solver_project_initial", self._solver, 0, self._stream())
         self.time = self.initial_time
         self._scal[L.S_TIME] = self.time
         self.current_step = self.initial_step
@@ -274,7 +285,11 @@ class ChannelFlow:
         history must exist (take at least one eager step first)."""
         if self._newest < 0:
             raise RuntimeError("run_graph needs one eager step first (forward-Euler start)")
-        key = (chunk_steps, dt_update_interval, self.cfl_target, self.dt_min, self.dt_max)
+        if self.body_force is not None:
+            raise RuntimeError("run_graph steps without a body force; use step_imex while body_force is set")
+        # the captured steps read R_{n-1} from the rhs set that is newest at capture: the parity
+        # of the AB2 history is part of what a graph was recorded for
+        key = (chunk_steps, dt_update_interval, self.cfl_target, self.dt_min, self.dt_max, self._newest)
         if self._graph_key != key:
             L.call("tc_solver_capture", self._solver, int(chunk_steps), int(dt_update_interval),
                    float(self.cfl_target), float(self.dt_min), float(self.dt_max), self._stream())
@@ -292,8 +307,14 @@ class ChannelFlow:
         """k IMEX steps with the adaptive-dt cadence: HIP-graph chunks when the chunk boundaries
         line up with the dt cadence and no body force is set, eager steps otherwise."""
         iv = self.dt_update_interval
+        if self._newest < 0 and k > 0:            # the forward-Euler start is an eager step
+            self.current_step += 1
+            if iv > 0 and self.current_step % iv == 0:
+                self.update_dt_device()
+            self.step_imex(None)
+            k -= 1
         chunk = self.n_out if self.n_out >= 2 and self.n_out % 2 == 0 else 0
-        if (chunk and self._newest >= 0 and self.body_force is None and k % chunk == 0
+        if (chunk and k > 0 and self.body_force is None and k % chunk == 0
                 and (iv <= 0 or (chunk % iv == 0 and self.current_step % iv == 0))):
             self.run_graph(k // chunk, chunk, dt_update_interval=max(iv, 0))
             return
@@ -315,20 +336,40 @@ class ChannelFlow:
                 print(f"{int(step):6d} {tm:10.6f} {dt:10.6f} {div:12.3e} {ub:10.6f} {ut:10.6f} {f:12.3e}",
                       flush=True)
         rows[0] = 0.0
+
+    def _time_refresh(self):
+        """One host sync: the device time and dt, and the largest dt any later step can take
+        (the adaptive rule clamps to dt_max, or keeps the current dt)."""
         self.time = float(self._scal[L.S_TIME].item())
         self.dt = float(self._scal[L.S_DT].item())
+        self._t_step = self.current_step
+        self._dt_bound = max(self.dt, self.dt_max) if self.dt_update_interval > 0 else self.dt
+
+    def _steps_below_tmax(self):
+        """How many further steps certainly start below t_max (the reference takes a step while
+        time < t_max, solver.py:793), from the last device time read and the dt bound."""
+        if not math.isfinite(self.t_max):
+            return 1 << 62
+        x = (self.t_max - self.time) / self._dt_bound - (self.current_step - self._t_step)
+        return int(math.floor(x * (1.0 - 1e-12))) if x > 0 else 0
 
     def run_simulation(self):
-        """The loop of solver.py:738-970: adaptive dt every dt_update_interval steps (evaluated on
-        the device), diagnostics every n_out steps, statistics every n_stats steps, and every
-        n_save steps the non-finite check (a device flag set by the projection kernel),
-        timeseries.npz append, fields.npz and the statistics state.
+        """The loop of solver.py:738-970: steps while step < n_steps and time < t_max, adaptive
+        dt every dt_update_interval steps (evaluated on the device), diagnostics every n_out
+        steps, statistics every n_stats steps once time >= t_stats, and every n_save steps the
+        non-finite check (a device flag set by the projection kernel), timeseries.npz append,
+        fields.npz and the statistics state; fields_final.npz at the end.
+
+        Output files follow the reference: the time-series rows recorded after the last n_save
+        step are not written (the reference writes them only at n_save steps), and a non-finite
+        state writes fields_error.npz, stops the loop and leaves fields.npz at the last good
+        checkpoint.
 
         The steps between two events run as HIP-graph chunks of n_out steps where the cadences
         allow it.  The n_out diagnostics (u_bulk, u_tau, forcing, max|div|) are rows of a device
-        time series written without a host sync; the host reads them, and the device time for
-        the t_max test, once per n_save steps (and at the statistics cadence, whose t_stats test
-        needs the time)."""
+        time series written without a host sync; the host reads them once per n_save steps and
+        reads the device time at the statistics cadence and near t_max: segments are cut so that
+        no step starts at or past t_max."""
         if self.time_scheme != "IMEX":
             raise ValueError(f"Unknown time stepping scheme: {self.time_scheme}. Use 'IMEX'.")
         os.makedirs(self.results_folder, exist_ok=True)
@@ -339,11 +380,7 @@ class ChannelFlow:
         rows = torch.zeros(8 * (cap + 1), dtype=torch.float64, device=self.device)
         t0 = _time.time()
         step = self.current_step
-        if self._newest < 0 and step < self.n_steps:        # forward-Euler start, eagerly
-            self._advance(1)
-            step = self.current_step
-            if step % n_out == 0:
-                self.record_diagnostics(rows, cap)
+        self._time_refresh()
 
         def ceil_to(x, m):
             return ((x + m - 1) // m) * m
@@ -352,24 +389,38 @@ class ChannelFlow:
             nxt = min(ceil_to(step + 1, n_out), ceil_to(step + 1, n_save), self.n_steps)
             if stats_on:
                 nxt = min(nxt, ceil_to(step + 1, self.n_stats))
-            self._advance(nxt - step)
+            k = nxt - step
+            room = self._steps_below_tmax()
+            if room < k:
+                if room == 0:
+                    if self._t_step != step:
+                        self._time_refresh()
+                        if self.time >= self.t_max:
+                            break
+                        room = self._steps_below_tmax()
+                    room = max(room, 1)          # time < t_max was just read: one step is allowed
+                k = min(k, room)
+            self._advance(k)
             step = self.current_step
             if step % n_out == 0:
                 self.record_diagnostics(rows, cap)
             if stats_on and step % self.n_stats == 0:
-                self.time = float(self._scal[L.S_TIME].item())
+                self._time_refresh()
                 if self.time >= self.t_stats:
                     self.turbulence_stats.accumulate_statistics(self.u, self.v, self.w)
             if step % n_save == 0:
                 self._drain(rows, ts)
+                self._time_refresh()
                 ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
                 if float(self._scal[L.S_NONFINITE].item()) != 0.0:          # solver.py:862-881
                     if self.verbose:
-                        print(f"ERROR: NaN or Inf detected in solution at step {step}", flush=True)
-                    ubn = float(self._scal[L.S_UBULK].item())
+                        print(f"ERROR: NaN or Inf detected in solution at step {step}, time = {self.time:.6f}",
+                              flush=True)
+                    ube = compute_bulk_velocity(self.u, self.cell_vol_ratio, self.total_volume)
                     save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step,
-                                     self.time, ut, (self.U_bulk - ubn) / self.dt, self.results_folder,
+                                     self.time, ut, (self.U_bulk - float(ube)) / self.dt, self.results_folder,
                                      "fields_error.npz")
+                    ts = {k: [] for k in ts}
                     break
                 if ts["step"]:
                     append_timeseries(os.path.join(self.results_folder, "timeseries.npz"), **ts)
@@ -378,11 +429,8 @@ class ChannelFlow:
                                  self.time, ut, float(self._scal[L.S_F].item()), self.results_folder, "fields.npz")
                 if self.turbulence_stats is not None and self.turbulence_stats.n_samples > 0:
                     self.turbulence_stats.save_state(self.stats_state_path)
-        self._drain(rows, ts)
-        if ts["step"]:
-            append_timeseries(os.path.join(self.results_folder, "timeseries.npz"), **ts)
-        self.time = float(self._scal[L.S_TIME].item())
-        self.dt = float(self._scal[L.S_DT].item())
+        self._drain(rows, ts)                   # rows after the last n_save step are not written
+        self._time_refresh()
         if self.verbose:
             print(f"Simulation complete: {step} steps, total time = {self.time:.6f}, final dt = {self.dt:.6f}, "
                   f"wall {(_time.time() - t0):.2f}s", flush=True)
@@ -392,5 +440,5 @@ class ChannelFlow:
         ut = compute_u_tau(self.u, self.z_c, self.nu, self.top_wall_bc_type)
         ubf = compute_bulk_velocity(self.u, self.cell_vol_ratio, self.total_volume)
         save_flow_fields(self.u, self.v, self.w, self.p, self.z_c, self.z_f, self.Lx, self.Ly, step, self.time, ut,
-                         (self.U_bulk - float(ubf)) / self.dt, self.results_folder, "fields.npz")
+                         (self.U_bulk - float(ubf)) / self.dt, self.results_folder, "fields_final.npz")
         return step
