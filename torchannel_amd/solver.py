@@ -199,6 +199,7 @@ class ChannelFlow:
         self.time = self.initial_time
         self._scal[L.S_TIME] = self.time
         self.current_step = self.initial_step
+        self._scal[L.S_STEP] = self.current_step      # stamps the diagnostics rows (kept on restart)
 
     # -- plumbing --------------------------------------------------------------------------
     def _stream(self):

@@ -25,7 +25,7 @@ def _prep(a, b, c, d):
     elif a.numel() == batch * n and b.numel() == batch * n and c.numel() == batch * n:
         stride = n
     else:
-        raise ValueError("a, b, c must be (n,) or match d's batch")
+        raise ValueError("tridiagonal solve: a, b, c must be (n,) or match d's batch")
     return n, batch, cplx, rd, a, b, c, dv, x, xv, stride
 
 
@@ -40,24 +40,7 @@ def thomas_solve(a, b, c, d):
 
 def pcr_solve(a, b, c, d):
     """Solve a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i (a_0, c_{n-1} ignored); x shaped like d."""
-    n = d.shape[-1]
-    cplx = d.is_complex()
-    rd = d.real.dtype if cplx else d.dtype
-    a = a.to(rd).contiguous()
-    b = b.to(rd).contiguous()
-    c = c.to(rd).contiguous()
-    dd = d.contiguous()
-    x = torch.empty_like(dd)
-    dv = torch.view_as_real(dd) if cplx else dd
-    xv = torch.view_as_real(x) if cplx else x
-    L.require_device(a, b, c, dv, xv)
-    batch = dd.numel() // n if n > 0 else 0
-    if a.dim() == 1:
-        stride = 0
-    else:
-        if a.numel() != batch * n or b.numel() != batch * n or c.numel() != batch * n:
-            raise ValueError("pcr_solve: a, b, c must be (n,) or match d's batch")
-        stride = n
+    n, batch, cplx, rd, a, b, c, dv, x, xv, stride = _prep(a, b, c, d)
     L.call("tc_pcr_batched", n, batch, L.dtype_code(rd), int(cplx), L.ptr(a), L.ptr(b), L.ptr(c),
            stride, L.ptr(dv), L.ptr(xv), L.stream_of(x))
     return x
