@@ -31,6 +31,7 @@ extern "C" {
 #define TC_ERR_UNSUPPORTED (-1003)  /* size outside the compiled kernel range */
 #define TC_ERR_NOMEM (-1004)        /* library workspace allocation failed */
 #define TC_ERR_STATE (-1005)        /* AB2 history parity differs from the captured graph's */
+#define TC_ERR_SCRATCH (-1006)      /* batch*n above the static scratch of tc_thomas_batched_static */
 
 #define TC_TOP_DIRICHLET 0          /* no-slip top wall  (solver.py:53-54) */
 #define TC_TOP_NEUMANN 1            /* free-slip top wall (solver.py:51-52) */
@@ -106,6 +107,18 @@ int tc_pcr_batched(int n, int batch, int dtype, int is_complex, const void* a, c
 int tc_thomas_batched(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
                       const void* c, int64_t abc_stride, const void* d, void* x, void* work,
                       void* stream);
+
+/* Allocation-free form of tc_thomas_batched.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The same sweep and results as tc_thomas_batched, with c' in
+ * a static device array of the library (TC_THOMAS_STATIC_MAX_F64 float64 words = 32 MiB per GPU,
+ * held until the process ends) instead of a `work` argument: batch*n at most
+ * TC_THOMAS_STATIC_MAX_F64 in float64 and twice that in float32, else TC_ERR_SCRATCH.  Calls on
+ * one GPU share that array, so calls that overlap on one GPU (several streams or threads) race.
+ * No hipMalloc, hipFree or allocating library call is made. */
+#define TC_THOMAS_STATIC_MAX_F64 (1 << 22)
+int tc_thomas_batched_static(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
+                             const void* c, int64_t abc_stride, const void* d, void* x, void* stream);
 
 /* compute_divergence (utils.py:226-240). */
 int tc_divergence(const tc_grid* g, const void* u, const void* v, const void* w, void* div,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "tc_cn_solve": [_vp, _i, _d, _vp, _d, _vp, _vp, _vp],
     "tc_pcr_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "tc_thomas_batched": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp],
+    "tc_thomas_batched_static": [_i, _i, _i, _i, _vp, _vp, _vp, _i64, _vp, _vp, _vp],
     "tc_divergence": [_vp, _vp, _vp, _vp, _vp, _vp],
     "tc_project": [_vp, _vp, _vp, _vp, _vp, _d, _vp, _vp],
     "tc_bulk_velocity": [_vp, _vp, _vp, _i64, _i64, _i64, _d, _vp, _vp],

@@ -70,6 +70,9 @@ const char* tc_strerror(int code) {
     case TC_ERR_DTYPE: return "torchannel: unsupported dtype (float64 or float32)";
     case TC_ERR_UNSUPPORTED: return "torchannel: size outside the compiled kernel range";
     case TC_ERR_NOMEM: return "torchannel: device workspace allocation failed";
+    case TC_ERR_SCRATCH:
+      return "torchannel: batch*n exceeds the static scratch of tc_thomas_batched_static (4194304 float64 or "
+             "8388608 float32 elements)";
     case TC_ERR_STATE:
       return "torchannel: the AB2 history is not in the state the captured graph was recorded for (capture again)";
     default: break;
@@ -228,6 +231,15 @@ int tc_thomas_batched(int n, int batch, int dtype, int is_complex, const void* a
   if (batch == 0) return TC_OK;
   return thomas_batched(n, batch, dtype, is_complex, a, b, c, (long long)abc_stride, d, x, work,
                         (hipStream_t)stream);
+}
+
+int tc_thomas_batched_static(int n, int batch, int dtype, int is_complex, const void* a, const void* b,
+                             const void* c, int64_t abc_stride, const void* d, void* x, void* stream) {
+  if (n < 1 || batch < 0 || !a || !b || !c || !d || !x) return TC_ERR_ARG;
+  if (dtype != TC_F64 && dtype != TC_F32) return TC_ERR_DTYPE;
+  if (batch == 0) return TC_OK;
+  return thomas_batched_static(n, batch, dtype, is_complex, a, b, c, (long long)abc_stride, d, x,
+                               (hipStream_t)stream);
 }
 
 int tc_divergence(const tc_grid* h, const void* u, const void* v, const void* w, void* div, void* stream) {

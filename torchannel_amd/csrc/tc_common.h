@@ -20,6 +20,16 @@
     if (e_ != hipSuccess) return -(int)e_;                     \
   } while (0)
 
+// Return a failed HIP call's status as the library's negative code.
+#define TC_HIP_RC(x)                                           \
+  do {                                                         \
+    hipError_t e_ = (x);                                       \
+    if (e_ != hipSuccess) return -(int)e_;                     \
+  } while (0)
+
+#define TC_MAX_DEVICES 64     // device ordinals with a per-device static scratch address
+#define TC_THOMAS_SCRATCH_WORDS TC_THOMAS_STATIC_MAX_F64
+
 // Extents and strides of the four ghosted arrays.
 struct Dims {
   int nx, ny, nz;

@@ -44,6 +44,10 @@ int poisson_phase(const Poisson* pl, int ph, bool from_vel, const void* u, const
 int plane_fft(const Poisson* pl, hipStream_t st);
 int thomas_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
                    long long s, const void* d, void* x, void* work, hipStream_t st);
+// The same sweep with c' in the library's static device scratch (no workspace argument, no
+// allocation); batch*n above the scratch capacity returns TC_ERR_SCRATCH.
+int thomas_batched_static(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
+                          long long s, const void* d, void* x, hipStream_t st);
 int pcr_batched(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
                 long long s, const void* d, void* x, hipStream_t st);
 

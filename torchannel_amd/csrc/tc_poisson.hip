@@ -10,6 +10,8 @@
 //                gauge-fixed in f64
 //   P4 k_fft_y : inverse complex FFT along y
 //   P5 k_inv_x : half spectra -> real rows, 1/(nx*ny), p written with every ghost image
+#include <mutex>
+
 #include "tc_ensemble.h"
 #include "tc_fft.h"
 #include "tc_internal.h"
@@ -692,6 +694,42 @@ int thomas_batched(int n, int batch, int dtype, int cplx_d, const void* a, const
 #undef TC_TH
   TC_CHECK_LAUNCH();
   return TC_OK;
+}
+
+// Allocation-free form of the batched Thomas sweep: c' lives in this file-scope device array
+// instead of a caller workspace.  Each GPU holds its own copy for the life of the process; every
+// call on one GPU shares it (calls that overlap on one GPU race), and it keeps the last call's
+// contents (c' is written before it is read, so nothing needs clearing).
+static __device__ __align__(256) double scratch[TC_THOMAS_SCRATCH_WORDS];
+
+// Device address of `scratch` per device ordinal, looked up once per device with that device
+// current.  In host code the array's name is only a host-side stand-in, never a device address.
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[TC_MAX_DEVICES] = {};
+
+static int scratch_address(void** out) {
+  int dev = -1;
+  TC_HIP_RC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= TC_MAX_DEVICES) return TC_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TC_HIP_RC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return TC_ERR_UNSUPPORTED;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TC_OK;
+}
+
+int thomas_batched_static(int n, int batch, int dtype, int cplx_d, const void* a, const void* b, const void* c,
+                          long long s, const void* d, void* x, hipStream_t st) {
+  const size_t words = dtype == TC_F64 ? (size_t)TC_THOMAS_SCRATCH_WORDS : 2 * (size_t)TC_THOMAS_SCRATCH_WORDS;
+  if ((size_t)n * (size_t)batch > words) return TC_ERR_SCRATCH;
+  void* work = nullptr;
+  const int rc = scratch_address(&work);
+  if (rc != TC_OK) return rc;
+  return thomas_batched(n, batch, dtype, cplx_d, a, b, c, s, d, x, work, st);
 }
 
 // ---------------------------------------------------------------------------------------------
