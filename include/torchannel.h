@@ -66,9 +66,14 @@ int tc_grid_destroy(tc_grid* g);
  * timing.  TC_TUNE_K1_JB: j-columns per explicit/CN pencil, 1..ny+2 (0 restores the default,
  * about ny/24 in steps of 4 within [8, 16]).
  * TC_TUNE_THOMAS_SB: (kx, ky) systems per block of the wall-normal Poisson solve (16 lanes each,
- * 32 above 128 rows): 4, 8 or 16, at most 256 threads per block (0 restores the default, 16). */
+ * 32 above 128 rows): 4, 8 or 16, at most 256 threads per block (0 restores the default, 16).
+ * TC_TUNE_K1_TILE: form of the explicit + CN kernel: 0 one component per block, each wavefront
+ * fetching its neighbour columns; 1 the three components of a band in one block with the
+ * neighbour columns staged through LDS (float64 up to 128 rows, float32 up to 256; larger
+ * columns, or a device with less LDS per block, run the first form). */
 #define TC_TUNE_K1_JB 1
 #define TC_TUNE_THOMAS_SB 2
+#define TC_TUNE_K1_TILE 3
 int tc_grid_set_tuning(tc_grid* g, int key, int value);
 int tc_grid_get_tuning(const tc_grid* g, int key, int* value);
 

@@ -42,7 +42,8 @@ def test_knobs_do_not_change_results(tc, precision):
     from torchannel_amd import _lib as L
     ref = run(tc, {}, precision)
     for knobs in ({L.TUNE_K1_JB: 1}, {L.TUNE_K1_JB: 3}, {L.TUNE_K1_JB: 26},
-                  {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 8}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 8}):
+                  {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 8}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 8},
+                  {L.TUNE_K1_TILE: 1}, {L.TUNE_K1_TILE: 1, L.TUNE_K1_JB: 3}):
         print(f"[tuning] {precision} {knobs}", flush=True)
         got = run(tc, knobs, precision)
         for a, b in zip(got, ref):

@@ -5,7 +5,9 @@ Each knob changes only how work is dealt to workgroups, never the arithmetic, so
 gives bit-identical results; the tuner times the kernel a knob drives with HIP events on the
 solver's stream and keeps the fastest setting on the grid.  Knobs:
 
-* ``TUNE_K1_JB``     j-columns per explicit/AB2/CN pencil (k_explicit_cn)
+* ``TUNE_K1_TILE``   form of the explicit/AB2/CN kernel: per-component pencils, or the tiled form
+  with the neighbour columns staged through LDS (searched jointly with the next knob)
+* ``TUNE_K1_JB``     j-columns per explicit/AB2/CN pencil or tile
 * ``TUNE_THOMAS_SB`` (kx, ky) systems per block of the wall-normal Poisson solve
 
 Chosen settings are cached per (nx, ny, nz, dtype, top wall, device name) in memory and, when
@@ -31,9 +33,11 @@ def _key(ch):
 
 
 def _candidates(ch):
+    """Groups of settings searched together: (list of {knob: value}, phase the group drives)."""
     jb = sorted({j for j in (2, 4, 6, 8, 12, 16, 24, 32) if j <= ch.ny + 2})
-    sb = [4, 8, 16]
-    return {L.TUNE_K1_JB: (jb, _PHASE_K1), L.TUNE_THOMAS_SB: (sb, _PHASE_THOMAS)}
+    k1 = [{L.TUNE_K1_TILE: t, L.TUNE_K1_JB: j} for t in (0, 1) for j in jb]
+    sb = [{L.TUNE_THOMAS_SB: v} for v in (4, 8, 16)]
+    return [(k1, _PHASE_K1), (sb, _PHASE_THOMAS)]
 
 
 def _time_phase(ch, phase, reps):
@@ -75,18 +79,21 @@ def tune(channel, reps=10, path=None, force=False):
     # the timing launches overwrite the step's intermediates and the newest rhs set: keep the state
     saved = [q.clone() for q in (ch.u, ch.v, ch.w, ch.p, ch._scal)] + [r.clone() for s in ch._rhs for r in s]
     chosen = {}
-    for knob, (values, phase) in _candidates(ch).items():
+    for group, phase in _candidates(ch):
         best, best_t = None, float("inf")
-        for v in values:
+        for setting in group:
             try:
-                ch._grid.set_tuning(knob, v)
+                for k, v in setting.items():
+                    ch._grid.set_tuning(k, v)
                 t = _time_phase(ch, phase, reps)
             except RuntimeError:
                 continue                     # setting outside the kernel's limits on this grid
             if t < best_t:
-                best, best_t = v, t
-        ch._grid.set_tuning(knob, best if best is not None else 0)
-        chosen[knob] = best if best is not None else 0
+                best, best_t = setting, t
+        best = best if best is not None else {k: 0 for k in group[0]}
+        for k, v in best.items():
+            ch._grid.set_tuning(k, v)
+        chosen.update(best)
     dst = [ch.u, ch.v, ch.w, ch.p, ch._scal] + [r for s in ch._rhs for r in s]
     for d, s in zip(dst, saved):
         d.copy_(s)

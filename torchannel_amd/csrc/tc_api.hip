@@ -165,6 +165,10 @@ int tc_grid_set_tuning(tc_grid* h, int key, int value) {
       if (value != 0 && value != 4 && value != 8 && value != 16) return TC_ERR_ARG;
       g.thomas_sb = value;
       return TC_OK;
+    case TC_TUNE_K1_TILE:
+      if (value != 0 && value != 1) return TC_ERR_ARG;
+      g.k1_tile = value;
+      return TC_OK;
     default:
       return TC_ERR_ARG;
   }
@@ -179,6 +183,9 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
       return TC_OK;
     case TC_TUNE_THOMAS_SB:
       *value = g.thomas_sb > 0 ? g.thomas_sb : 16;
+      return TC_OK;
+    case TC_TUNE_K1_TILE:
+      *value = g.k1_tile;
       return TC_OK;
     default:
       return TC_ERR_ARG;

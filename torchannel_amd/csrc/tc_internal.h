@@ -48,10 +48,12 @@ struct Grid {
   double* tabs = nullptr;         // CN tables: cell, face
   double* scal = nullptr;         // scalar block for stand-alone ops
   int k1_jb = 0;                  // j-columns per K1 pencil (0: the default below; set by the tuner)
+  int k1_tile = 0;                // K1 form: 0 pencils, 1 tiled (LDS-staged neighbour columns)
   int thomas_sb = 0;              // systems per wall-normal Thomas block (0: plan default)
 
   // K1 pencil length: the tuned value, else about ny/24 in steps of 4, within [8, 16] (measured
   // in float32 and float64: ny = 192 fastest at 8, ny = 384 at 16, by 4-6 % over 8)
+  bool k1_tiled() const { return k1_tile == 1; }
   int k1_pencil() const { return k1_jb > 0 ? k1_jb : std::min(16, std::max(8, 4 * ((d.ny + 48) / 96))); }
 
   template <typename T>

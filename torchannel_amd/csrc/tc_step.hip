@@ -751,6 +751,355 @@ __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
                          branch, theta, jb, xcd_remap(blockIdx.x, gridDim.x));
 }
 
+// ---------------------------------------------------------------------------------------------
+// K1, tiled form.  The same explicit terms, forcing, AB2 and CN solve as the pencils above
+// (every expression is the pencils' own, so results are bit-identical), with the neighbour
+// columns staged through LDS instead of each wavefront fetching them:
+//
+// * a block of TC_KT_WAVES = 12 wavefronts owns the three components of a band of TC_KT_I = 4
+//   consecutive i (wavefront w: component w / 4, i = i0 + w % 4) and all of them march along j
+//   together over the band's j range;
+// * each j-plane's columns of u, v and w for i0-1 .. i0+4 (the band plus one halo column each
+//   side, every row including the wall-normal ghosts) are read from HBM once, by the owning
+//   wavefront (the edge wavefronts also read the halo columns), into a two-plane LDS ring;
+// * a wavefront reads its i+-1 / j+1 / other-component neighbours, and their k+-1 rows, from
+//   the ring (LDS entry e of a column is global row k = e), so the only HBM loads per (i, j)
+//   and component are its own column, the AB2 history and the halo share;
+// * plane j+2 and the next AB2 history are fetched while step j computes; the ring holds three
+//   planes, so one block barrier per j step suffices: plane j+1 is written into the slot of plane
+//   j-2, which every wavefront finished reading before it reached the barrier of step j-1.
+// The pencils' per-wavefront row buffer still serves the CN right-hand side and PCR exchanges.
+// ---------------------------------------------------------------------------------------------
+#define TC_KT_I 4
+#define TC_KT_WAVES (3 * TC_KT_I)
+
+template <typename T, int R>
+struct KT {
+  static constexpr int CW = TC_WAVE * R + 2;                 // LDS column: global rows 0..64R+1
+  static constexpr int SLOTS = TC_KT_I + 2;                  // i0-1 .. i0+4
+  static constexpr int PLANE = 3 * SLOTS * CW;
+  static constexpr int TAB = (2 * TC_MAX_STAGES + 1) * TC_WAVE * R;
+  static constexpr int ROWS = 3 * TC_WAVE * R;
+  static constexpr int NPL = 3;                              // planes in the ring
+  static constexpr size_t BYTES = sizeof(T) * ((size_t)NPL * PLANE + 2 * TAB + (size_t)TC_KT_WAVES * ROWS);
+};
+
+// Source column of ring slot (component c, global i) for plane p, or null when the slot has no
+// reader: u at i = nx+1 is u(1) (the periodic wrap the CPU branch uses for u-face nx); v at
+// p = ny+1 is v(i, 1) in the CPU branch (its wrap for v-face ny).
+template <typename T>
+__device__ __forceinline__ const T* kt_src(const Dims& d, int c, int i, int p, bool cpu, const T* u, const T* v,
+                                           const T* w, int& kc) {
+  if (i < 0 || p < 0) return nullptr;
+  if (c == 0) {
+    kc = d.nz + 2;
+    if (i == d.nx + 1 && cpu) i = 1;
+    if (i > d.nx || p > d.ny + 1) return nullptr;
+    return u + (long long)i * d.sxu + (long long)p * d.syu;
+  }
+  if (c == 1) {
+    kc = d.nz + 2;
+    if (p == d.ny + 1 && cpu) p = 1;
+    if (i > d.nx + 1 || p > d.ny) return nullptr;
+    return v + (long long)i * d.sxv + (long long)p * d.syv;
+  }
+  kc = d.nz + 1;
+  if (i > d.nx + 1 || p > d.ny + 1) return nullptr;
+  return w + (long long)i * d.sxw + (long long)p * d.syw;
+}
+
+// A column in registers on its way to the ring: rows 1..64R, plus row 0 (lane 0) and row 64R+1
+// (lane 1) in g; rows past the column read zero.
+template <typename T, int R>
+struct KTCol {
+  T x[R];
+  T g;
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void kt_fetch(const T* col, int kc, KTCol<T, R>& c, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    const int k = r * TC_WAVE + lane + 1;
+    c.x[r] = k < kc ? col[k] : T(0);
+  }
+  const int kg = lane == 0 ? 0 : TC_WAVE * R + 1;
+  c.g = (lane < 2 && kg < kc) ? col[kg] : T(0);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void kt_put(T* colL, const KTCol<T, R>& c, int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) colL[r * TC_WAVE + lane + 1] = c.x[r];
+  if (lane == 0) colL[0] = c.g;
+  if (lane == 1) colL[TC_WAVE * R + 1] = c.g;
+}
+
+// rows m = r*64 + lane of an LDS column at global row m + OFF (OFF = 0: k-1, 1: k, 2: k+1)
+template <int OFF, typename T, int R>
+__device__ __forceinline__ void kt_rows(const T* colL, T (&x)[R], int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = colL[r * TC_WAVE + lane + OFF];
+}
+
+// Columns this wavefront puts into the ring for one plane: its own slot, and the halo slot of
+// the band's edge wavefronts.
+template <typename T, int R>
+struct KTLoads {
+  KTCol<T, R> own, halo;
+  bool has_own, has_halo;
+};
+
+template <typename T, int R>
+__device__ __forceinline__ void kt_load_plane(const Dims& d, int comp, int i, int il, int i0, int p, bool cpu,
+                                              const T* u, const T* v, const T* w, KTLoads<T, R>& L, int lane) {
+  int kc = 0;
+  const T* s = kt_src<T>(d, comp, i, p, cpu, u, v, w, kc);
+  L.has_own = s != nullptr;
+  if (L.has_own) kt_fetch<T, R>(s, kc, L.own, lane);
+  L.has_halo = false;
+  if (il == 0 || il == TC_KT_I - 1) {
+    const int ih = il == 0 ? i0 - 1 : i0 + TC_KT_I;
+    const T* h = kt_src<T>(d, comp, ih, p, cpu, u, v, w, kc);
+    L.has_halo = h != nullptr;
+    if (L.has_halo) kt_fetch<T, R>(h, kc, L.halo, lane);
+  }
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void kt_store_plane(T* plane, int comp, int il, const KTLoads<T, R>& L, int lane) {
+  constexpr int CW = KT<T, R>::CW;
+  T* base = plane + comp * KT<T, R>::SLOTS * CW;
+  if (L.has_own) kt_put<T, R>(base + (il + 1) * CW, L.own, lane);
+  if (L.has_halo) kt_put<T, R>(base + (il == 0 ? 0 : TC_KT_I + 1) * CW, L.halo, lane);
+}
+
+template <typename T, int R>
+__device__ __forceinline__ const T* kt_col(const T* plane, int c, int slot) {
+  return plane + (c * KT<T, R>::SLOTS + slot) * KT<T, R>::CW;
+}
+
+// Column kinds of a step: 0 none (no such column), 1 ghost (rhs = forcing only), 2 inner
+// without explicit terms (rhs = forcing, AB2 + CN), 3 stencil.
+template <int COMP>
+__device__ __forceinline__ int kt_kind(const Dims& d, int i, int j, bool cpu) {
+  if (COMP == 0) {
+    if (i > d.nx || j > d.ny + 1) return 0;
+    const bool i_in = i >= 1, j_in = j >= 1 && j <= d.ny;
+    if (!i_in || !j_in) return 1;
+    return (i <= d.nx - 1 || cpu) ? 3 : 2;
+  }
+  if (COMP == 1) {
+    if (i > d.nx + 1 || j > d.ny) return 0;
+    const bool i_in = i >= 1 && i <= d.nx, j_in = j >= 1;
+    if (!i_in || !j_in) return 1;
+    return (j <= (cpu ? d.ny : d.ny - 1)) ? 3 : 2;
+  }
+  if (i > d.nx + 1 || j > d.ny + 1) return 0;
+  const bool i_in = i >= 1 && i <= d.nx, j_in = j >= 1 && j <= d.ny;
+  return (i_in && j_in) ? 3 : 1;
+}
+
+// One component's march over the band's j range.  `ring` holds the LDS planes; plane p sits in
+// slot p % 3.
+template <typename T, int R, int COMP>
+__device__ __forceinline__ void kt_march(const K1Ctx<T, R>& c, int i, int il, int i0, int ja, int j0, int j1,
+                                         T* ring, const T* __restrict__ u, const T* __restrict__ v,
+                                         const T* __restrict__ w, const T* __restrict__ pq,
+                                         T* __restrict__ rq, T* __restrict__ sq) {
+  const Dims& d = c.d;
+  const int lane = c.lane;
+  constexpr bool FACE = COMP == 2;
+  const int n = FACE ? d.nz - 1 : d.nz;
+  const int Kc = FACE ? d.nz + 1 : d.nz + 2;
+  const long long sx = COMP == 0 ? d.sxu : COMP == 1 ? d.sxv : d.sxw;
+  const long long sy = COMP == 0 ? d.syu : COMP == 1 ? d.syv : d.syw;
+  const T h = T(0.5);
+  const bool step = (c.mode & TC_K1_STEP) != 0, ab2 = step && !(c.mode & TC_K1_FIRST);
+  T rz0[R], rz1[R], rz2[R];
+  k1_metric<T, R, FACE>(c, n, rz0, rz1, rz2);
+  const long long oi = (long long)i * sx;
+  const T* bi = (COMP == 0 && c.body != nullptr) ? c.body + oi : nullptr;
+  constexpr int PL = KT<T, R>::PLANE;
+  // own column at j-1 (uS / vS / wS) and the carried other-component rows of plane j-1:
+  // u: v(i, j-1), v(i+1, j-1); w: v(i, j-1) and its row k+1
+  T qS[R], ca[R], cb[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) { qS[r] = T(0); ca[r] = T(0); cb[r] = T(0); }
+  KTLoads<T, R> L;
+  kt_load_plane<T, R>(d, COMP, i, il, i0, ja, c.cpu, u, v, w, L, lane);
+  kt_store_plane<T, R>(ring + (ja % 3) * PL, COMP, il, L, lane);
+  if (ja + 1 <= j1) kt_load_plane<T, R>(d, COMP, i, il, i0, ja + 1, c.cpu, u, v, w, L, lane);
+  T rp[R], rpn[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) { rp[r] = T(0); rpn[r] = T(0); }
+  int kind = kt_kind<COMP>(d, i, ja, c.cpu);
+  if (ab2 && ja >= j0 && kind >= 2) col_load_hist<R, T>(pq + oi + (long long)ja * sy, rp, n, lane);
+  __syncthreads();
+  for (int j = ja; j < j1; ++j) {
+    const T* Pj = ring + (j % 3) * PL;
+    T* Pn = ring + ((j + 1) % 3) * PL;
+    if (j + 1 <= j1) kt_store_plane<T, R>(Pn, COMP, il, L, lane);
+    __syncthreads();
+    if (j + 2 <= j1) kt_load_plane<T, R>(d, COMP, i, il, i0, j + 2, c.cpu, u, v, w, L, lane);
+    const int kind_n = kt_kind<COMP>(d, i, j + 1, c.cpu);
+    if (ab2 && j + 1 < j1 && j + 1 >= j0 && kind_n >= 2)
+      col_load_hist<R, T>(pq + oi + (long long)(j + 1) * sy, rpn, n, lane);
+    const T* own = kt_col<T, R>(Pj, COMP, il + 1);
+    T qC[R];
+    if (kind != 0) kt_rows<1, T, R>(own, qC, lane);
+    const long long cj = (long long)j * sy;
+    if (j >= j0 && kind != 0) {
+      const T* bcol = bi != nullptr ? bi + cj : nullptr;
+      // ghost columns, and in rhs-only launches the inner columns without explicit terms, carry
+      // the forcing alone (k1_plain_cols)
+      if (kind == 1 || (kind == 2 && !step)) {
+        k1_ghost_col<T, R>(c, rq + oi + cj, bcol, n, Kc);
+      } else {
+        T rhs[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) rhs[r] = T(0);
+        if (kind == 3) {
+          T qN[R], qW[R], qE[R];
+          kt_rows<1, T, R>(kt_col<T, R>(Pn, COMP, il + 1), qN, lane);
+          kt_rows<1, T, R>(kt_col<T, R>(Pj, COMP, il), qW, lane);
+          kt_rows<1, T, R>(kt_col<T, R>(Pj, COMP, il + 2), qE, lane);
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            rhs[r] = (qE[r] - T(2) * qC[r] + qW[r]) * c.mt.kxx + (qN[r] - T(2) * qC[r] + qS[r]) * c.mt.kyy;
+          if constexpr (COMP == 0) {
+            if (i <= d.nx - 1) {                                   // advection (not the CPU wrap face)
+              T ub[R], ut[R], v00[R], v10[R], w00[R], w10[R], w0m[R], w1m[R];
+              kt_rows<0, T, R>(own, ub, lane);
+              kt_rows<2, T, R>(own, ut, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 1, il + 1), v00, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 1, il + 2), v10, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 2, il + 1), w00, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 2, il + 2), w10, lane);
+              kt_rows<0, T, R>(kt_col<T, R>(Pj, 2, il + 1), w0m, lane);
+              kt_rows<0, T, R>(kt_col<T, R>(Pj, 2, il + 2), w1m, lane);
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const T ur = h * (qC[r] + qE[r]), ul = h * (qW[r] + qC[r]);
+                const T fx = (ur * ur - ul * ul) * c.mt.rdx;
+                const T fy = (h * (v00[r] + v10[r]) * (h * (qC[r] + qN[r])) - h * (ca[r] + cb[r]) * (h * (qS[r] + qC[r]))) * c.mt.rdy;
+                const T fz = (h * (w00[r] + w10[r]) * (h * (qC[r] + ut[r])) - h * (w0m[r] + w1m[r]) * (h * (ub[r] + qC[r]))) * rz2[r];
+                rhs[r] = rhs[r] - (fx + fy + fz);
+              }
+            }
+          } else if constexpr (COMP == 1) {
+            if (j < d.ny) {
+              T vb[R], vt[R], u00[R], um0[R], u01[R], um1[R], w00[R], w0m[R], w01[R], w1m[R];
+              kt_rows<0, T, R>(own, vb, lane);
+              kt_rows<2, T, R>(own, vt, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 0, il + 1), u00, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 0, il), um0, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pn, 0, il + 1), u01, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pn, 0, il), um1, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pj, 2, il + 1), w00, lane);
+              kt_rows<0, T, R>(kt_col<T, R>(Pj, 2, il + 1), w0m, lane);
+              kt_rows<1, T, R>(kt_col<T, R>(Pn, 2, il + 1), w01, lane);
+              kt_rows<0, T, R>(kt_col<T, R>(Pn, 2, il + 1), w1m, lane);
+#pragma unroll
+              for (int r = 0; r < R; ++r) {
+                const T fx = (h * (u00[r] + u01[r]) * (h * (qC[r] + qE[r])) - h * (um0[r] + um1[r]) * (h * (qW[r] + qC[r]))) * c.mt.rdx;
+                const T vr = h * (qC[r] + qN[r]), vl = h * (qS[r] + qC[r]);
+                const T fy = (vr * vr - vl * vl) * c.mt.rdy;
+                const T fz = (h * (w00[r] + w01[r]) * (h * (qC[r] + vt[r])) - h * (w0m[r] + w1m[r]) * (h * (vb[r] + qC[r]))) * rz2[r];
+                rhs[r] = rhs[r] - (fx + fy + fz);
+              }
+            }
+          } else {
+            T wb[R], wt[R], u00[R], um0[R], u01[R], um1[R], v00[R], v01[R];
+            kt_rows<0, T, R>(own, wb, lane);
+            kt_rows<2, T, R>(own, wt, lane);
+            kt_rows<1, T, R>(kt_col<T, R>(Pj, 0, il + 1), u00, lane);
+            kt_rows<1, T, R>(kt_col<T, R>(Pj, 0, il), um0, lane);
+            kt_rows<2, T, R>(kt_col<T, R>(Pj, 0, il + 1), u01, lane);
+            kt_rows<2, T, R>(kt_col<T, R>(Pj, 0, il), um1, lane);
+            kt_rows<1, T, R>(kt_col<T, R>(Pj, 1, il + 1), v00, lane);
+            kt_rows<2, T, R>(kt_col<T, R>(Pj, 1, il + 1), v01, lane);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+              const T lap = (qE[r] - T(2) * qC[r] + qW[r]) * c.mt.kxx + (qN[r] - T(2) * qC[r] + qS[r]) * c.mt.kyy;
+              const T fx = (h * (u00[r] + u01[r]) * (h * (qC[r] + qE[r])) - h * (um0[r] + um1[r]) * (h * (qW[r] + qC[r]))) * c.mt.rdx;
+              const T fy = (h * (v00[r] + v01[r]) * (h * (qC[r] + qN[r])) - h * (ca[r] + cb[r]) * (h * (qS[r] + qC[r]))) * c.mt.rdy;
+              const T wr = h * (qC[r] + wt[r]), wl = h * (wb[r] + qC[r]);
+              const T fz = (wr * wr - wl * wl) * rz2[r];
+              rhs[r] = lap - (fx + fy + fz);
+            }
+          }
+        }
+        T x[R];
+        if (k1_rhs_ab2<T, R>(c, rhs, qC, rp, rq + oi + cj, bcol, n, Kc, true, x))
+          k1_cn_store<T, R, FACE>(c, x, sq + oi + cj, n, rz0, rz1, rz2, T(0));
+      }
+    }
+    // carry plane j into step j+1
+    if (kind != 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) qS[r] = qC[r];
+      if constexpr (COMP == 0) {
+        kt_rows<1, T, R>(kt_col<T, R>(Pj, 1, il + 1), ca, lane);
+        kt_rows<1, T, R>(kt_col<T, R>(Pj, 1, il + 2), cb, lane);
+      } else if constexpr (COMP == 2) {
+        kt_rows<1, T, R>(kt_col<T, R>(Pj, 1, il + 1), ca, lane);
+        kt_rows<2, T, R>(kt_col<T, R>(Pj, 1, il + 1), cb, lane);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) rp[r] = rpn[r];
+    kind = kind_n;
+  }
+}
+
+template <typename T, int R>
+__global__ __launch_bounds__(64 * TC_KT_WAVES) void k_explicit_cn_tile(
+    Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+    const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
+    T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
+    const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
+    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
+  extern __shared__ __align__(16) unsigned char kt_lds[];
+  T* ring = reinterpret_cast<T*>(kt_lds);
+  T* tabs = ring + KT<T, R>::NPL * KT<T, R>::PLANE;          // cell table, then face table
+  T* rows = tabs + 2 * KT<T, R>::TAB;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int nbj = k1_bands_j(d.ny, jb);
+  const int bi = b / nbj, bj = b - bi * nbj;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int comp = wv / TC_KT_I, il = wv - comp * TC_KT_I;
+  const int i0 = bi * TC_KT_I, i = i0 + il;
+  const int j0 = bj * jb, j1 = min(j0 + jb, d.ny + 2);
+  const int ja = j0 > 0 ? j0 - 1 : 0;
+  if (mode & TC_K1_STEP) {
+    for (int e = threadIdx.x; e < KT<T, R>::TAB; e += 64 * TC_KT_WAVES) {
+      const int row = e / (TC_WAVE * R), m = e - row * (TC_WAVE * R);
+      tabs[e] = (T)tab_cell[row * TC_MAX_NZ + m];
+      tabs[KT<T, R>::TAB + e] = (T)tab_face[row * TC_MAX_NZ + m];
+    }
+  }
+  K1Ctx<T, R> c;
+  c.d = d;
+  c.mt = mt;
+  c.rows = rows + wv * KT<T, R>::ROWS;
+  c.tabl = comp < 2 ? tabs : tabs + KT<T, R>::TAB;
+  c.lane = lane;
+  c.mode = mode;
+  c.cpu = branch == TC_BRANCH_CPU;
+  const double dt = scal[TC_S_DT];
+  c.dt = (T)dt;
+  c.beta = (T)((1.0 - theta) * dt * mt.nu);
+  c.F = (comp == 0 && (mode & TC_K1_FORCE)) ? (T)scal[TC_S_F] : T(0);
+  c.body = (comp == 0 && (mode & TC_K1_FORCE)) ? body : nullptr;
+  rows_zero_pads<R, T>(c.rows, lane);
+  // the tables and pads are visible after the march's first barrier
+  if (comp == 0) kt_march<T, R, 0>(c, i, il, i0, ja, j0, j1, ring, u, v, w, pu, ru, su);
+  else if (comp == 1) kt_march<T, R, 1>(c, i, il, i0, ja, j0, j1, ring, u, v, w, pv, rv, sv);
+  else kt_march<T, R, 2>(c, i, il, i0, ja, j0, j1, ring, u, v, w, pw, rw, sw);
+}
+
 // Ensemble form over the channels chans[0..n) (all of row count R); start = prefix sums of
 // their block counts.  rhs set `cur` receives the new rhs, `prev` holds the previous one.
 template <typename T, int R>
@@ -1003,6 +1352,29 @@ __global__ __launch_bounds__(1024) void k_step_finalize_ens(const EnsChan<T>* __
 // ---------------------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------------------
+// The tiled K1 needs KT<T, R>::BYTES of LDS per block; TC_ERR_UNSUPPORTED when the device's
+// per-block limit is smaller (the caller then launches the pencils).
+template <typename T, int R>
+static int launch_tile(int blocks, hipStream_t st, Dims d, Metric<T> mt, const T* u, const T* v, const T* w,
+                       const T* pu, const T* pv, const T* pw, T* ru, T* rv, T* rw, T* su, T* sv, T* sw,
+                       const double* tab_cell, const double* tab_face, const double* scal, const T* body,
+                       int mode, int branch, double theta, int jb) {
+  static int limit = -1;
+  if (limit < 0) {
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      lim = 0;
+    limit = lim;
+  }
+  const size_t bytes = KT<T, R>::BYTES;
+  if ((long long)bytes > (long long)limit) return TC_ERR_UNSUPPORTED;
+  hipLaunchKernelGGL((k_explicit_cn_tile<T, R>), dim3(blocks), dim3(64 * TC_KT_WAVES), bytes, st, d, mt, u, v, w,
+                     pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body, mode, branch, theta, jb);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
 template <typename T>
 static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void* w, const void* pu,
                        const void* pv, const void* pw, void* ru, void* rv, void* rw, void* su,
@@ -1011,9 +1383,26 @@ static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void*
                        hipStream_t st) {
   const Dims& d = gr->d;
   const int jb = gr->k1_pencil();
-  const int blocks = 3 * k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
   const Metric<T> mt = gr->metric<T>();
   const int R = (d.nz + TC_WAVE - 1) / TC_WAVE;
+  if (gr->k1_tiled()) {
+    const int tblocks = k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
+    int rc = TC_ERR_UNSUPPORTED;
+#define TC_KTL(RR)                                                                                          \
+  rc = launch_tile<T, RR>(tblocks, st, d, mt, (const T*)u, (const T*)v, (const T*)w, (const T*)pu,            \
+                          (const T*)pv, (const T*)pw, (T*)ru, (T*)rv, (T*)rw, (T*)su, (T*)sv, (T*)sw, tab_cell, \
+                          tab_face, scal, (const T*)body, mode, branch, theta, jb)
+    switch (R) {
+      case 1: TC_KTL(1); break;
+      case 2: TC_KTL(2); break;
+      case 3: if constexpr (sizeof(T) == 4) TC_KTL(3); break;
+      case 4: if constexpr (sizeof(T) == 4) TC_KTL(4); break;
+      default: break;
+    }
+#undef TC_KTL
+    if (rc != TC_ERR_UNSUPPORTED) return rc;      // else the block's LDS does not fit: pencils
+  }
+  const int blocks = 3 * k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
 #define TC_K1(RR)                                                                                 \
   hipLaunchKernelGGL((k_explicit_cn<T, RR>), dim3(blocks), dim3(64 * TC_K1_WAVES), 0, st, d, mt,  \
                      (const T*)u, (const T*)v, (const T*)w, (const T*)pu, (const T*)pv,            \
