@@ -188,6 +188,38 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
     return out
 
 
+def column_ceiling(precision, nx, ny, nz, jb, reps=20):
+    """Bandwidth of the explicit + CN kernel's column access pattern without its arithmetic
+    (tc_calib_columns): pattern 0 with the kernel's neighbour-column loads, pattern 1 with only
+    its compulsory streams; GB/s of the kernel's 12 algorithmic words per cell, from HIP events
+    on the launch stream."""
+    import ctypes
+    import torch
+    from torchannel_amd import _lib as L
+
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    arrs = [torch.rand(nx + 2, ny + 2, nz + 2, dtype=dt, device="cuda") for _ in range(12)]
+    ptrs = (ctypes.c_void_p * 12)(*[a.data_ptr() for a in arrs])
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    out = {}
+    for pat, name in ((0, "k1_pattern"), (1, "compulsory")):
+        for _ in range(3):
+            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, jb, ptrs, st)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, jb, ptrs, st)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        byts = 12 * (8 if precision == "fp64" else 4) * nx * ny * nz
+        out[name] = {"avg_us": round(us, 2), "gbs": round(byts / (us * 1e-6) / 1e9, 1)}
+    del arrs
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_adjoint(n_steps=100, ckpt=10, reps=3):
     """Adjoint/forward cost at 96x96x64: 100 fixed-dt float64 steps, checkpoints every 10, a
     seeded Gaussian body force (sigma 1e-3).  forward = objective(f) (the steps, the per-step
@@ -269,10 +301,14 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
     for name, avg in phase_times(ch, phase_reps).items():
         byts = words[name] * 4 * n_cells
         per[name] = {"avg_us": round(avg * 1e3, 2), "gbs": round(byts / (avg * 1e-3) / 1e9, 1) if byts else 0.0}
+    jb = ch._grid.get_tuning(L.TUNE_K1_JB)
+    del ch
+    torch.cuda.empty_cache()
     return {"workload": "384x384x256, Lx=2pi, Ly=pi, Re_bulk=10000, float32, adaptive dt CFL 0.25 (first dt "
                         f"from the CFL rule), {(steps // chunk) * chunk} steps", "finite": finite,
             "value": round(n_cells / (ms * 1e-3) / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(ms, 4),
-            "dt_final": dt_final, "u_bulk": u_bulk, "kernels": per}
+            "dt_final": dt_final, "u_bulk": u_bulk, "kernels": per,
+            "column_pattern_ceiling": column_ceiling("fp32", nx, ny, nz, jb)}
 
 
 def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
@@ -423,6 +459,9 @@ def main():
     if not args.no_fp32 and args.precision == "fp64":
         other = run_gpu("fp32", steps, warmup)
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
+    ceiling = column_ceiling(args.precision, NX, NY, NZ, 8)
+    print(f"[bench] column pattern: {ceiling['k1_pattern']['gbs']} GB/s, compulsory streams "
+          f"{ceiling['compulsory']['gbs']} GB/s", file=sys.stderr, flush=True)
     c3 = None
     if not args.no_c3:
         c3 = run_c3(steps=max(10, (args.c3_steps // 10) * 10))
@@ -469,7 +508,8 @@ def main():
                      "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes": k["alg_bytes"], "avg_us": round(k["avg_us"], 2),
                      "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "
-                                       "profiles/imex_step_192x192x128_hbm_traffic.md"},
+                                       "profiles/imex_step_192x192x128_hbm_traffic.md",
+                     "column_pattern_ceiling": ceiling},
         "fixed_dt": {"value": round(NX * NY * NZ / (main_run["fixed_dt_ms_per_step"] * 1e-3) / 1e6, 2),
                      "ms_per_step": round(main_run["fixed_dt_ms_per_step"], 5)},
         "step_roofline": {"alg_bytes_per_step": main_run["step_alg_bytes"],

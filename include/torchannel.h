@@ -8,15 +8,18 @@
  *
  * in float64 (TC_F64) or float32 (TC_F32), a hipStream_t passed as `void*`, and returns 0 on
  * success, a negative hipError_t on a HIP failure, or a TC_ERR_* code.  tc_strerror() maps
- * codes to text.  The library allocates only its own plans and workspaces (freed by the
- * matching *_destroy); it never allocates or frees caller memory.  Calls that share a grid,
- * Poisson plan or solver object must be issued on one stream at a time.
+ * codes to text.  The library allocates only its own grid tables, Poisson plans, graphs and the
+ * adjoint's and ensembles' buffers (freed by the matching *_destroy); a solver's scratch comes
+ * from a workspace the caller allocates (tc_solver_workspace_size).  It never allocates or frees
+ * caller memory.  Calls that share a grid, Poisson plan or solver object must be issued on one
+ * stream at a time.
  *
  * Reference citations are to gmcavallazzi/torChannel (file:line).
  */
 #ifndef TORCHANNEL_H
 #define TORCHANNEL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -174,10 +177,15 @@ int tc_stats_accumulate(tc_poisson* pl, const void* u, const void* v, const void
  *                U_bulk target, step count, non-finite flag (see TC_S_* in the library sources).
  * body (optional, shaped like u) is an additive body force next to F in rhs_u.
  * Volume V = Lx*Ly*Lz (solver.py:234).  branch selects the explicit-RHS semantics of the
- * reference's device branches (TC_BRANCH_GPU: solver.py:593-600, TC_BRANCH_CPU: 601-622). */
+ * reference's device branches (TC_BRANCH_GPU: solver.py:593-600, TC_BRANCH_CPU: 601-622).
+ * workspace: caller-allocated device memory of at least tc_solver_workspace_size bytes, 256-byte
+ * aligned, alive and unshared until tc_solver_destroy (it holds the post-CN velocities, the CN
+ * tables and the reduction partials; the solver itself allocates only its capture stream and
+ * graphs, and the grid and Poisson plan hold the rest). */
+int tc_solver_workspace_size(const tc_grid* g, size_t* bytes);
 int tc_solver_create(const tc_grid* g, tc_poisson* pl, void* u, void* v, void* w, void* p,
                      void* const* rhs_a, void* const* rhs_b, double* scal, double volume,
-                     int branch, tc_solver** out);
+                     int branch, void* workspace, size_t workspace_bytes, tc_solver** out);
 int tc_solver_destroy(tc_solver* s);
 /* One IMEX step.  dt > 0 sets scal[dt] first; dt <= 0 keeps the device value.  first != 0 is
  * the forward-Euler start (no rhs history).  Returns which rhs set now holds the newest rhs
@@ -217,6 +225,15 @@ int tc_solver_phase(tc_solver* s, int phase, int reps, void* stream);
  * start: restoring a checkpointed state (rhs arrays included) into the solver's buffers.  The
  * reference keeps this as the rhs_prev / rhs_curr swap of solver.py:649-651. */
 int tc_solver_set_newest(tc_solver* s, int newest);
+
+/* ---- roofline calibration ------------------------------------------------------------------
+ * No counterpart in the reference.  Sweeps twelve arrays shaped like p ((nx+2) x (ny+2) x
+ * (nz+2), arrays[0..5] read, arrays[6..11] written) in the column order of the explicit + CN
+ * kernel without its arithmetic: pattern 0 moves that kernel's eight input and two output
+ * columns per column step (neighbour columns included), pattern 1 only its compulsory streams
+ * (one input column and the AB2 history in, two columns out).  Timed beside the kernel, it gives
+ * the bandwidth this access pattern reaches on the device.  nz <= 256. */
+int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, void* const* arrays, void* stream);
 
 /* ---- discrete adjoint --------------------------------------------------------------------
  * No counterpart in the reference.  Gradient of J = sum_n weight_n * tau_w(u_n) with respect to

@@ -62,8 +62,9 @@ SIGNATURES = {
     "tc_poisson_destroy": [_vp],
     "tc_poisson_solve": [_vp, _vp, _vp, _vp],
     "tc_stats_accumulate": [_vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp],
+    "tc_solver_workspace_size": [_vp, ctypes.POINTER(ctypes.c_size_t)],
     "tc_solver_create": [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _d,
-                         _i, ctypes.POINTER(_vp)],
+                         _i, _vp, ctypes.c_size_t, ctypes.POINTER(_vp)],
     "tc_solver_destroy": [_vp],
     "tc_solver_step": [_vp, _d, _i, _vp, ctypes.POINTER(_i), _vp],
     "tc_solver_project_initial": [_vp, _i, _vp],
@@ -73,6 +74,7 @@ SIGNATURES = {
     "tc_solver_capture": [_vp, _i, _i, _d, _d, _d, _vp],
     "tc_solver_replay": [_vp, _vp],
     "tc_solver_phase": [_vp, _i, _i, _vp],
+    "tc_calib_columns": [_i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp), _vp],
 }
 
 _lib = None

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libtorchannel_hip.so")
-SOURCES = ["tc_step.hip", "tc_poisson.hip", "tc_ops.hip", "tc_adjoint.hip", "tc_api.hip"]
+SOURCES = ["tc_step.hip", "tc_poisson.hip", "tc_ops.hip", "tc_adjoint.hip", "tc_api.hip", "tc_calib.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TC_OFFLOAD_ARCH", "gfx950")

@@ -45,6 +45,7 @@ struct tc_solver {
   void* rs[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   double* scal = nullptr;
   double inv_volume = 0.0;
+  // scratch in the caller's workspace (tc_solver_workspace_size)
   void *su = nullptr, *sv = nullptr, *sw = nullptr;     // post-CN velocities
   double* tabs = nullptr;                                // CN tables: cell, face
   double* partials = nullptr;                            // K5 bulk partials
@@ -330,11 +331,39 @@ int tc_stats_accumulate(tc_poisson* pl, const void* u, const void* v, const void
 // ------------------------------------------------------------------------------------------
 // Solver
 // ------------------------------------------------------------------------------------------
+// The solver's scratch, carved from the caller's workspace in this order, each part aligned to
+// 256 bytes: post-CN velocities su, sv, sw (the state's shapes and dtype), the CN tables, the K5
+// bulk partials, the reduction partials, and the dt = 1 scalar block of the initial projection.
+static size_t ws_align(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static size_t solver_ws_parts(const Grid& g, size_t sizes[7]) {
+  const Dims& d = g.d;
+  const size_t esz = g.dtype == TC_F64 ? 8 : 4;
+  sizes[0] = esz * (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
+  sizes[1] = esz * (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
+  sizes[2] = esz * (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
+  sizes[3] = sizeof(double) * 2 * TC_TAB_STRIDE;
+  sizes[4] = sizeof(double) * (size_t)k5_blocks(&g);
+  sizes[5] = sizeof(double) * (size_t)std::max(kMaxRedBlocks, kCflBlocks);
+  sizes[6] = sizeof(double) * TC_NSCAL;
+  size_t tot = 0;
+  for (int q = 0; q < 7; ++q) tot += ws_align(sizes[q]);
+  return tot;
+}
+
+int tc_solver_workspace_size(const tc_grid* h, size_t* bytes) {
+  if (!h || !bytes) return TC_ERR_ARG;
+  size_t sizes[7];
+  *bytes = solver_ws_parts(h->g, sizes);
+  return TC_OK;
+}
+
 int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w, void* p,
                      void* const* rhs_a, void* const* rhs_b, double* scal, double volume,
-                     int branch, tc_solver** out) {
-  if (!h || !pl || !u || !v || !w || !p || !rhs_a || !rhs_b || !scal || !out || !(volume > 0.0))
+                     int branch, void* workspace, size_t workspace_bytes, tc_solver** out) {
+  if (!h || !pl || !u || !v || !w || !p || !rhs_a || !rhs_b || !scal || !out || !(volume > 0.0) || !workspace)
     return TC_ERR_ARG;
+  if (((uintptr_t)workspace & 255) != 0) return TC_ERR_ARG;
   if (branch != TC_BRANCH_GPU && branch != TC_BRANCH_CPU) return TC_ERR_ARG;
   for (int c = 0; c < 3; ++c)
     if (!rhs_a[c] || !rhs_b[c]) return TC_ERR_ARG;
@@ -348,18 +377,21 @@ int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w
   s->scal = scal;
   s->inv_volume = 1.0 / volume;
   s->branch = branch;
-  const Dims& d = g.d;
-  const size_t esz = g.dtype == TC_F64 ? 8 : 4;
-  const size_t nu_ = (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
-  const size_t nv_ = (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
-  const size_t nw_ = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
-  bool ok = hipMalloc(&s->su, esz * nu_) == hipSuccess && hipMalloc(&s->sv, esz * nv_) == hipSuccess &&
-            hipMalloc(&s->sw, esz * nw_) == hipSuccess &&
-            hipMalloc(&s->tabs, sizeof(double) * 2 * TC_TAB_STRIDE) == hipSuccess &&
-            hipMalloc(&s->partials, sizeof(double) * k5_blocks(&g)) == hipSuccess &&
-            hipMalloc(&s->cfl_partials, sizeof(double) * std::max(kMaxRedBlocks, kCflBlocks)) == hipSuccess &&
-            hipMalloc(&s->init_scal, sizeof(double) * TC_NSCAL) == hipSuccess &&
-            hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) == hipSuccess;
+  size_t sizes[7];
+  if (workspace_bytes < solver_ws_parts(g, sizes)) {
+    delete s;
+    return TC_ERR_ARG;
+  }
+  void* parts[7];
+  char* cur = (char*)workspace;
+  for (int q = 0; q < 7; ++q) {
+    parts[q] = cur;
+    cur += ws_align(sizes[q]);
+  }
+  s->su = parts[0]; s->sv = parts[1]; s->sw = parts[2];
+  s->tabs = (double*)parts[3]; s->partials = (double*)parts[4]; s->cfl_partials = (double*)parts[5];
+  s->init_scal = (double*)parts[6];
+  bool ok = hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) == hipSuccess;
   if (ok) {
     double init[TC_NSCAL] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     ok = hipMemcpy(s->init_scal, init, sizeof(init), hipMemcpyHostToDevice) == hipSuccess;
@@ -377,9 +409,7 @@ int tc_solver_destroy(tc_solver* s) {
   if (s->gexec) hipGraphExecDestroy(s->gexec);
   if (s->graph) hipGraphDestroy(s->graph);
   if (s->cap_stream) hipStreamDestroy(s->cap_stream);
-  hipFree(s->su); hipFree(s->sv); hipFree(s->sw); hipFree(s->tabs); hipFree(s->partials);
-  hipFree(s->cfl_partials); hipFree(s->init_scal);
-  delete s;
+  delete s;                                    // the workspace belongs to the caller
   return TC_OK;
 }
 

@@ -178,10 +178,17 @@ class ChannelFlow:
         self.tuning = None
         ra = (ctypes.c_void_p * 3)(*[q.data_ptr() for q in self._rhs[0]])
         rb = (ctypes.c_void_p * 3)(*[q.data_ptr() for q in self._rhs[1]])
+        # the solver's scratch (post-CN velocities, CN tables, reduction partials) in a
+        # torch-allocated workspace, kept alive with the solver
+        nbytes = ctypes.c_size_t()
+        L.call("tc_solver_workspace_size", self._grid.handle, ctypes.byref(nbytes))
+        self._workspace = torch.empty(nbytes.value + 256, dtype=torch.uint8, device=self.device)
+        ws_ptr = (self._workspace.data_ptr() + 255) & ~255
         h = ctypes.c_void_p()
         L.call("tc_solver_create", self._grid.handle, self._plan, L.ptr(self.u), L.ptr(self.v),
                L.ptr(self.w), L.ptr(self.p), ra, rb, L.ptr(self._scal), float(self.total_volume),
-               L.TC_BRANCH_CPU if branch == "cpu" else L.TC_BRANCH_GPU, ctypes.byref(h))
+               L.TC_BRANCH_CPU if branch == "cpu" else L.TC_BRANCH_GPU, ctypes.c_void_p(ws_ptr),
+               ctypes.c_size_t(nbytes.value), ctypes.byref(h))
         self._solver = h
         self._graph_key = None
 
