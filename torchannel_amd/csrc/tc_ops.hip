@@ -293,9 +293,11 @@ __global__ __launch_bounds__(256) void k_bulk(Dims d, const T* __restrict__ u, c
 // With update_dt, the fold (k_finalize) also applies compute_cfl_dt (solver.py:483-494) and the 5%
 // rule (solver.py:811-812) to the scalar block.  Column pencils: a block holds TC_CFL_WAVES
 // consecutive i, one wavefront per i marching j over jb columns, lane rows k = r*64 + lane + 1
-// (r < R), so the i +- 1 and j +- 1 neighbour columns come back from the CU's caches.  The loads of
-// two rows are issued together (rows past nz read row nz and are masked out).  The maximum does not
-// depend on the order, so the result is the same for any decomposition.
+// (r < R; rows past nz read row nz and are masked out).  Each step loads only the columns it has
+// not seen (u(i, j+1), u(i-1, j+1), v(i, j), v(i+1, j), w(i, j+1), w(i+1, j), with their k+1 or
+// k-1 rows) and carries the rest from the previous step in registers, so every column comes from
+// HBM about once per pencil.  The maximum does not depend on the order, so the result is the same
+// for any decomposition.
 #define TC_CFL_WAVES 4
 template <typename T, int R>
 __global__ __launch_bounds__(64 * TC_CFL_WAVES) void k_cfl(Dims d, Metric<T> mt, const T* __restrict__ u,
@@ -311,34 +313,59 @@ __global__ __launch_bounds__(64 * TC_CFL_WAVES) void k_cfl(Dims d, Metric<T> mt,
   const T q = T(0.25);
   double best = 0.0;
   if (i <= d.nx) {
-    for (int j = j0; j < j1; ++j) {
-      const T* uc = u + (long long)i * d.sxu + (long long)j * d.syu;
-      const T* vc = v + (long long)i * d.sxv + (long long)j * d.syv;
-      const T* wc = w + (long long)i * d.sxw + (long long)j * d.syw;
-      // two rows per batch: all R = 4 rows at once spills the CU's caches (measured 2x slower at
-      // 384x384x256)
-#pragma unroll 1
-      for (int r0 = 0; r0 < R; r0 += 2)
+    const T* ui = u + (long long)i * d.sxu;
+    const T* um = ui - d.sxu;
+    const T* vi = v + (long long)i * d.sxv;
+    const T* ve = vi + d.sxv;
+    const T* wi = w + (long long)i * d.sxw;
+    const T* we = wi + d.sxw;
+    int k[R];
+    T rzf[R], rzc[R];
 #pragma unroll
-      for (int rr = 0; rr < 2; ++rr) {
-        const int r = r0 + rr;
-        if (r >= R) break;
-        const int kr = r * TC_WAVE + lane + 1;
-        const int k = min(kr, d.nz);
-        const T rzf = mt.rdzf[k - 1];
-        const T ex = fabs(uc[k]) * mt.rdx +
-                     q * fabs(vc[k] + vc[k - d.syv] + vc[k + d.sxv] + vc[k + d.sxv - d.syv]) * mt.rdy +
-                     q * fabs(wc[k] + wc[k - 1] + wc[k + d.sxw] + wc[k + d.sxw - 1]) * rzf;
-        const T ey = q * fabs(uc[k] + uc[k + d.syu] + uc[k - d.sxu + d.syu] + uc[k - d.sxu]) * mt.rdx +
-                     fabs(vc[k]) * mt.rdy +
-                     q * fabs(wc[k] + wc[k + d.syw] + wc[k + d.syw - 1] + wc[k - 1]) * rzf;
+    for (int r = 0; r < R; ++r) {
+      k[r] = min(r * TC_WAVE + lane + 1, d.nz);
+      rzf[r] = mt.rdzf[k[r] - 1];
+      rzc[r] = mt.rdzc[k[r]];
+    }
+    // carried: u(i, j), u(i-1, j) with their k+1 rows; v(i, j-1) with its k+1 row, v(i+1, j-1);
+    // w(i, j) with its k-1 row
+    T uA[R], uA1[R], uB[R], uB1[R], vP[R], vP1[R], vQ[R], wA[R], wA0[R];
+    {
+      const long long cu = (long long)j0 * d.syu, cv = (long long)(j0 - 1) * d.syv, cw = (long long)j0 * d.syw;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int kk = k[r];
+        uA[r] = ui[cu + kk]; uA1[r] = ui[cu + kk + 1];
+        uB[r] = um[cu + kk]; uB1[r] = um[cu + kk + 1];
+        vP[r] = vi[cv + kk]; vP1[r] = vi[cv + kk + 1]; vQ[r] = ve[cv + kk];
+        wA[r] = wi[cw + kk]; wA0[r] = wi[cw + kk - 1];
+      }
+    }
+    for (int j = j0; j < j1; ++j) {
+      const long long cu = (long long)(j + 1) * d.syu, cv = (long long)j * d.syv;
+      const long long cwn = (long long)(j + 1) * d.syw, cwe = (long long)j * d.syw;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int kk = k[r];
+        const T uN = ui[cu + kk], uN1 = ui[cu + kk + 1], umN = um[cu + kk], umN1 = um[cu + kk + 1];
+        const T vC = vi[cv + kk], vC1 = vi[cv + kk + 1], vE = ve[cv + kk];
+        const T wN = wi[cwn + kk], wN0 = wi[cwn + kk - 1], wE = we[cwe + kk], wE0 = we[cwe + kk - 1];
+        const T ex = fabs(uA[r]) * mt.rdx +
+                     q * fabs(vC + vP[r] + vE + vQ[r]) * mt.rdy +
+                     q * fabs(wA[r] + wA0[r] + wE + wE0) * rzf[r];
+        const T ey = q * fabs(uA[r] + uN + umN + uB[r]) * mt.rdx +
+                     fabs(vC) * mt.rdy +
+                     q * fabs(wA[r] + wN + wN0 + wA0[r]) * rzf[r];
         // rows k = nz read the ghost row nz + 1 of u and v here; their ez is not used
-        const T ez = q * fabs(uc[k] + uc[k - d.sxu] + uc[k - d.sxu + 1] + uc[k + 1]) * mt.rdx +
-                     q * fabs(vc[k] + vc[k - d.syv] + vc[k - d.syv + 1] + vc[k + 1]) * mt.rdy +
-                     fabs(wc[k]) * mt.rdzc[k];
+        const T ez = q * fabs(uA[r] + uB[r] + uB1[r] + uA1[r]) * mt.rdx +
+                     q * fabs(vC + vP[r] + vP1[r] + vC1) * mt.rdy +
+                     fabs(wA[r]) * rzc[r];
         double m = fmax((double)ex, (double)ey);
-        if (k < d.nz) m = fmax(m, (double)ez);
-        if (kr <= d.nz) best = fmax(best, m);
+        if (kk < d.nz) m = fmax(m, (double)ez);
+        if (r * TC_WAVE + lane + 1 <= d.nz) best = fmax(best, m);
+        uA[r] = uN; uA1[r] = uN1; uB[r] = umN; uB1[r] = umN1;
+        vP[r] = vC; vP1[r] = vC1; vQ[r] = vE;
+        wA[r] = wN; wA0[r] = wN0;
       }
     }
   }
