@@ -73,10 +73,14 @@ int tc_grid_destroy(tc_grid* g);
  * TC_TUNE_K1_TILE: form of the explicit + CN kernel: 0 one component per block, each wavefront
  * fetching its neighbour columns; 1 the three components of a band in one block with the
  * neighbour columns staged through LDS (float64 up to 128 rows, float32 up to 256; larger
- * columns, or a device with less LDS per block, run the first form). */
+ * columns, or a device with less LDS per block, run the first form).
+ * TC_TUNE_THOMAS_LDS: form of the wall-normal Poisson solve: 0 each lane loads its own rows; 1 (the
+ * default) a block's systems are copied into LDS and back with consecutive 16-byte accesses
+ * (where the rows are 16-byte vectors; otherwise the first form runs). */
 #define TC_TUNE_K1_JB 1
 #define TC_TUNE_THOMAS_SB 2
 #define TC_TUNE_K1_TILE 3
+#define TC_TUNE_THOMAS_LDS 4
 int tc_grid_set_tuning(tc_grid* g, int key, int value);
 int tc_grid_get_tuning(const tc_grid* g, int key, int* value);
 

@@ -37,15 +37,21 @@ def run(tc, knobs, precision, steps=4, shape=(32, 24, 48)):
     return [q.clone() for q in (ch.u, ch.v, ch.w, ch.p, ch._scal)]
 
 
+# nz = 48: rows per lane padded (scalar row accesses); 64: 16 lanes x 4 rows and 256: 32 lanes x
+# 8 rows, both as 16-byte row vectors (where the LDS-staged Poisson form applies); 17 x 24 = 408
+# systems leave a partial last block at every systems-per-block setting
+@pytest.mark.parametrize("shape", [(32, 24, 48), (32, 24, 64), (32, 24, 256)])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_knobs_do_not_change_results(tc, precision):
+def test_knobs_do_not_change_results(tc, precision, shape):
     from torchannel_amd import _lib as L
-    ref = run(tc, {}, precision)
+    ref = run(tc, {}, precision, shape=shape)
     for knobs in ({L.TUNE_K1_JB: 1}, {L.TUNE_K1_JB: 3}, {L.TUNE_K1_JB: 26},
                   {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 8}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 8},
-                  {L.TUNE_K1_TILE: 1}, {L.TUNE_K1_TILE: 1, L.TUNE_K1_JB: 3}):
-        print(f"[tuning] {precision} {knobs}", flush=True)
-        got = run(tc, knobs, precision)
+                  {L.TUNE_K1_TILE: 1}, {L.TUNE_K1_TILE: 1, L.TUNE_K1_JB: 3},
+                  {L.TUNE_THOMAS_LDS: 1}, {L.TUNE_THOMAS_LDS: 1, L.TUNE_THOMAS_SB: 4},
+                  {L.TUNE_THOMAS_LDS: 1, L.TUNE_THOMAS_SB: 8}):
+        print(f"[tuning] {precision} {shape} {knobs}", flush=True)
+        got = run(tc, knobs, precision, shape=shape)
         for a, b in zip(got, ref):
             assert torch.equal(a, b), knobs
 
@@ -72,6 +78,8 @@ def test_bad_knob_rejected(tc):
         ch._grid.set_tuning(L.TUNE_K1_JB, 10 ** 6)
     with pytest.raises(RuntimeError):
         ch._grid.set_tuning(L.TUNE_THOMAS_SB, 17)
+    with pytest.raises(RuntimeError):
+        ch._grid.set_tuning(L.TUNE_THOMAS_LDS, 2)
     with pytest.raises(RuntimeError):
         ch._grid.set_tuning(99, 1)
 

@@ -24,6 +24,7 @@ S_DT, S_F, S_UBULK, S_TIME, S_CFL, S_UTARGET, S_STEP, S_NONFINITE = 0, 1, 2, 3, 
 TUNE_K1_JB = 1
 TUNE_THOMAS_SB = 2
 TUNE_K1_TILE = 3
+TUNE_THOMAS_LDS = 4
 
 _vp, _i, _d, _i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int64
 _pd = ctypes.POINTER(ctypes.c_double)

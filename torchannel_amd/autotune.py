@@ -9,6 +9,8 @@ solver's stream and keeps the fastest setting on the grid.  Knobs:
   with the neighbour columns staged through LDS (searched jointly with the next knob)
 * ``TUNE_K1_JB``     j-columns per explicit/AB2/CN pencil or tile
 * ``TUNE_THOMAS_SB`` (kx, ky) systems per block of the wall-normal Poisson solve
+* ``TUNE_THOMAS_LDS`` that solve's rows copied through LDS with whole-block 16-byte accesses, or
+  loaded by each lane directly (searched jointly with the previous knob)
 
 Chosen settings are cached per (nx, ny, nz, dtype, top wall, device name) in memory and, when
 ``path`` (or the ``TORCHANNEL_TUNE_CACHE`` environment variable) names a JSON file, on disk.
@@ -36,7 +38,7 @@ def _candidates(ch):
     """Groups of settings searched together: (list of {knob: value}, phase the group drives)."""
     jb = sorted({j for j in (2, 4, 6, 8, 12, 16, 24, 32) if j <= ch.ny + 2})
     k1 = [{L.TUNE_K1_TILE: t, L.TUNE_K1_JB: j} for t in (0, 1) for j in jb]
-    sb = [{L.TUNE_THOMAS_SB: v} for v in (4, 8, 16)]
+    sb = [{L.TUNE_THOMAS_LDS: s, L.TUNE_THOMAS_SB: v} for s in (1, 0) for v in (4, 8, 16)]
     return [(k1, _PHASE_K1), (sb, _PHASE_THOMAS)]
 
 

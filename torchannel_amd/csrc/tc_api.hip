@@ -170,6 +170,10 @@ int tc_grid_set_tuning(tc_grid* h, int key, int value) {
       if (value != 0 && value != 1) return TC_ERR_ARG;
       g.k1_tile = value;
       return TC_OK;
+    case TC_TUNE_THOMAS_LDS:
+      if (value != 0 && value != 1) return TC_ERR_ARG;
+      g.thomas_lds = value;
+      return TC_OK;
     default:
       return TC_ERR_ARG;
   }
@@ -187,6 +191,9 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
       return TC_OK;
     case TC_TUNE_K1_TILE:
       *value = g.k1_tile;
+      return TC_OK;
+    case TC_TUNE_THOMAS_LDS:
+      *value = g.thomas_lds;
       return TC_OK;
     default:
       return TC_ERR_ARG;
@@ -744,10 +751,8 @@ static void fill_chan(EnsChan<T>& c, tc_solver* s) {
   c.body = nullptr;
   c.fx = pl->fx; c.fy = pl->fy; c.twx = pl->twx; c.twy = pl->twy;
   c.spec = (cplx<T>*)pl->spec;
-  c.cl = pl->cl;
-  c.th_cp = (const T*)pl->th_cp; c.th_last = (const T*)pl->th_last;
-  c.th_clT = (const T*)pl->th_clT; c.th_rcrT = (const T*)pl->th_rcrT;
-  c.th_cp0 = pl->th_cp0; c.th_iden0 = pl->th_iden0; c.th_rcr = pl->th_rcr;
+  c.cl = pl->cl; c.cr = pl->cr; c.K2 = pl->K2;
+  c.th_cb = pl->th_cb;
   c.pairs = pl->pairs; c.kzy = pl->kzy;
   c.nkx = pl->nkx;
   c.jb = g->k1_pencil();

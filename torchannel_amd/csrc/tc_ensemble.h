@@ -22,9 +22,8 @@ struct EnsChan {
   FftDesc fx, fy;
   const double2 *twx, *twy;
   cplx<T>* spec;
-  const double* cl;
-  const T *th_cp, *th_last, *th_clT, *th_rcrT;
-  const double *th_cp0, *th_iden0, *th_rcr;
+  const double *cl, *cr, *K2;
+  const double* th_cb;
   int pairs, kzy, nkx;
   // launch shapes and scalars
   int jb, k5_blocks;

@@ -50,6 +50,7 @@ struct Grid {
   int k1_jb = 0;                  // j-columns per K1 pencil (0: the default below; set by the tuner)
   int k1_tile = 0;                // K1 form: 0 pencils, 1 tiled (LDS-staged neighbour columns)
   int thomas_sb = 0;              // systems per wall-normal Thomas block (0: plan default)
+  int thomas_lds = 1;             // wall-normal Thomas form: 0 direct row loads, 1 staged through LDS
 
   // K1 pencil length: the tuned value, else about ny/24 in steps of 4, within [8, 16] (measured
   // in float32 and float64: ny = 192 fastest at 8, ny = 384 at 16, by 4-6 % over 8)

@@ -23,14 +23,8 @@ struct Poisson {
   int pairs_fx = 1;            // ... of the solve's forward x transform
   bool wide_fx = false;        // that transform runs with the TC_FFT_ELEMS_WIDE budget
   int kzy = 1;                 // z-rows per y-transform block
-  void* th_cp = nullptr;       // Thomas c'_k per system, [system][k], working precision
-  void* th_last = nullptr;     // 1/pivot of each system's last row, [system], working precision
-  double* th_rcr = nullptr;    // 1/c_k (c_k = cr[k], k < nz - 1; 0 for the last row): the other
-                               // inverse pivots are 1/pivot_k = c'_k / c_k
-  void* th_clT = nullptr;      // cl and th_rcr in the working precision (the systems other than
-  void* th_rcrT = nullptr;     // the gauge-fixed one)
-  double* th_cp0 = nullptr;    // float64 rows of the gauge-fixed (0,0) system
-  double* th_iden0 = nullptr;
+  double* th_cb = nullptr;     // Thomas c' of the row before each lane's first row of the
+                               // wall-normal solve, [system][lanes] (0 for lane 0)
   double* planes = nullptr;    // statistics: (nx, ny, 6) float64
   void* plane_spec = nullptr;  // statistics: (nkx, ny, 6) complex float64
 };

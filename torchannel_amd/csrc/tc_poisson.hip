@@ -181,7 +181,10 @@ __global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDe
 
 // P5: half spectra -> real rows (irfft semantics: the imaginary parts of the kx = 0 and Nyquist
 // bins are dropped), scaled by 1/(nx*ny), written to p with the ghost images of
-// projection_fft.py:150-163 (x copies, y copies, Neumann z walls).
+// projection_fft.py:150-163 (x copies, y copies, Neumann z walls).  Two real z-rows ka, ka + 1
+// travel as one complex sequence (pair b): input x of the pair is A + iB at bin m = x for x <= nx/2
+// and its conjugate-symmetric image at bin m = nx - x above, with A, B the rows' bins; output x
+// holds row ka in its real and row ka + 1 in its imaginary part.
 template <typename T>
 __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const double2* __restrict__ twx,
                                                const cplx<T>* __restrict__ S, T* __restrict__ p,
@@ -198,6 +201,25 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
   const int lpr = 31 - __clz(pairs);   // pairs is a power of two (checked by the launchers)
   cplx<T>* twl = Z + pairs * ldb;
   fft_stage_twiddles<T>(twl, twx, nx);
+  // bins ka and ka + 1 of column (m, j), clamped into the column (the caller zeroes rows >= nz)
+  auto bins = [&](int m, int ka, cplx<T>& A, cplx<T>& B) {
+    const cplx<T>* col = S + ((size_t)m * ny + j) * nz;
+    if constexpr (sizeof(T) == 8) {
+      // the spectrum's last read of the step: non-temporal in float64 (measured: step -1 %;
+      // float32 keeps plain loads)
+      typedef T v2 __attribute__((ext_vector_type(2)));
+      const v2 a2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka, nz - 1)));
+      const v2 b2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka + 1, nz - 1)));
+      A = {a2.x, a2.y};
+      B = {b2.x, b2.y};
+    } else {
+      A = col[min(ka, nz - 1)];
+      B = col[min(ka + 1, nz - 1)];
+    }
+  };
+  const int J = j + 1;
+  const int Y1 = J == ny ? 0 : (J == 1 ? ny + 1 : -1);
+  const int sxp = d.sxp, syp = d.syp;
   // loads ahead of stores, as in fwd_x_body
   const int tot = (h + 1) * pairs;
   const int spare = pairs * ldb + nx;
@@ -206,21 +228,7 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
 #pragma unroll
     for (int q = 0; q < FFT_U; ++q) {
       const int t = min(t0 + 256 * q, tot - 1);
-      const int qq = t & (pairs - 1), m = t >> lpr;
-      const int ka = k0 + 2 * qq;
-      const cplx<T>* col = S + ((size_t)m * ny + j) * nz;
-      if constexpr (sizeof(T) == 8) {
-        // the spectrum's last read of the step: non-temporal in float64 (measured: step -1 %;
-        // float32 keeps plain loads)
-        typedef T v2 __attribute__((ext_vector_type(2)));
-        const v2 a2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka, nz - 1)));
-        const v2 b2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka + 1, nz - 1)));
-        va[q] = {a2.x, a2.y};
-        vb[q] = {b2.x, b2.y};
-      } else {
-        va[q] = col[min(ka, nz - 1)];
-        vb[q] = col[min(ka + 1, nz - 1)];
-      }
+      bins(t >> lpr, k0 + 2 * (t & (pairs - 1)), va[q], vb[q]);
     }
 #pragma unroll
     for (int q = 0; q < FFT_U; ++q) {
@@ -237,31 +245,30 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
   }
   __syncthreads();
   lds_fft<T>(Z, ldb, pairs, fx, twl, +1);
+  // 16 consecutive lanes write 16 consecutive z-rows of one x-row.  Each axis has the cell's index
+  // and at most one image (x: face nx -> 0, face 1 -> nx + 1; y likewise; z: Neumann copies
+  // 1 -> 0, nz -> nz + 1), so a value goes to at most 8 places.
   for (int t = threadIdx.x; t < nx * KZ; t += 256) {
     const int kk = t & (KZ - 1), i = t >> (lpr + 1);
     const int k = k0 + kk;
     if (k >= nz) continue;
     const cplx<T> z = Z[(kk >> 1) * ldb + fpad<T>(i)];
     const T val = ((kk & 1) ? z.im : z.re) * scale;
-    const int I = i + 1, J = j + 1, K = k + 1;
-    p[I * d.sxp + J * d.syp + K] = val;
-    if (I == 1 || I == nx || J == 1 || J == ny || K == 1 || K == nz) {
-      // every combination of the cell and its images along each axis, the cell itself excepted
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const int X = a == 0 ? I : (a == 1 ? (I == nx ? 0 : -1) : (I == 1 ? nx + 1 : -1));
-        if (X < 0) continue;
-#pragma unroll
-        for (int bq = 0; bq < 3; ++bq) {
-          const int Y = bq == 0 ? J : (bq == 1 ? (J == ny ? 0 : -1) : (J == 1 ? ny + 1 : -1));
-          if (Y < 0) continue;
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const int Zc = c == 0 ? K : (c == 1 ? (K == 1 ? 0 : -1) : (K == nz ? nz + 1 : -1));
-            if (Zc < 0 || (a | bq | c) == 0) continue;
-            p[X * d.sxp + Y * d.syp + Zc] = val;
-          }
-        }
+    const int I = i + 1, K = k + 1;
+    const int X1 = I == nx ? 0 : (I == 1 ? nx + 1 : -1);
+    const int Z1 = K == 1 ? 0 : (K == nz ? nz + 1 : -1);
+    p[I * sxp + J * syp + K] = val;
+    if (X1 >= 0) p[X1 * sxp + J * syp + K] = val;
+    if (Y1 >= 0) {
+      p[I * sxp + Y1 * syp + K] = val;
+      if (X1 >= 0) p[X1 * sxp + Y1 * syp + K] = val;
+    }
+    if (Z1 >= 0) {
+      p[I * sxp + J * syp + Z1] = val;
+      if (X1 >= 0) p[X1 * sxp + J * syp + Z1] = val;
+      if (Y1 >= 0) {
+        p[I * sxp + Y1 * syp + Z1] = val;
+        if (X1 >= 0) p[X1 * sxp + Y1 * syp + Z1] = val;
       }
     }
   }
@@ -327,10 +334,14 @@ __device__ __forceinline__ void pcr_general(U (&a)[R], U (&b)[R], U (&c)[R], U (
   }
 }
 
-// P3: one wall-normal solve per (kx, ky) system (projection_fft.py:184-215) with the Thomas
-// factorisation computed once per plan (extended precision on the host, stored [system][k]):
-// forward y_k = (d_k - a_k y_{k-1}) / pivot_k, backward x_k = y_k - c'_k x_{k+1}.  Both are
-// first-order linear recurrences, evaluated here as scans of affine maps y -> A y + B, whose
+// P3: one wall-normal solve per (kx, ky) system (projection_fft.py:184-215): forward
+// y_k = (d_k - a_k y_{k-1}) / pivot_k, backward x_k = y_k - c'_k x_{k+1}, with
+// pivot_k = b_k - a_k c'_{k-1} and c'_k = c_k / pivot_k (projection_fft.py:56-82, 184-200).
+// The factorisation is the one sequential recurrence of the solve that does not involve the
+// data: each lane recomputes it in float64 over its own R rows from the c' of the row before
+// its first row, which the plan stores per (system, lane) (factorised on the host in extended
+// precision), so the per-system table is L values instead of nz.  The two data recurrences are
+// first-order linear, evaluated here as scans of affine maps y -> A y + B, whose
 // composition is associative: a system is a group of L = 16 lanes (one DPP row; L = 32, two rows,
 // above 128 rows), lane g holds rows g*R .. g*R+R-1, composes its R maps in registers, and the L
 // lane maps are combined by a Hillis-Steele scan over row_shr (forward) / row_shl (backward)
@@ -376,16 +387,19 @@ __device__ __forceinline__ void vstore_rows(E* __restrict__ p, const E (&x)[RM])
 }
 
 // VEC: nz == L * RM (R == RM, every lane holds RM rows of the system) and every lane's first row
-// sits on a 16-byte boundary of each array, so a lane's rows move as 16-byte vectors instead of
-// RM strided element accesses.
-// The inverse pivots: 1/pivot_k = c'_k / c_k = c'_k * rcr[k] for k < nz - 1 and `last` for the
-// last row (c'_k = c_k / pivot_k, c_k > 0 above the last row); the gauge-fixed system (gauge) reads
-// its own float64 table ids instead (its pinned row 0 has c_0 = 0).
-template <typename U, typename T, typename TT, int RM, bool VEC>
-__device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT* __restrict__ cps,
-                                            const TT* __restrict__ ids, const TT* __restrict__ rcr, U last,
-                                            const TT* __restrict__ cl, int nz, int R, int L, int g,
-                                            bool gauge, bool store) {
+// sits on a 16-byte boundary of each array, so a lane's rows move as 16-byte vectors (from `lrow`,
+// the lane's first row in the spectrum or in an LDS copy of it) instead of RM strided element
+// accesses (from `col`, the system's row 0).
+// Rows (projection_fft.py:44-82, as the plan builds them on the host): a_k = cl[k] (0 for k = 0),
+// b_k = -((cl[k] + cr[k]) + K2) with the Neumann corrections +cl[0] at k = 0 and +cr[nz-1] at
+// k = nz - 1, c_k = cr[k] (0 for the last row).  The gauge-fixed (0,0) system (gauge) pins row 0
+// (b = 1, c = 0, and y_0 = 0 whatever d_0) and decouples row 1 from it (a_1 = 0).  cb: c' of the
+// row before the lane's first row (0 for lane 0).
+template <typename U, typename T, int RM, bool VEC>
+__device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, cplx<T>* __restrict__ lrow,
+                                            const double* __restrict__ cl, const double* __restrict__ cr,
+                                            double k2, double cb, int nz, int R, int L, int g, bool gauge,
+                                            bool store) {
   const int wl = lane_id();
   const int m0 = g * R;
   U A[RM], C[RM];
@@ -393,33 +407,42 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   cplx<T> dv[RM];
   // row m (mm = m clamped into the system) -> the forward map y -> A y + B and the backward
   // factor C; rows past nz are identity maps
-  auto row = [&](int r, cplx<T> d, TT cpv, TT clv, TT rcv, TT idv) {
+  double cprev = cb;
+  auto row = [&](int r, cplx<T> d, double clv, double crv) {
     const int m = m0 + r, mm = min(m, nz - 1);
-    C[r] = (U)cpv;
-    U be;
-    if (gauge) be = mm == 0 ? U(0) : (U)idv;
-    else be = mm == nz - 1 ? last : (U)cpv * (U)rcv;
-    const U a = (mm == 0 || (gauge && mm == 1)) ? U(0) : (U)clv;
+    const double a = (mm == 0 || (gauge && mm == 1)) ? 0.0 : clv;
+    double b = -((clv + crv) + k2);
+    if (mm == 0) b += clv;
+    if (mm == nz - 1) b += crv;
+    double c = mm <= nz - 2 ? crv : 0.0;
+    if (gauge && mm == 0) {
+      b = 1.0;
+      c = 0.0;
+    }
+    const double rpiv = 1.0 / (b - a * cprev);
+    const double cpk = c * rpiv;
+    cprev = cpk;
+    const double be = (gauge && mm == 0) ? 0.0 : rpiv;
     const bool in = m < nz;
-    A[r] = in ? -a * be : U(1);
-    B[r] = in ? cplx<U>{be * (U)d.re, be * (U)d.im} : cplx<U>{U(0), U(0)};
+    C[r] = (U)cpk;
+    A[r] = in ? (U)(-a * be) : U(1);
+    const U bu = (U)be;
+    B[r] = in ? cplx<U>{bu * (U)d.re, bu * (U)d.im} : cplx<U>{U(0), U(0)};
   };
   if constexpr (VEC) {
-    TT cpv[RM], idv[RM], clv[RM], rcv[RM];
-    vload_rows<cplx<T>, RM>(col + m0, dv);
-    vload_rows<TT, RM>(cps + m0, cpv);
-    vload_rows<TT, RM>(cl + m0, clv);
-    vload_rows<TT, RM>(rcr + m0, rcv);
-    if (gauge) vload_rows<TT, RM>(ids + m0, idv);
+    double clv[RM], crv[RM];
+    vload_rows<cplx<T>, RM>(lrow, dv);
+    vload_rows<double, RM>(cl + m0, clv);
+    vload_rows<double, RM>(cr + m0, crv);
 #pragma unroll
-    for (int r = 0; r < RM; ++r) row(r, dv[r], cpv[r], clv[r], rcv[r], gauge ? idv[r] : TT(0));
+    for (int r = 0; r < RM; ++r) row(r, dv[r], clv[r], crv[r]);
   } else {
     // every load is issued up front and unconditionally (rows past nz read row nz - 1)
 #pragma unroll
     for (int r = 0; r < RM; ++r) {
       if (r < R) {
         const int mm = min(m0 + r, nz - 1);
-        row(r, col[mm], cps[mm], cl[mm], rcr[mm], gauge ? ids[mm] : TT(0));
+        row(r, col[mm], cl[mm], cr[mm]);
       }
     }
   }
@@ -496,7 +519,7 @@ __device__ __forceinline__ void thomas_rows(cplx<T>* __restrict__ col, const TT*
   if constexpr (VEC) {
 #pragma unroll
     for (int r = 0; r < RM; ++r) dv[r] = {(T)(A[r] * xin_r + B[r].re), (T)(A[r] * xin_i + B[r].im)};
-    vstore_rows<cplx<T>, RM>(col + m0, dv);
+    vstore_rows<cplx<T>, RM>(lrow, dv);
   } else {
 #pragma unroll
     for (int r = 0; r < RM; ++r) {
@@ -516,41 +539,87 @@ __host__ __device__ __forceinline__ int thomas_rows_per_lane(int nz) {
   return r <= 1 ? 1 : r <= 2 ? 2 : r <= 4 ? 4 : 8;
 }
 
-// Block = sb systems of L lanes each; R <= RM rows per lane.
-// cl, rcr: float64 (the gauge-fixed system); clT, rcrT: the same in the working precision
-template <typename T, int RM, bool VEC = false>
+// LDS segment of one lane in the staged form: its RM rows plus 16 bytes, so that the 16-byte row
+// reads and writes of 8 consecutive lanes fall in distinct banks.
+template <typename T, int RM>
+__host__ __device__ constexpr int thomas_seg() { return RM * (int)sizeof(cplx<T>) + 16; }
+
+// Block = sb systems of L lanes each; R <= RM rows per lane.  cl, cr: the rows' couplings
+// (float64, [nz]); K2: kx~^2 + ky~^2 per system; cb: [system][L] boundary c'.
+// STG (with VEC): the block's sb systems, one contiguous stretch of the spectrum, are copied
+// into LDS and back with 16-byte accesses of consecutive lanes, so that each wavefront-wide load
+// or store covers 1 KiB of consecutive memory; the lanes then read their rows from LDS.
+template <typename T, int RM, bool VEC = false, bool STG = false>
 __device__ __forceinline__ void thomas_body(int nsys, int nz, int R, int L, int sb, const double* __restrict__ cl,
-                                            const double* __restrict__ rcr, const T* __restrict__ clT,
-                                            const T* __restrict__ rcrT, const T* __restrict__ cp,
-                                            const T* __restrict__ last, const double* __restrict__ cp0,
-                                            const double* __restrict__ iden0, cplx<T>* __restrict__ S, int blk) {
+                                            const double* __restrict__ cr, const double* __restrict__ K2,
+                                            const double* __restrict__ cb, cplx<T>* __restrict__ S, int blk) {
+  static_assert(!STG || VEC, "the staged form moves whole 16-byte row vectors");
   const int lane = threadIdx.x & (L - 1);
   const int s = blk * sb + (int)(threadIdx.x / L);
   const bool live = s < nsys;
   const int sc = live ? s : nsys - 1;
   cplx<T>* col = S + (size_t)sc * nz;
-  const size_t to = (size_t)sc * nz;
-  const T lv = last[sc];
-  if constexpr (sizeof(T) == sizeof(double)) {
-    const bool gauge = sc == 0;
-    thomas_rows<double, T, double, RM, VEC>(col, gauge ? cp0 : (const double*)cp + to, iden0, rcr, (double)lv, cl,
-                                            nz, R, L, lane, gauge, live);
+  const double k2 = K2[sc];
+  const double c0 = cb[(size_t)sc * L + lane];
+  const bool gauge = sc == 0;
+  if constexpr (STG) {
+    constexpr int E = (int)sizeof(cplx<T>);
+    constexpr int PER = 16 / E;   // rows per 16-byte unit
+    constexpr int SEG = thomas_seg<T, RM>();
+    constexpr int UPT = RM / PER;   // units per thread: the block's sb * L * RM rows / (sb * L) threads
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    unsigned char* stg = smem_raw;   // sb * L * SEG bytes
+    const int nth = sb * L;
+    const int nlive = min(sb, nsys - blk * sb);
+    const int nunits = nlive * nz / PER;
+    const int lnz = 31 - __clz(nz);   // nz = L * RM, a power of two
+    constexpr int lrm = RM == 1 ? 0 : RM == 2 ? 1 : RM == 4 ? 2 : 3;
+    uint4* gsrc = reinterpret_cast<uint4*>(S + (size_t)blk * sb * nz);
+    // unit u holds rows e = u*PER .. of system e >> lnz, row k = e & (nz - 1): lane k >> lrm of
+    // that system, position k & (RM - 1) in its segment
+    auto lds_unit = [&](int u) -> uint4* {
+      const int e = u * PER;
+      const int sl = e >> lnz, k = e & (nz - 1);
+      return reinterpret_cast<uint4*>(stg + (sl * L + (k >> lrm)) * SEG + (k & (RM - 1)) * E);
+    };
+    uint4 w[UPT];
+#pragma unroll
+    for (int q = 0; q < UPT; ++q) w[q] = gsrc[min((int)threadIdx.x + q * nth, nunits - 1)];
+#pragma unroll
+    for (int q = 0; q < UPT; ++q) {
+      const int u = threadIdx.x + q * nth;
+      if (u < nunits) *lds_unit(u) = w[q];
+    }
+    __syncthreads();
+    cplx<T>* lrow = reinterpret_cast<cplx<T>*>(stg + threadIdx.x * SEG);
+    if (sizeof(T) == sizeof(double) || !gauge)
+      thomas_rows<T, T, RM, true>(col, lrow, cl, cr, k2, c0, nz, R, L, lane, gauge, live);
+    else
+      thomas_rows<double, T, RM, true>(col, lrow, cl, cr, k2, c0, nz, R, L, lane, true, live);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < UPT; ++q) {
+      const int u = threadIdx.x + q * nth;
+      if (u < nunits) gsrc[u] = *lds_unit(u);
+    }
   } else {
-    if (sc == 0) thomas_rows<double, T, double, RM, VEC>(col, cp0, iden0, rcr, 0.0, cl, nz, R, L, lane, true, live);
-    else thomas_rows<T, T, T, RM, VEC>(col, cp + to, nullptr, rcrT, lv, clT, nz, R, L, lane, false, live);
+    cplx<T>* lrow = col + lane * R;
+    // the gauge-fixed system runs in float64 in both precisions
+    if (sizeof(T) == sizeof(double) || !gauge)
+      thomas_rows<T, T, RM, VEC>(col, lrow, cl, cr, k2, c0, nz, R, L, lane, gauge, live);
+    else
+      thomas_rows<double, T, RM, VEC>(col, lrow, cl, cr, k2, c0, nz, R, L, lane, true, live);
   }
 }
 
-template <typename T, int RM, bool VEC>
+template <typename T, int RM, bool VEC, bool STG>
 __global__ __launch_bounds__(256) void k_poisson_thomas(int nsys, int nz, int sb, const double* __restrict__ cl,
-                                                        const double* __restrict__ rcr, const T* __restrict__ clT,
-                                                        const T* __restrict__ rcrT, const T* __restrict__ cp,
-                                                        const T* __restrict__ last,
-                                                        const double* __restrict__ cp0,
-                                                        const double* __restrict__ iden0,
+                                                        const double* __restrict__ cr,
+                                                        const double* __restrict__ K2,
+                                                        const double* __restrict__ cb,
                                                         cplx<T>* __restrict__ S) {
-  thomas_body<T, RM, VEC>(nsys, nz, RM, thomas_lanes(nz), sb, cl, rcr, clT, rcrT, cp, last, cp0, iden0, S,
-                          xcd_remap(blockIdx.x, gridDim.x));
+  thomas_body<T, RM, VEC, STG>(nsys, nz, RM, thomas_lanes(nz), sb, cl, cr, K2, cb, S,
+                               xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // Ensemble forms of P1-P5: global block -> (channel, block of that channel) via prefix sums.
@@ -580,8 +649,8 @@ __global__ __launch_bounds__(256) void k_poisson_thomas_ens(const EnsChan<T>* __
   const int q = ens_chan(start, n, gb);
   const EnsChan<T>& c = ch[q];   // fields are passed to the body by value
   const int L = thomas_lanes(c.d.nz);
-  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), L, TH_THREADS / L, c.cl, c.th_rcr,
-                     c.th_clT, c.th_rcrT, c.th_cp, c.th_last, c.th_cp0, c.th_iden0, c.spec, gb - start[q]);
+  thomas_body<T, RM>(c.nkx * c.d.ny, c.d.nz, thomas_rows_per_lane(c.d.nz), L, TH_THREADS / L, c.cl, c.cr, c.K2,
+                     c.th_cb, c.spec, gb - start[q]);
 }
 
 template <typename T>
@@ -827,16 +896,17 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   hipMemcpy(pl->cr, cr.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   hipMemcpy(pl->K2, k2.data(), sizeof(double) * k2.size(), hipMemcpyHostToDevice);
   // Thomas factorisation of every (kx, ky) system of projection_fft.py:56-82 (rows built in
-  // float64 exactly as the reference builds tri_a/b/c, factorised in long double), [system][k]
+  // float64 exactly as the reference builds tri_a/b/c, and as k_poisson_thomas rebuilds them,
+  // factorised in long double); kept: c' of the row before each lane's first row, [system][L]
   const int nsys = nkx * ny;
-  std::vector<double> cp64((size_t)nz * nsys), last64(nsys);
-  std::vector<double> cp0(nz), id0(nz), rcr(nz);
-  for (int k = 0; k < nz; ++k) rcr[k] = k <= nz - 2 ? 1.0 / cr[k] : 0.0;
+  const int TL = thomas_lanes(nz), TR = thomas_rows_per_lane(nz);
+  std::vector<double> cb((size_t)nsys * TL, 0.0);
   for (int sy = 0; sy < nsys; ++sy) {
     const bool gauge = sy == 0;
     const double K2 = k2[sy];
     long double cprev = 0.0L;
     for (int k = 0; k < nz; ++k) {
+      if (k % TR == 0) cb[(size_t)sy * TL + k / TR] = (double)cprev;
       const double a = (k >= 1 && !(gauge && k == 1)) ? cl[k] : 0.0;
       double b = -((cl[k] + cr[k]) + K2);
       if (k == 0) b += cl[0];
@@ -844,38 +914,12 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
       double c = k <= nz - 2 ? cr[k] : 0.0;
       if (gauge && k == 0) { b = 1.0; c = 0.0; }
       const long double den = (long double)b - (long double)a * cprev;
-      const long double cpk = (long double)c / den;
-      cp64[(size_t)sy * nz + k] = (double)cpk;
-      if (k == nz - 1) last64[sy] = (double)(1.0L / den);
-      if (gauge) { cp0[k] = (double)cpk; id0[k] = (double)(1.0L / den); }
-      cprev = cpk;
+      cprev = (long double)c / den;
     }
   }
+  if (hipMalloc(&pl->th_cb, sizeof(double) * cb.size()) != hipSuccess) return TC_ERR_NOMEM;
+  hipMemcpy(pl->th_cb, cb.data(), sizeof(double) * cb.size(), hipMemcpyHostToDevice);
   const size_t esz = g->dtype == TC_F64 ? 16 : 8;
-  const size_t rsz = esz / 2;
-  if (hipMalloc(&pl->th_cp, rsz * cp64.size()) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_last, rsz * last64.size()) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_rcr, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_clT, rsz * nz) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_rcrT, rsz * nz) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_cp0, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
-  if (hipMalloc(&pl->th_iden0, sizeof(double) * nz) != hipSuccess) return TC_ERR_NOMEM;
-  if (g->dtype == TC_F64) {
-    hipMemcpy(pl->th_cp, cp64.data(), 8 * cp64.size(), hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_last, last64.data(), 8 * last64.size(), hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_clT, cl.data(), 8 * nz, hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_rcrT, rcr.data(), 8 * nz, hipMemcpyHostToDevice);
-  } else {
-    std::vector<float> f1(cp64.begin(), cp64.end()), f2(last64.begin(), last64.end());
-    hipMemcpy(pl->th_cp, f1.data(), 4 * f1.size(), hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_last, f2.data(), 4 * f2.size(), hipMemcpyHostToDevice);
-    std::vector<float> f3(cl.begin(), cl.end()), f4(rcr.begin(), rcr.end());
-    hipMemcpy(pl->th_clT, f3.data(), 4 * nz, hipMemcpyHostToDevice);
-    hipMemcpy(pl->th_rcrT, f4.data(), 4 * nz, hipMemcpyHostToDevice);
-  }
-  hipMemcpy(pl->th_cp0, cp0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
-  hipMemcpy(pl->th_iden0, id0.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
-  hipMemcpy(pl->th_rcr, rcr.data(), sizeof(double) * nz, hipMemcpyHostToDevice);
   if (hipMalloc(&pl->spec, esz * (size_t)nkx * ny * nz) != hipSuccess) return TC_ERR_NOMEM;
   // statistics workspace: 6 planes (u, v, w at the two sampled heights), float64
   if (hipMalloc(&pl->planes, sizeof(double) * (size_t)nx * ny * 6) != hipSuccess) return TC_ERR_NOMEM;
@@ -886,8 +930,7 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
 void poisson_free(Poisson* pl) {
   hipFree(pl->twx); hipFree(pl->twy); hipFree(pl->cl); hipFree(pl->cr); hipFree(pl->K2);
   hipFree(pl->spec); hipFree(pl->planes); hipFree(pl->plane_spec);
-  hipFree(pl->th_cp); hipFree(pl->th_last); hipFree(pl->th_rcr); hipFree(pl->th_clT); hipFree(pl->th_rcrT);
-  hipFree(pl->th_cp0); hipFree(pl->th_iden0);
+  hipFree(pl->th_cb);
 }
 
 // pairs * nx must fit the budget the kernel is built for (TC_FFT_ELEMS, or TC_FFT_ELEMS_WIDE
@@ -952,24 +995,32 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
       // 16-byte row vectors when every lane holds rm whole rows that start on 16-byte boundaries
       // (all arrays are [system][nz] with 256-byte aligned bases)
       const bool vec = d.nz == L * rm && (rm * (int)sizeof(T)) % 16 == 0;
+      // staged through LDS (tuning knob) where the rows move as vectors
+      const bool stg = vec && pl->g->thomas_lds == 1;
       switch (rm) {
-#define TC_TH(RM, V)                                                                                   \
-  hipLaunchKernelGGL((k_poisson_thomas<T, RM, V>), dim3(blocks), dim3(sb * L), 0, st, nsys, d.nz, sb,  \
-                     pl->cl, pl->th_rcr, (const T*)pl->th_clT, (const T*)pl->th_rcrT,                 \
-                     (const T*)pl->th_cp, (const T*)pl->th_last, pl->th_cp0, pl->th_iden0,            \
-                     (cplx<T>*)pl->spec)
-        case 1: TC_TH(1, false); break;
+#define TC_TH(RM, V, S)                                                                                 \
+  hipLaunchKernelGGL((k_poisson_thomas<T, RM, V, S>), dim3(blocks), dim3(sb * L),                      \
+                     (S ? (size_t)sb * L * thomas_seg<T, RM>() : 0), st, nsys, d.nz, sb, pl->cl, pl->cr, \
+                     pl->K2, pl->th_cb, (cplx<T>*)pl->spec)
+        case 1: TC_TH(1, false, false); break;
         case 2:
-          if (vec) TC_TH(2, sizeof(T) == 8);
-          else TC_TH(2, false);
+          if constexpr (sizeof(T) == 8) {
+            if (stg) TC_TH(2, true, true);
+            else if (vec) TC_TH(2, true, false);
+            else TC_TH(2, false, false);
+          } else {
+            TC_TH(2, false, false);
+          }
           break;
         case 4:
-          if (vec) TC_TH(4, true);
-          else TC_TH(4, false);
+          if (stg) TC_TH(4, true, true);
+          else if (vec) TC_TH(4, true, false);
+          else TC_TH(4, false, false);
           break;
         case 8:
-          if (vec) TC_TH(8, true);
-          else TC_TH(8, false);
+          if (stg) TC_TH(8, true, true);
+          else if (vec) TC_TH(8, true, false);
+          else TC_TH(8, false, false);
           break;
 #undef TC_TH
         default: return TC_ERR_UNSUPPORTED;
