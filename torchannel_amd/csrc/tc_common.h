@@ -250,7 +250,7 @@ __device__ __forceinline__ void pcr_shared_solve_lds(T (&dv)[R], T* buf, const d
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int m = r * TC_WAVE + lane;
-    dv[r] = dv[r] / (T)tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m];
+    dv[r] = dv[r] * (T)tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m];   // reciprocal diagonal
   }
 }
 
@@ -313,9 +313,10 @@ __device__ __forceinline__ void cn_rhs_face(T (&x)[R], T* rows, const Metric<T>&
 
 // Parallel cyclic reduction of one column whose matrix is shared by every column (the CN
 // systems), so the whole a/b/c reduction is a precomputed table: per stage st, alpha[m] at
-// tab[st*2*MAXNZ + m], gamma[m] at tab[(st*2+1)*MAXNZ + m]; final diagonal b at
-// tab[2*STAGES*MAXNZ + m].  Rows >= n carry alpha = gamma = 0, b = 1, d = 0.
-// Recurrence of tridiag.py:50-58 restricted to d: d += alpha*d[m-s] + gamma*d[m+s]; x = d/b.
+// tab[st*2*MAXNZ + m], gamma[m] at tab[(st*2+1)*MAXNZ + m]; the reciprocal of the
+// final diagonal b at tab[2*STAGES*MAXNZ + m].  Rows >= n carry alpha = gamma = 0, b = 1, d = 0.
+// Recurrence of tridiag.py:50-58 restricted to d: d += alpha*d[m-s] + gamma*d[m+s]; x = d/b, as
+// d times the table's reciprocal of b.
 template <int ST, int R, typename T>
 __device__ __forceinline__ void pcr_shared_stage(T (&dv)[R], const double* __restrict__ tab, int n, int lane) {
   constexpr int s = 1 << ST;
@@ -345,7 +346,7 @@ __device__ __forceinline__ void pcr_shared_solve(T (&dv)[R], const double* __res
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int m = r * TC_WAVE + lane;
-    dv[r] = dv[r] / (T)tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m];
+    dv[r] = dv[r] * (T)tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m];   // reciprocal diagonal
   }
 }
 

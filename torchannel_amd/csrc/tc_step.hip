@@ -13,7 +13,9 @@ namespace tc {
 // ---------------------------------------------------------------------------------------------
 // Crank-Nicolson coefficient tables (operators.py:581-620 for u/v, 754-781 for w), reduced by
 // the PCR recurrence of tridiag.py:40-56 once per dt.  Block 0: cell rows (u, v; n = nz);
-// block 1: face rows (w; n = nz-1).  One thread per row, neighbours through LDS.
+// block 1: face rows (w; n = nz-1).  One thread per row, neighbours through LDS.  The last table row
+// holds the reciprocal of each row's final diagonal, so every solve ends with a multiplication
+// (tridiag.py:58 divides; the two differ by at most one rounding).
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ void cn_tables_body(const double* __restrict__ dzf, const double* __restrict__ dzc,
                                                int nz, int top, double nu, double theta,
@@ -64,7 +66,7 @@ __device__ __forceinline__ void cn_tables_body(const double* __restrict__ dzf, c
     tab[(2 * st) * TC_MAX_NZ + m] = al;
     tab[(2 * st + 1) * TC_MAX_NZ + m] = ga;
   }
-  tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m] = m < n ? b : 1.0;
+  tab[2 * TC_MAX_STAGES * TC_MAX_NZ + m] = m < n ? 1.0 / b : 1.0;
   if (dt_set > 0.0 && !face && m == 0) scal[TC_S_DT] = dt_set;
 }
 
@@ -228,7 +230,7 @@ __device__ __forceinline__ void pcr_solve_tab(T (&dv)[R], T* buf, const T* tabl,
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int m = r * TC_WAVE + lane;
-    dv[r] = dv[r] / tabl[2 * TC_MAX_STAGES * TC_WAVE * R + m];
+    dv[r] = dv[r] * tabl[2 * TC_MAX_STAGES * TC_WAVE * R + m];
   }
 }
 
@@ -692,7 +694,10 @@ __device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0,
 __host__ __device__ __forceinline__ int k1_bands_i(int nx) { return (nx + 2 + TC_K1_WAVES - 1) / TC_K1_WAVES; }
 __host__ __device__ __forceinline__ int k1_bands_j(int ny, int jb) { return (ny + 2 + jb - 1) / jb; }
 
-template <typename T, int R>
+// FULL: the launcher has checked nz == 64*R, so every lane's rows lie inside the u and v columns
+// (and all but the last lane's last row inside the w column); with nz a compile-time constant the
+// bounds guards of the column loads and stores fold away.
+template <typename T, int R, bool FULL = false>
 __device__ __forceinline__ void explicit_cn_body(
     Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
@@ -702,6 +707,7 @@ __device__ __forceinline__ void explicit_cn_body(
     int b) {
   __shared__ T lds_tab[(2 * TC_MAX_STAGES + 1) * TC_WAVE * R];
   __shared__ T lds_rows[TC_K1_WAVES][3 * TC_WAVE * R];
+  if constexpr (FULL) d.nz = TC_WAVE * R;
   const int comp = b % 3, band = b / 3;
   const int nbj = k1_bands_j(d.ny, jb);
   const int bi = band / nbj, bj = band - bi * nbj;
@@ -740,15 +746,15 @@ __device__ __forceinline__ void explicit_cn_body(
   else k1_pencil_w<T, R>(c, i, j0, j1, u, v, w, pw, rw, sw);
 }
 
-template <typename T, int R>
+template <typename T, int R, bool FULL>
 __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
     Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
     T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
     const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
-  explicit_cn_body<T, R>(d, mt, u, v, w, pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body, mode,
-                         branch, theta, jb, xcd_remap(blockIdx.x, gridDim.x));
+  explicit_cn_body<T, R, FULL>(d, mt, u, v, w, pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body,
+                               mode, branch, theta, jb, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1403,16 +1409,17 @@ static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void*
     if (rc != TC_ERR_UNSUPPORTED) return rc;      // else the block's LDS does not fit: pencils
   }
   const int blocks = 3 * k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
-#define TC_K1(RR)                                                                                 \
-  hipLaunchKernelGGL((k_explicit_cn<T, RR>), dim3(blocks), dim3(64 * TC_K1_WAVES), 0, st, d, mt,  \
-                     (const T*)u, (const T*)v, (const T*)w, (const T*)pu, (const T*)pv,            \
-                     (const T*)pw, (T*)ru, (T*)rv, (T*)rw, (T*)su, (T*)sv, (T*)sw, tab_cell,       \
+#define TC_K1(RR, F)                                                                                \
+  hipLaunchKernelGGL((k_explicit_cn<T, RR, F>), dim3(blocks), dim3(64 * TC_K1_WAVES), 0, st, d, mt,  \
+                     (const T*)u, (const T*)v, (const T*)w, (const T*)pu, (const T*)pv,               \
+                     (const T*)pw, (T*)ru, (T*)rv, (T*)rw, (T*)su, (T*)sv, (T*)sw, tab_cell,          \
                      tab_face, scal, (const T*)body, mode, branch, theta, jb)
+  const bool full = d.nz == TC_WAVE * R;
   switch (R) {
-    case 1: TC_K1(1); break;
-    case 2: TC_K1(2); break;
-    case 3: TC_K1(3); break;
-    case 4: TC_K1(4); break;
+    case 1: if (full) TC_K1(1, true); else TC_K1(1, false); break;
+    case 2: if (full) TC_K1(2, true); else TC_K1(2, false); break;
+    case 3: if (full) TC_K1(3, true); else TC_K1(3, false); break;
+    case 4: if (full) TC_K1(4, true); else TC_K1(4, false); break;
     default: return TC_ERR_UNSUPPORTED;
   }
 #undef TC_K1
