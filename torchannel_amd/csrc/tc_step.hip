@@ -1059,7 +1059,8 @@ __device__ __forceinline__ void kt_march(const K1Ctx<T, R>& c, int i, int il, in
   }
 }
 
-template <typename T, int R>
+// FULL: nz == 64*R, checked by the launcher (as for k_explicit_cn).
+template <typename T, int R, bool FULL>
 __global__ __launch_bounds__(64 * TC_KT_WAVES) void k_explicit_cn_tile(
     Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
@@ -1067,6 +1068,7 @@ __global__ __launch_bounds__(64 * TC_KT_WAVES) void k_explicit_cn_tile(
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
     const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
   extern __shared__ __align__(16) unsigned char kt_lds[];
+  if constexpr (FULL) d.nz = TC_WAVE * R;
   T* ring = reinterpret_cast<T*>(kt_lds);
   T* tabs = ring + KT<T, R>::NPL * KT<T, R>::PLANE;          // cell table, then face table
   T* rows = tabs + 2 * KT<T, R>::TAB;
@@ -1375,8 +1377,14 @@ static int launch_tile(int blocks, hipStream_t st, Dims d, Metric<T> mt, const T
   }
   const size_t bytes = KT<T, R>::BYTES;
   if ((long long)bytes > (long long)limit) return TC_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL((k_explicit_cn_tile<T, R>), dim3(blocks), dim3(64 * TC_KT_WAVES), bytes, st, d, mt, u, v, w,
-                     pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body, mode, branch, theta, jb);
+  if (d.nz == TC_WAVE * R)
+    hipLaunchKernelGGL((k_explicit_cn_tile<T, R, true>), dim3(blocks), dim3(64 * TC_KT_WAVES), bytes, st, d, mt, u,
+                       v, w, pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body, mode, branch, theta,
+                       jb);
+  else
+    hipLaunchKernelGGL((k_explicit_cn_tile<T, R, false>), dim3(blocks), dim3(64 * TC_KT_WAVES), bytes, st, d, mt, u,
+                       v, w, pu, pv, pw, ru, rv, rw, su, sv, sw, tab_cell, tab_face, scal, body, mode, branch, theta,
+                       jb);
   TC_CHECK_LAUNCH();
   return TC_OK;
 }
