@@ -47,7 +47,8 @@ def test_knobs_do_not_change_results(tc, precision, shape):
     ref = run(tc, {}, precision, shape=shape)
     for knobs in ({L.TUNE_K1_JB: 1}, {L.TUNE_K1_JB: 3}, {L.TUNE_K1_JB: 26},
                   {L.TUNE_THOMAS_SB: 4}, {L.TUNE_THOMAS_SB: 8}, {L.TUNE_K1_JB: 5, L.TUNE_THOMAS_SB: 8},
-                  {L.TUNE_K1_TILE: 1}, {L.TUNE_K1_TILE: 1, L.TUNE_K1_JB: 3},
+                  {L.TUNE_K1_TILE: 0}, {L.TUNE_K1_TILE: 1}, {L.TUNE_K1_TILE: 1, L.TUNE_K1_JB: 3},
+                  {L.TUNE_K1_TILE: 2}, {L.TUNE_K1_TILE: 2, L.TUNE_K1_JB: 3},
                   {L.TUNE_THOMAS_LDS: 1}, {L.TUNE_THOMAS_LDS: 1, L.TUNE_THOMAS_SB: 4},
                   {L.TUNE_THOMAS_LDS: 1, L.TUNE_THOMAS_SB: 8}):
         print(f"[tuning] {precision} {shape} {knobs}", flush=True)
@@ -81,6 +82,8 @@ def test_bad_knob_rejected(tc):
     with pytest.raises(RuntimeError):
         ch._grid.set_tuning(L.TUNE_THOMAS_LDS, 2)
     with pytest.raises(RuntimeError):
+        ch._grid.set_tuning(L.TUNE_K1_TILE, 3)
+    with pytest.raises(RuntimeError):
         ch._grid.set_tuning(99, 1)
 
 
@@ -92,3 +95,18 @@ def test_default_pencil_length_follows_ny(tc):
     for ny, jb in ((12, 8), (192, 8), (288, 12), (384, 16), (768, 16)):
         g = L.get_grid(4, ny, 8, 0.1, 0.1, 1e-3, L.TC_TOP_DIRICHLET, torch.float64, "cuda", dz_f=dzf, dz_c=dzc)
         assert g.get_tuning(L.TUNE_K1_JB) == jb
+
+
+def test_default_k1_form(tc):
+    """Untuned K1 form: the three-component pencils where a lane holds two or more rows and the
+    form is built for the row count (float64 up to 128 rows, float32 up to 256), with their own
+    default pencil length; elsewhere the per-component pencils."""
+    from torchannel_amd import _lib as L
+    for dt, nz, ny, form, jb in ((torch.float64, 48, 192, 0, 8), (torch.float64, 128, 192, 2, 8),
+                                 (torch.float64, 256, 384, 0, 16), (torch.float32, 128, 192, 2, 4),
+                                 (torch.float32, 256, 384, 2, 6), (torch.float32, 64, 192, 0, 8)):
+        dzf = torch.full((nz,), 2.0 / nz, dtype=torch.float64, device="cuda")
+        dzc = torch.full((nz + 1,), 2.0 / nz, dtype=torch.float64, device="cuda")
+        g = L.get_grid(4, ny, nz, 0.1, 0.1, 1e-3, L.TC_TOP_DIRICHLET, dt, "cuda", dz_f=dzf, dz_c=dzc)
+        assert g.get_tuning(L.TUNE_K1_TILE) == form, (dt, nz)
+        assert g.get_tuning(L.TUNE_K1_JB) == jb, (dt, nz)

@@ -5,8 +5,9 @@ Each knob changes only how work is dealt to workgroups, never the arithmetic, so
 gives bit-identical results; the tuner times the kernel a knob drives with HIP events on the
 solver's stream and keeps the fastest setting on the grid.  Knobs:
 
-* ``TUNE_K1_TILE``   form of the explicit/AB2/CN kernel: per-component pencils, or the tiled form
-  with the neighbour columns staged through LDS (searched jointly with the next knob)
+* ``TUNE_K1_TILE``   form of the explicit/AB2/CN kernel: per-component pencils, the tiled form
+  with the neighbour columns staged through LDS, or three-component pencils (one wavefront marching
+  u, v and w of one i; searched jointly with the next knob)
 * ``TUNE_K1_JB``     j-columns per explicit/AB2/CN pencil or tile
 * ``TUNE_THOMAS_SB`` (kx, ky) systems per block of the wall-normal Poisson solve
 * ``TUNE_THOMAS_LDS`` that solve's rows copied through LDS with whole-block 16-byte accesses, or
@@ -37,7 +38,7 @@ def _key(ch):
 def _candidates(ch):
     """Groups of settings searched together: (list of {knob: value}, phase the group drives)."""
     jb = sorted({j for j in (2, 4, 6, 8, 12, 16, 24, 32) if j <= ch.ny + 2})
-    k1 = [{L.TUNE_K1_TILE: t, L.TUNE_K1_JB: j} for t in (0, 1) for j in jb]
+    k1 = [{L.TUNE_K1_TILE: t, L.TUNE_K1_JB: j} for t in (2, 0, 1) for j in jb]
     sb = [{L.TUNE_THOMAS_LDS: s, L.TUNE_THOMAS_SB: v} for s in (1, 0) for v in (4, 8, 16)]
     return [(k1, _PHASE_K1), (sb, _PHASE_THOMAS)]
 

@@ -167,7 +167,7 @@ int tc_grid_set_tuning(tc_grid* h, int key, int value) {
       g.thomas_sb = value;
       return TC_OK;
     case TC_TUNE_K1_TILE:
-      if (value != 0 && value != 1) return TC_ERR_ARG;
+      if (value < 0 || value > 2) return TC_ERR_ARG;
       g.k1_tile = value;
       return TC_OK;
     case TC_TUNE_THOMAS_LDS:
@@ -190,7 +190,7 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
       *value = g.thomas_sb > 0 ? g.thomas_sb : 16;
       return TC_OK;
     case TC_TUNE_K1_TILE:
-      *value = g.k1_tile;
+      *value = g.k1_form();
       return TC_OK;
     case TC_TUNE_THOMAS_LDS:
       *value = g.thomas_lds;
@@ -755,7 +755,7 @@ static void fill_chan(EnsChan<T>& c, tc_solver* s) {
   c.th_cb = pl->th_cb;
   c.pairs = pl->pairs; c.kzy = pl->kzy;
   c.nkx = pl->nkx;
-  c.jb = g->k1_pencil();
+  c.jb = g->k1_pencil_of(0);   // the ensemble runs the per-component pencils
   c.k5_blocks = k5_blocks(g);
   c.inv_volume = s->inv_volume;
   c.nu = g->nu;

@@ -48,14 +48,32 @@ struct Grid {
   double* tabs = nullptr;         // CN tables: cell, face
   double* scal = nullptr;         // scalar block for stand-alone ops
   int k1_jb = 0;                  // j-columns per K1 pencil (0: the default below; set by the tuner)
-  int k1_tile = 0;                // K1 form: 0 pencils, 1 tiled (LDS-staged neighbour columns)
+  int k1_tile = -1;               // K1 form: 0 pencils, 1 tiled (LDS-staged neighbour columns),
+                                  // 2 three-component pencils, -1 the default below
   int thomas_sb = 0;              // systems per wall-normal Thomas block (0: plan default)
   int thomas_lds = 1;             // wall-normal Thomas form: 0 direct row loads, 1 staged through LDS
 
-  // K1 pencil length: the tuned value, else about ny/24 in steps of 4, within [8, 16] (measured
-  // in float32 and float64: ny = 192 fastest at 8, ny = 384 at 16, by 4-6 % over 8)
-  bool k1_tiled() const { return k1_tile == 1; }
-  int k1_pencil() const { return k1_jb > 0 ? k1_jb : std::min(16, std::max(8, 4 * ((d.ny + 48) / 96))); }
+  // K1 form: the tuned value, else the three-component pencils where they are built for the row
+  // count (float64 up to 128 rows, float32 up to 256) and a lane holds two or more rows (measured
+  // 6-8 % faster than the per-component pencils at 192x192x128 in both precisions and
+  // 384x384x256 in float32), else the per-component pencils.
+  int k1_form() const {
+    if (k1_tile >= 0) return k1_tile;
+    const int rows = (d.nz + 63) / 64;
+    return rows >= 2 && rows <= (dtype == TC_F64 ? 2 : 4) ? 2 : 0;
+  }
+  bool k1_tiled() const { return k1_form() == 1; }
+  // K1 pencil length of a form: the tuned value, else for the per-component pencils about ny/24
+  // in steps of 4, within [8, 16] (ny = 192 fastest at 8, ny = 384 at 16, by 4-6 % over 8), and
+  // for the three-component pencils 8 in float64, in float32 4 below ny = 256 and 6 from there
+  // (192x192x128: 8 beats 4 by 2 % in float64, 4 beats 8 by 4 % in float32; 384x384x256 float32:
+  // 6 beats 4 and 16 by 3-4 %).
+  int k1_pencil_of(int form) const {
+    if (k1_jb > 0) return k1_jb;
+    if (form == 2) return dtype == TC_F64 ? 8 : (d.ny >= 256 ? 6 : 4);
+    return std::min(16, std::max(8, 4 * ((d.ny + 48) / 96)));
+  }
+  int k1_pencil() const { return k1_pencil_of(k1_form()); }
 
   template <typename T>
   Metric<T> metric() const;

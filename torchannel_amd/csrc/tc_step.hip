@@ -189,11 +189,25 @@ __device__ __forceinline__ void rows_up(T* buf, const T (&x)[R], T (&up)[R], T h
     if (r * TC_WAVE + lane == n - 1) up[r] = hi;
 }
 
-// One PCR stage with the block's LDS copy of the shared reduction table (layout of k_cn_tables
-// with row stride 64*R).
+// PCR stages a column of at most 64*R rows can need (stride 1 << ST below the row count), and
+// the block's LDS copy of the shared reduction table: k_cn_tables' rows 0 .. 2*NST-1 (alpha,
+// gamma per stage) and its reciprocal diagonal as row 2*NST, row stride 64*R.
+template <int R>
+__host__ __device__ constexpr int k1_nst() { return R == 1 ? 6 : R == 2 ? 7 : 8; }
+template <int R>
+__host__ __device__ constexpr int k1_tab_len() { return (2 * k1_nst<R>() + 1) * TC_WAVE * R; }
+// Table entry e of the LDS layout, read from k_cn_tables' layout.
+template <int R>
+__device__ __forceinline__ double k1_tab_src(const double* __restrict__ tab, int e) {
+  const int row = e / (TC_WAVE * R), m = e - row * (TC_WAVE * R);
+  return tab[(row < 2 * k1_nst<R>() ? row : 2 * TC_MAX_STAGES) * TC_MAX_NZ + m];
+}
+
+// One PCR stage with the block's LDS copy of the shared reduction table.
 template <int ST, int R, typename T>
 __device__ __forceinline__ void pcr_stage_tab(T (&dv)[R], T* buf, const T* tabl, int n, int lane) {
   constexpr int s = 1 << ST;
+  if constexpr (ST >= k1_nst<R>()) return;
   if (s >= n) return;
   T dn[R], up[R];
   if constexpr (s >= TC_WAVE) {
@@ -230,7 +244,7 @@ __device__ __forceinline__ void pcr_solve_tab(T (&dv)[R], T* buf, const T* tabl,
 #pragma unroll
   for (int r = 0; r < R; ++r) {
     const int m = r * TC_WAVE + lane;
-    dv[r] = dv[r] * tabl[2 * TC_MAX_STAGES * TC_WAVE * R + m];
+    dv[r] = dv[r] * tabl[2 * k1_nst<R>() * TC_WAVE * R + m];
   }
 }
 
@@ -688,6 +702,192 @@ __device__ __forceinline__ void k1_pencil_w(const K1Ctx<T, R>& c, int i, int j0,
   k1_plain_cols<T, R, true>(c, wi, pw + oi, rw + oi, sw + oi, nullptr, d.syw, jhi + 1, j1, 1, 0, n, Kc, rz0, rz1, rz2);
 }
 
+// ---------------------------------------------------------------------------------------------
+// Three-component pencils.  One wavefront owns one i and marches j for u, v and w together: per
+// step it loads the nine columns u, v, w at i-1, i, i+1 of a plane (four of them a plane ahead) and
+// the three history columns, so each column that another component's stencil reads is fetched
+// once per wavefront (12 column loads per (i, j) instead of the per-component pencils' 22).  The
+// explicit terms are the per-component pencils' expressions, evaluated on the same values, so the
+// results are bit-identical to theirs.  Interior only (1 <= i <= nx-1 and 1 <= j0, j1-1 <= ny-1,
+// where all three components have stencils in both rhs branches); the caller runs the
+// per-component pencils elsewhere.  Every column is held with nz rows (w's row nz is the top
+// face), which changes no value the w stencil keeps: its rows past n = nz-1 are dropped.
+// ---------------------------------------------------------------------------------------------
+template <typename T, int R>
+struct K3Next {
+  T uE[R], vW[R], vE[R], wW[R], wE[R];   // plane j+1: columns only step j+1 reads
+  T uN[R], umN[R], vN[R], wN[R];         // plane j+2: u(i), u(i-1), v(i), w(i)
+  T pu[R], pv[R], pw[R];                 // history at j+1
+  T ub0, ubn, ubz, umbz, vb0, vbn, vbz, wbz, wEb0;       // ghost / top rows at j+1
+  T wNb0;                                // w(i, j+2)[0]
+};
+
+// met: the block's LDS copy of the wall-normal metric of the lane rows (k1_metric: cell rz0, rz1,
+// rz2, then face rz0, rz1, rz2, each 64*R values), read where used instead of held in registers.
+template <int Q, typename T, int R>
+__device__ __forceinline__ void k3_met(const T* met, T (&x)[R], int lane) {
+#pragma unroll
+  for (int r = 0; r < R; ++r) x[r] = met[(Q * R + r) * TC_WAVE + lane];
+}
+
+template <typename T, int R>
+__device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx<T, R>& cv, const K1Ctx<T, R>& cw,
+                                              const T* met, int i, int j0, int j1, const T* __restrict__ u,
+                                              const T* __restrict__ v, const T* __restrict__ w,
+                                              const T* __restrict__ pu, const T* __restrict__ pv,
+                                              const T* __restrict__ pw, T* __restrict__ ru, T* __restrict__ rv,
+                                              T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv,
+                                              T* __restrict__ sw) {
+  const Dims& d = cu.d;
+  const int lane = cu.lane, n = d.nz, nw = d.nz - 1;
+  const T h = T(0.5);
+  const bool step = (cu.mode & TC_K1_STEP) != 0, ab2 = step && !(cu.mode & TC_K1_FIRST);
+  const long long ou = (long long)i * d.sxu, ov = (long long)i * d.sxv, ow = (long long)i * d.sxw;
+  const T* ui = u + ou;
+  const T* vi = v + ov;
+  const T* wi = w + ow;
+  const T* bi = cu.body != nullptr ? cu.body + ou : nullptr;
+  // loads of step j's inputs that are not yet in registers
+  auto fetch = [&](int j, K3Next<T, R>& L) {
+    const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
+    col_load<R, T>(ui + d.sxu + cu1, L.uE, n, lane);
+    col_load<R, T>(vi - d.sxv + cv1, L.vW, n, lane);
+    col_load<R, T>(vi + d.sxv + cv1, L.vE, n, lane);
+    col_load<R, T>(wi - d.sxw + cw1, L.wW, n, lane);
+    col_load<R, T>(wi + d.sxw + cw1, L.wE, n, lane);
+    col_load<R, T>(ui + cu1 + d.syu, L.uN, n, lane);
+    col_load<R, T>(ui - d.sxu + cu1 + d.syu, L.umN, n, lane);
+    col_load<R, T>(vi + cv1 + d.syv, L.vN, n, lane);
+    col_load<R, T>(wi + cw1 + d.syw, L.wN, n, lane);
+    if (ab2) {
+      col_load_hist<R, T>(pu + ou + cu1, L.pu, n, lane);
+      col_load_hist<R, T>(pv + ov + cv1, L.pv, n, lane);
+      col_load_hist<R, T>(pw + ow + cw1, L.pw, n, lane);
+    }
+    L.ub0 = ui[cu1];
+    L.ubn = ui[cu1 + n + 1];
+    L.ubz = ui[cu1 + n];
+    L.umbz = ui[cu1 + n - d.sxu];
+    L.vb0 = vi[cv1];
+    L.vbn = vi[cv1 + n + 1];
+    L.vbz = vi[cv1 + n];
+    L.wbz = wi[cw1 + n];
+    L.wEb0 = wi[cw1 + d.sxw];
+    L.wNb0 = wi[cw1 + d.syw];
+  };
+  // plane j0-1: the columns step j0 reads from it, and v(i, j0-1) one row up
+  T uS[R], vS[R], v1m[R], wS[R], vm1[R];
+  col_load<R, T>(ui + (long long)(j0 - 1) * d.syu, uS, n, lane);
+  col_load<R, T>(vi + (long long)(j0 - 1) * d.syv, vS, n, lane);
+  col_load<R, T>(vi + d.sxv + (long long)(j0 - 1) * d.syv, v1m, n, lane);
+  col_load<R, T>(wi + (long long)(j0 - 1) * d.syw, wS, n, lane);
+  rows_up<R, T>(cu.rows, vS, vm1, vi[(long long)(j0 - 1) * d.syv + n], nw, lane);
+  // plane j0: u(i-1), u(i), v(i), w(i) (the rest comes with fetch)
+  T uC[R], uW[R], vC[R], wC[R];
+  col_load<R, T>(ui + (long long)j0 * d.syu, uC, n, lane);
+  col_load<R, T>(ui - d.sxu + (long long)j0 * d.syu, uW, n, lane);
+  col_load<R, T>(vi + (long long)j0 * d.syv, vC, n, lane);
+  col_load<R, T>(wi + (long long)j0 * d.syw, wC, n, lane);
+  T wC0 = wi[(long long)j0 * d.syw];
+  K3Next<T, R> A;
+  fetch(j0, A);
+  for (int j = j0; j < j1; ++j) {
+    const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
+    T xu[R], xv[R], xw[R];
+    bool cn_u, cn_v, cn_w;
+    {
+      // u (k1_pencil_u): v00 = v(i,j), v10 = v(i+1,j), v0m = v(i,j-1), v1m = v(i+1,j-1), w00 = w(i,j),
+      // w10 = w(i+1,j)
+      T rhs[R], ub[R], ut[R], w0m[R], w1m[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        rhs[r] = (A.uE[r] - T(2) * uC[r] + uW[r]) * cu.mt.kxx + (A.uN[r] - T(2) * uC[r] + uS[r]) * cu.mt.kyy;
+      rows_pm<R, T>(cu.rows, uC, ub, ut, A.ub0, A.ubn, n, lane);
+      rows_dn<R, T>(cu.rows, wC, w0m, wC0, lane);
+      rows_dn<R, T>(cu.rows, A.wE, w1m, A.wEb0, lane);
+      T rc2[R];
+      k3_met<2, T, R>(met, rc2, lane);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T ur = h * (uC[r] + A.uE[r]), ul = h * (uW[r] + uC[r]);
+        const T fx = (ur * ur - ul * ul) * cu.mt.rdx;
+        const T fy = (h * (vC[r] + A.vE[r]) * (h * (uC[r] + A.uN[r])) - h * (vS[r] + v1m[r]) * (h * (uS[r] + uC[r]))) * cu.mt.rdy;
+        const T fz = (h * (wC[r] + A.wE[r]) * (h * (uC[r] + ut[r])) - h * (w0m[r] + w1m[r]) * (h * (ub[r] + uC[r]))) * rc2[r];
+        rhs[r] = rhs[r] - (fx + fy + fz);
+      }
+      cn_u = k1_rhs_ab2<T, R>(cu, rhs, uC, A.pu, ru + ou + cu1, bi != nullptr ? bi + cu1 : nullptr, n, n + 2, true,
+                              xu);
+      // v (k1_pencil_v): u00 = u(i,j), u01 = u(i,j+1), um0 = u(i-1,j), um1 = u(i-1,j+1), w00 = w(i,j),
+      // w01 = w(i,j+1), w0m = w(i,j) one row down (as above)
+      T vb[R], vt[R], w1mv[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        rhs[r] = (A.vE[r] - T(2) * vC[r] + A.vW[r]) * cv.mt.kxx + (A.vN[r] - T(2) * vC[r] + vS[r]) * cv.mt.kyy;
+      rows_pm<R, T>(cu.rows, vC, vb, vt, A.vb0, A.vbn, n, lane);
+      rows_dn<R, T>(cu.rows, A.wN, w1mv, A.wNb0, lane);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T fx = (h * (uC[r] + A.uN[r]) * (h * (vC[r] + A.vE[r])) - h * (uW[r] + A.umN[r]) * (h * (A.vW[r] + vC[r]))) * cv.mt.rdx;
+        const T vr = h * (vC[r] + A.vN[r]), vl = h * (vS[r] + vC[r]);
+        const T fy = (vr * vr - vl * vl) * cv.mt.rdy;
+        const T fz = (h * (wC[r] + A.wN[r]) * (h * (vC[r] + vt[r])) - h * (w0m[r] + w1mv[r]) * (h * (vb[r] + vC[r]))) * rc2[r];
+        rhs[r] = rhs[r] - (fx + fy + fz);
+      }
+      cn_v = k1_rhs_ab2<T, R>(cv, rhs, vC, A.pv, rv + ov + cv1, nullptr, n, n + 2, true, xv);
+      // w (k1_pencil_w, rows k = 1..nz-1): u00 = u(i,j), um0 = u(i-1,j), v00 = v(i,j), vm0 = v(i,j-1),
+      // each with its row above
+      T wb[R], wt[R], u01[R], um1[R], v01[R];
+      rows_pm<R, T>(cu.rows, wC, wb, wt, wC0, A.wbz, nw, lane);
+      rows_up<R, T>(cu.rows, uC, u01, A.ubz, nw, lane);
+      rows_up<R, T>(cu.rows, uW, um1, A.umbz, nw, lane);
+      rows_up<R, T>(cu.rows, vC, v01, A.vbz, nw, lane);
+      T rf2[R];
+      k3_met<5, T, R>(met, rf2, lane);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const T lap = (A.wE[r] - T(2) * wC[r] + A.wW[r]) * cw.mt.kxx + (A.wN[r] - T(2) * wC[r] + wS[r]) * cw.mt.kyy;
+        const T fx = (h * (uC[r] + u01[r]) * (h * (wC[r] + A.wE[r])) - h * (uW[r] + um1[r]) * (h * (A.wW[r] + wC[r]))) * cw.mt.rdx;
+        const T fy = (h * (vC[r] + v01[r]) * (h * (wC[r] + A.wN[r])) - h * (vS[r] + vm1[r]) * (h * (wS[r] + wC[r]))) * cw.mt.rdy;
+        const T wr = h * (wC[r] + wt[r]), wl = h * (wb[r] + wC[r]);
+        const T fz = (wr * wr - wl * wl) * rf2[r];
+        rhs[r] = lap - (fx + fy + fz);
+      }
+      cn_w = k1_rhs_ab2<T, R>(cw, rhs, wC, A.pw, rw + ow + cw1, nullptr, nw, n + 1, true, xw);
+      // carry plane j as plane j-1 and plane j+1 as plane j
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        uS[r] = uC[r];
+        vS[r] = vC[r];
+        v1m[r] = A.vE[r];
+        wS[r] = wC[r];
+        vm1[r] = v01[r];
+        uC[r] = A.uN[r];
+        uW[r] = A.umN[r];
+        vC[r] = A.vN[r];
+        wC[r] = A.wN[r];
+      }
+      wC0 = A.wNb0;
+    }
+    // the next step's loads are in flight during the three CN solves
+    if (j + 1 < j1) fetch(j + 1, A);
+    if (cn_u || cn_v) {
+      T a0[R], a1[R], a2[R];
+      k3_met<0, T, R>(met, a0, lane);
+      k3_met<1, T, R>(met, a1, lane);
+      k3_met<2, T, R>(met, a2, lane);
+      if (cn_u) k1_cn_store<T, R, false>(cu, xu, su + ou + cu1, n, a0, a1, a2, T(0));
+      if (cn_v) k1_cn_store<T, R, false>(cv, xv, sv + ov + cv1, n, a0, a1, a2, T(0));
+    }
+    if (cn_w) {
+      T a0[R], a1[R], a2[R];
+      k3_met<3, T, R>(met, a0, lane);
+      k3_met<4, T, R>(met, a1, lane);
+      k3_met<5, T, R>(met, a2, lane);
+      k1_cn_store<T, R, true>(cw, xw, sw + ow + cw1, nw, a0, a1, a2, T(0));
+    }
+  }
+}
+
 // Pencil bands: a band is TC_K1_WAVES consecutive i (one per wavefront) by jb consecutive j, on
 // the common (nx+2) x (ny+2) column plane; block b serves component b % 3 of band b / 3, so the
 // three components of one band run back to back on the same XCD.
@@ -705,7 +905,7 @@ __device__ __forceinline__ void explicit_cn_body(
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
     const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb,
     int b) {
-  __shared__ T lds_tab[(2 * TC_MAX_STAGES + 1) * TC_WAVE * R];
+  __shared__ T lds_tab[k1_tab_len<R>()];
   __shared__ T lds_rows[TC_K1_WAVES][3 * TC_WAVE * R];
   if constexpr (FULL) d.nz = TC_WAVE * R;
   const int comp = b % 3, band = b / 3;
@@ -715,10 +915,7 @@ __device__ __forceinline__ void explicit_cn_body(
   const int lane = lane_id();
   if (mode & TC_K1_STEP) {
     const double* tab = comp < 2 ? tab_cell : tab_face;
-    for (int e = threadIdx.x; e < (2 * TC_MAX_STAGES + 1) * TC_WAVE * R; e += 64 * TC_K1_WAVES) {
-      const int row = e / (TC_WAVE * R), m = e - row * (TC_WAVE * R);
-      lds_tab[e] = (T)tab[row * TC_MAX_NZ + m];
-    }
+    for (int e = threadIdx.x; e < k1_tab_len<R>(); e += 64 * TC_K1_WAVES) lds_tab[e] = (T)k1_tab_src<R>(tab, e);
     __syncthreads();
   }
   const int i = bi * TC_K1_WAVES + wv;
@@ -757,6 +954,83 @@ __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
                                mode, branch, theta, jb, xcd_remap(blockIdx.x, gridDim.x));
 }
 
+
+// Three-component form (TC_TUNE_K1_TILE = 2): a block of W wavefronts owns a band of W
+// consecutive i by jb consecutive j for all three components, with both CN tables in LDS;
+// interior bands march k1_pencil_uvw, the others run the per-component pencils in turn.
+// W = 8 where a wavefront fits 128 VGPRs (one row per lane, or float32 with two), so that two
+// blocks give a CU 4 wavefronts per SIMD; else W = 4 (more VGPRs, three blocks per CU).
+template <typename T, int R>
+__host__ __device__ constexpr int k3_waves() { return (R == 1 || (sizeof(T) == 4 && R == 2)) ? 8 : 4; }
+template <typename T, int R>
+__host__ __device__ constexpr int k3_min_waves() { return k3_waves<T, R>() == 8 ? 4 : 1; }
+__host__ __device__ __forceinline__ int k3_bands_i(int nx, int W) { return (nx + 2 + W - 1) / W; }
+
+template <typename T, int R, bool FULL>
+__global__ __launch_bounds__((64 * k3_waves<T, R>()), (k3_min_waves<T, R>())) void k_explicit_cn_uvw(
+    Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
+    const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
+    T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
+    const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
+    const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb) {
+  constexpr int W = k3_waves<T, R>();
+  __shared__ T lds_tab[2][k1_tab_len<R>()];
+  __shared__ T lds_rows[W][3 * TC_WAVE * R];
+  __shared__ T lds_met[6 * TC_WAVE * R];
+  if constexpr (FULL) d.nz = TC_WAVE * R;
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int nbj = k1_bands_j(d.ny, jb);
+  const int bi = b / nbj, bj = b - bi * nbj;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = lane_id();
+  if (mode & TC_K1_STEP) {
+    for (int e = threadIdx.x; e < k1_tab_len<R>(); e += 64 * W) {
+      lds_tab[0][e] = (T)k1_tab_src<R>(tab_cell, e);
+      lds_tab[1][e] = (T)k1_tab_src<R>(tab_face, e);
+    }
+  }
+  // the metric of k1_metric (cell rows: n = nz, face rows: n = nz - 1)
+  for (int e = threadIdx.x; e < TC_WAVE * R; e += 64 * W) {
+    const int nc = d.nz, nf = d.nz - 1;
+    lds_met[e] = e < nc ? mt.rdzc[e] : T(0);
+    lds_met[TC_WAVE * R + e] = e < nc ? mt.rdzc[e + 1] : T(0);
+    lds_met[2 * TC_WAVE * R + e] = e < nc ? mt.rdzf[e] : T(0);
+    lds_met[3 * TC_WAVE * R + e] = e < nf ? mt.rdzf[e] : T(0);
+    lds_met[4 * TC_WAVE * R + e] = e < nf ? mt.rdzf[e + 1] : T(0);
+    lds_met[5 * TC_WAVE * R + e] = e < nf ? mt.rdzc[e + 1] : T(0);
+  }
+  __syncthreads();
+  const int i = bi * W + wv;
+  const int j0 = bj * jb, jend = j0 + jb;
+  if (i > d.nx + 1 || j0 > d.ny + 1) return;
+  K1Ctx<T, R> cu;
+  cu.d = d;
+  cu.mt = mt;
+  cu.rows = lds_rows[wv];
+  cu.tabl = lds_tab[0];
+  cu.lane = lane;
+  cu.mode = mode;
+  cu.cpu = branch == TC_BRANCH_CPU;
+  const double dt = scal[TC_S_DT];
+  cu.dt = (T)dt;
+  cu.beta = (T)((1.0 - theta) * dt * mt.nu);
+  cu.F = (mode & TC_K1_FORCE) ? (T)scal[TC_S_F] : T(0);
+  cu.body = (mode & TC_K1_FORCE) ? body : nullptr;
+  K1Ctx<T, R> cv = cu;
+  cv.F = T(0);
+  cv.body = nullptr;
+  K1Ctx<T, R> cw = cv;
+  cw.tabl = lds_tab[1];
+  rows_zero_pads<R, T>(cu.rows, lane);
+  if (i >= 1 && i <= d.nx - 1 && j0 >= 1 && jend - 1 <= d.ny - 1) {
+    k1_pencil_uvw<T, R>(cu, cv, cw, lds_met, i, j0, jend, u, v, w, pu, pv, pw, ru, rv, rw, su, sv, sw);
+  } else {
+    if (i <= d.nx) k1_pencil_u<T, R>(cu, i, j0, min(jend, d.ny + 2), u, v, w, pu, ru, su);
+    if (j0 <= d.ny) k1_pencil_v<T, R>(cv, i, j0, min(jend, d.ny + 1), u, v, w, pv, rv, sv);
+    k1_pencil_w<T, R>(cw, i, j0, min(jend, d.ny + 2), u, v, w, pw, rw, sw);
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // K1, tiled form.  The same explicit terms, forcing, AB2 and CN solve as the pencils above
 // (every expression is the pencils' own, so results are bit-identical), with the neighbour
@@ -784,7 +1058,7 @@ struct KT {
   static constexpr int CW = TC_WAVE * R + 2;                 // LDS column: global rows 0..64R+1
   static constexpr int SLOTS = TC_KT_I + 2;                  // i0-1 .. i0+4
   static constexpr int PLANE = 3 * SLOTS * CW;
-  static constexpr int TAB = (2 * TC_MAX_STAGES + 1) * TC_WAVE * R;
+  static constexpr int TAB = k1_tab_len<R>();
   static constexpr int ROWS = 3 * TC_WAVE * R;
   static constexpr int NPL = 3;                              // planes in the ring
   static constexpr size_t BYTES = sizeof(T) * ((size_t)NPL * PLANE + 2 * TAB + (size_t)TC_KT_WAVES * ROWS);
@@ -1083,9 +1357,8 @@ __global__ __launch_bounds__(64 * TC_KT_WAVES) void k_explicit_cn_tile(
   const int ja = j0 > 0 ? j0 - 1 : 0;
   if (mode & TC_K1_STEP) {
     for (int e = threadIdx.x; e < KT<T, R>::TAB; e += 64 * TC_KT_WAVES) {
-      const int row = e / (TC_WAVE * R), m = e - row * (TC_WAVE * R);
-      tabs[e] = (T)tab_cell[row * TC_MAX_NZ + m];
-      tabs[KT<T, R>::TAB + e] = (T)tab_face[row * TC_MAX_NZ + m];
+      tabs[e] = (T)k1_tab_src<R>(tab_cell, e);
+      tabs[KT<T, R>::TAB + e] = (T)k1_tab_src<R>(tab_face, e);
     }
   }
   K1Ctx<T, R> c;
@@ -1416,6 +1689,30 @@ static int launch_k1_t(const Grid* gr, const void* u, const void* v, const void*
 #undef TC_KTL
     if (rc != TC_ERR_UNSUPPORTED) return rc;      // else the block's LDS does not fit: pencils
   }
+  if (gr->k1_form() == 2) {
+    // three-component pencils: static LDS (both CN tables and the row buffers) within 64 KiB
+    const bool full = d.nz == TC_WAVE * R;
+    bool launched = true;
+#define TC_K3(RR, F)                                                                                    \
+  hipLaunchKernelGGL((k_explicit_cn_uvw<T, RR, F>),                                                    \
+                     dim3(k3_bands_i(d.nx, k3_waves<T, RR>()) * k1_bands_j(d.ny, jb)),                 \
+                     dim3(64 * k3_waves<T, RR>()), 0, st, d, mt,                                        \
+                     (const T*)u, (const T*)v, (const T*)w, (const T*)pu, (const T*)pv, (const T*)pw,     \
+                     (T*)ru, (T*)rv, (T*)rw, (T*)su, (T*)sv, (T*)sw, tab_cell, tab_face, scal,          \
+                     (const T*)body, mode, branch, theta, jb)
+    switch (R) {
+      case 1: if (full) TC_K3(1, true); else TC_K3(1, false); break;
+      case 2: if (full) TC_K3(2, true); else TC_K3(2, false); break;
+      case 3: if constexpr (sizeof(T) == 4) { if (full) TC_K3(3, true); else TC_K3(3, false); } else launched = false; break;
+      case 4: if constexpr (sizeof(T) == 4) { if (full) TC_K3(4, true); else TC_K3(4, false); } else launched = false; break;
+      default: launched = false; break;
+    }
+#undef TC_K3
+    if (launched) {
+      TC_CHECK_LAUNCH();
+      return TC_OK;
+    }
+  }
   const int blocks = 3 * k1_bands_i(d.nx) * k1_bands_j(d.ny, jb);
 #define TC_K1(RR, F)                                                                                \
   hipLaunchKernelGGL((k_explicit_cn<T, RR, F>), dim3(blocks), dim3(64 * TC_K1_WAVES), 0, st, d, mt,  \
@@ -1599,8 +1896,9 @@ int ens_launch_finalize(int dtype, const void* dch, int n, int flags, hipStream_
   return TC_OK;
 }
 
+// Blocks of the per-component pencils (the form the ensemble kernels run).
 int k1_blocks(const Grid* g) {
-  const int jb = g->k1_pencil();
+  const int jb = g->k1_pencil_of(0);
   return 3 * k1_bands_i(g->d.nx) * k1_bands_j(g->d.ny, jb);
 }
 
