@@ -748,22 +748,25 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
   const T* wi = w + ow;
   const T* bi = cu.body != nullptr ? cu.body + ou : nullptr;
   // loads of step j's inputs that are not yet in registers
-  auto fetch = [&](int j, K3Next<T, R>& L) {
+  auto fetch_now = [&](int j, K3Next<T, R>& L) {
     const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
     col_load<R, T>(ui + d.sxu + cu1, L.uE, n, lane);
     col_load<R, T>(vi - d.sxv + cv1, L.vW, n, lane);
     col_load<R, T>(vi + d.sxv + cv1, L.vE, n, lane);
     col_load<R, T>(wi - d.sxw + cw1, L.wW, n, lane);
     col_load<R, T>(wi + d.sxw + cw1, L.wE, n, lane);
-    col_load<R, T>(ui + cu1 + d.syu, L.uN, n, lane);
-    col_load<R, T>(ui - d.sxu + cu1 + d.syu, L.umN, n, lane);
-    col_load<R, T>(vi + cv1 + d.syv, L.vN, n, lane);
-    col_load<R, T>(wi + cw1 + d.syw, L.wN, n, lane);
     if (ab2) {
       col_load_hist<R, T>(pu + ou + cu1, L.pu, n, lane);
       col_load_hist<R, T>(pv + ov + cv1, L.pv, n, lane);
       col_load_hist<R, T>(pw + ow + cw1, L.pw, n, lane);
     }
+  };
+  auto fetch_ahead = [&](int j, K3Next<T, R>& L) {
+    const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
+    col_load<R, T>(ui + cu1 + d.syu, L.uN, n, lane);
+    col_load<R, T>(ui - d.sxu + cu1 + d.syu, L.umN, n, lane);
+    col_load<R, T>(vi + cv1 + d.syv, L.vN, n, lane);
+    col_load<R, T>(wi + cw1 + d.syw, L.wN, n, lane);
     L.ub0 = ui[cu1];
     L.ubn = ui[cu1 + n + 1];
     L.ubz = ui[cu1 + n];
@@ -790,7 +793,8 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
   col_load<R, T>(wi + (long long)j0 * d.syw, wC, n, lane);
   T wC0 = wi[(long long)j0 * d.syw];
   K3Next<T, R> A;
-  fetch(j0, A);
+  fetch_ahead(j0, A);
+  fetch_now(j0, A);
   for (int j = j0; j < j1; ++j) {
     const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
     T xu[R], xv[R], xw[R];
@@ -869,7 +873,10 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
       wC0 = A.wNb0;
     }
     // the next step's loads are in flight during the three CN solves
-    if (j + 1 < j1) fetch(j + 1, A);
+    if (j + 1 < j1) {
+      fetch_ahead(j + 1, A);
+      fetch_now(j + 1, A);
+    }
     if (cn_u || cn_v) {
       T a0[R], a1[R], a2[R];
       k3_met<0, T, R>(met, a0, lane);
@@ -959,7 +966,9 @@ __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
 // consecutive i by jb consecutive j for all three components, with both CN tables in LDS;
 // interior bands march k1_pencil_uvw, the others run the per-component pencils in turn.
 // W = 8 where a wavefront fits 128 VGPRs (one row per lane, or float32 with two), so that two
-// blocks give a CU 4 wavefronts per SIMD; else W = 4 (more VGPRs, three blocks per CU).
+// blocks give a CU 4 wavefronts per SIMD; else W = 4 (more VGPRs, three blocks per CU).  Capping
+// the float64 two-row instance at 128 VGPRs spills inside the march (measured: 146 -> 248 us at
+// 192x192x128), so it keeps its 152 VGPRs and 3 wavefronts per SIMD.
 template <typename T, int R>
 __host__ __device__ constexpr int k3_waves() { return (R == 1 || (sizeof(T) == 4 && R == 2)) ? 8 : 4; }
 template <typename T, int R>
