@@ -26,7 +26,7 @@ NX, NY, NZ = 192, 192, 128
 HBM_PEAK_GBS = 8000.0
 PHASES = ["cn_tables", "explicit_ab2_cn", "div_fft_x", "fft_y_fwd", "poisson_thomas", "fft_y_inv",
           "ifft_x_p", "project_bc_bulk", "cfl_max"]
-KERNEL_OF = {"explicit_ab2_cn": "k_explicit_cn", "div_fft_x": "k_fwd_x", "fft_y_fwd": "k_fft_y",
+KERNEL_OF = {"explicit_ab2_cn": "k_explicit_cn_uvw", "div_fft_x": "k_fwd_x", "fft_y_fwd": "k_fft_y",
              "poisson_thomas": "k_poisson_thomas", "fft_y_inv": "k_fft_y", "ifft_x_p": "k_inv_x",
              "project_bc_bulk": "k_project_bc", "cfl_max": "k_cfl"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "imex_step_192x192x128_hbm_traffic.json")
@@ -177,6 +177,8 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
         per[name] = {"avg_us": avg_ms * 1e3, "alg_bytes": byts,
                      "gbs": byts / (avg_ms * 1e-3) / 1e9 if byts else 0.0}
     out["kernels"] = per
+    out["k1_form"] = ch._grid.get_tuning(L.TUNE_K1_TILE)
+    out["k1_jb"] = ch._grid.get_tuning(L.TUNE_K1_JB)
     step_kernels = [k for k in PHASES if k not in ("cfl_max",)]
     dom = max(step_kernels, key=lambda k: per[k]["avg_us"])
     out["dominant"] = dom
@@ -189,10 +191,11 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
 
 
 def column_ceiling(precision, nx, ny, nz, jb, reps=20):
-    """Bandwidth of the explicit + CN kernel's column access pattern without its arithmetic
-    (tc_calib_columns): pattern 0 with the kernel's neighbour-column loads, pattern 1 with only
-    its compulsory streams; GB/s of the kernel's 12 algorithmic words per cell, from HIP events
-    on the launch stream."""
+    """Bandwidth of the explicit + CN kernel's column access patterns without its arithmetic
+    (tc_calib_columns): the three-component pencils' pattern (the form the kernel runs at these
+    sizes), the per-component pencils' pattern with their neighbour-column loads, and the
+    compulsory streams alone; GB/s of the kernel's 12 algorithmic words per cell, from HIP events
+    on the launch stream.  jb: the pencil length of the three-component form."""
     import ctypes
     import torch
     from torchannel_amd import _lib as L
@@ -202,14 +205,14 @@ def column_ceiling(precision, nx, ny, nz, jb, reps=20):
     ptrs = (ctypes.c_void_p * 12)(*[a.data_ptr() for a in arrs])
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     out = {}
-    for pat, name in ((0, "k1_pattern"), (1, "compulsory")):
+    for pat, name, pjb in ((2, "k1_pattern", jb), (0, "pencil_pattern", 8), (1, "compulsory", 8)):
         for _ in range(3):
-            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, jb, ptrs, st)
+            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, pjb, ptrs, st)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, jb, ptrs, st)
+            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, pjb, ptrs, st)
         e1.record()
         torch.cuda.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / reps
@@ -459,7 +462,7 @@ def main():
     if not args.no_fp32 and args.precision == "fp64":
         other = run_gpu("fp32", steps, warmup)
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
-    ceiling = column_ceiling(args.precision, NX, NY, NZ, 8)
+    ceiling = column_ceiling(args.precision, NX, NY, NZ, main_run["k1_jb"])
     print(f"[bench] column pattern: {ceiling['k1_pattern']['gbs']} GB/s, compulsory streams "
           f"{ceiling['compulsory']['gbs']} GB/s", file=sys.stderr, flush=True)
     c3 = None
@@ -504,7 +507,8 @@ def main():
                                "Re_bulk=2870, adaptive dt CFL 0.25 every 10 steps, constant flow rate, "
                                "turbstats every 200 steps",
                    "precision": args.precision, "graph_chunk_steps": 10},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "k1_form": main_run["k1_form"], "k1_jb": main_run["k1_jb"],
+                     "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "alg_bytes": k["alg_bytes"], "avg_us": round(k["avg_us"], 2),
                      "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (copy-calibrated), "

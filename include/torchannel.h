@@ -233,10 +233,12 @@ int tc_solver_set_newest(tc_solver* s, int newest);
 /* ---- roofline calibration ------------------------------------------------------------------
  * No counterpart in the reference.  Sweeps twelve arrays shaped like p ((nx+2) x (ny+2) x
  * (nz+2), arrays[0..5] read, arrays[6..11] written) in the column order of the explicit + CN
- * kernel without its arithmetic: pattern 0 moves that kernel's eight input and two output
- * columns per column step (neighbour columns included), pattern 1 only its compulsory streams
- * (one input column and the AB2 history in, two columns out).  Timed beside the kernel, it gives
- * the bandwidth this access pattern reaches on the device.  nz <= 256. */
+ * kernel without its arithmetic: pattern 0 moves the per-component pencils' eight input and two
+ * output columns per column step (neighbour columns included), pattern 1 only the compulsory
+ * streams (one input column and the AB2 history in, two columns out), pattern 2 the
+ * three-component pencils' nine input, three history and six output columns per (i, j) step.
+ * Timed beside the kernel, it gives the bandwidth each access pattern reaches on the device.
+ * nz <= 256. */
 int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, void* const* arrays, void* stream);
 
 /* ---- discrete adjoint --------------------------------------------------------------------

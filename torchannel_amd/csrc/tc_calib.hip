@@ -9,7 +9,11 @@
 // and storing two columns per step, in the order and XCD mapping of k_explicit_cn.
 //   pattern 0: per column the kernel's eight input columns (own j+1, i-1, i+1, history, and two
 //              columns of each other component) and two output columns;
-//   pattern 1: only the compulsory streams: own j+1 and history in, two columns out.
+//   pattern 1: only the compulsory streams: own j+1 and history in, two columns out;
+//   pattern 2: the three-component pencils' pattern: blocks of 4 wavefronts own a band of 4
+//              consecutive i for all three components; each wavefront marches its i along j,
+//              per step loading the nine columns (components 0-2 at i-1, i, i+1) of plane j+1
+//              and the three history columns of (i, j), and storing six columns.
 #include "tc_internal.h"
 
 namespace tc {
@@ -98,9 +102,80 @@ __global__ __launch_bounds__(256) void k_calib_columns(int nx, int ny, int nz, i
   }
 }
 
+
+template <typename T, int R>
+__global__ __launch_bounds__(256) void k_calib_uvw(int nx, int ny, int nz, int jb, const T* __restrict__ A0,
+                                                   const T* __restrict__ A1, const T* __restrict__ A2,
+                                                   const T* __restrict__ P0, const T* __restrict__ P1,
+                                                   const T* __restrict__ P2, T* __restrict__ O0,
+                                                   T* __restrict__ O1, T* __restrict__ O2, T* __restrict__ S0,
+                                                   T* __restrict__ S1, T* __restrict__ S2) {
+  const long long sy = nz + 2, sx = (long long)(ny + 2) * (nz + 2);
+  const int b = xcd_remap(blockIdx.x, gridDim.x);
+  const int nbj = (ny + jb - 1) / jb;
+  const int bi = b / nbj, bj = b - bi * nbj;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = lane_id();
+  const int i = 1 + bi * 4 + wv;
+  if (i > nx) return;
+  const int j0 = 1 + bj * jb, j1 = min(j0 + jb, ny + 1);
+  const long long oi = (long long)i * sx;
+  const T* A[3] = {A0, A1, A2};
+  const T* P[3] = {P0, P1, P2};
+  T* O[3] = {O0, O1, O2};
+  T* S[3] = {S0, S1, S2};
+  // c: plane j (component q at i-1, i, i+1 in c[3q..3q+2]); nx9: plane j+1; hs: plane j-1 at i
+  T c[9][R], nx9[9][R], hp[3][R], hs[3][R];
+  auto plane = [&](int j, T (&x)[9][R]) {
+    const long long cj = (long long)j * sy;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      cal_load<R, T>(A[q] + oi - sx + cj, x[3 * q], nz, lane);
+      cal_load<R, T>(A[q] + oi + cj, x[3 * q + 1], nz, lane);
+      cal_load<R, T>(A[q] + oi + sx + cj, x[3 * q + 2], nz, lane);
+    }
+  };
+  plane(j0, c);
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int r = 0; r < R; ++r) hs[q][r] = c[3 * q + 1][r];
+  for (int j = j0; j < j1; ++j) {
+    const long long cj = (long long)j * sy;
+    plane(j + 1, nx9);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) cal_load<R, T>(P[q] + oi + cj, hp[q], nz, lane);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int a = (q + 1) % 3, e = (q + 2) % 3;
+      T x[R], y[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T v = nx9[3 * q + 1][r] - T(2) * c[3 * q + 1][r] + hs[q][r] + T(0.5) * hp[q][r];
+        v = v + (c[3 * q][r] + c[3 * q + 2][r]) * (c[3 * a + 1][r] + c[3 * a + 2][r]) - c[3 * e + 1][r] * c[3 * e + 2][r];
+        x[r] = v;
+        y[r] = v * T(0.25) + c[3 * q + 1][r];
+      }
+      cal_store<R, T>(O[q] + oi + cj, x, nz, lane);
+      cal_store<R, T>(S[q] + oi + cj, y, nz, lane);
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q)
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (q % 3 == 1) hs[q / 3][r] = c[q][r];
+        c[q][r] = nx9[q][r];
+      }
+  }
+}
+
 template <typename T, int R>
 static void calib_launch(int pattern, int blocks, hipStream_t st, int nx, int ny, int nz, int jb, void* const* a) {
-  if (pattern == 0)
+  if (pattern == 2)
+    hipLaunchKernelGGL((k_calib_uvw<T, R>), dim3(blocks / 3), dim3(256), 0, st, nx, ny, nz, jb, (const T*)a[0],
+                       (const T*)a[1], (const T*)a[2], (const T*)a[3], (const T*)a[4], (const T*)a[5], (T*)a[6],
+                       (T*)a[7], (T*)a[8], (T*)a[9], (T*)a[10], (T*)a[11]);
+  else if (pattern == 0)
     hipLaunchKernelGGL((k_calib_columns<T, R, 0>), dim3(blocks), dim3(256), 0, st, nx, ny, nz, jb, (const T*)a[0],
                        (const T*)a[1], (const T*)a[2], (const T*)a[3], (const T*)a[4], (const T*)a[5], (T*)a[6],
                        (T*)a[7], (T*)a[8], (T*)a[9], (T*)a[10], (T*)a[11]);
@@ -116,7 +191,7 @@ using namespace tc;
 
 extern "C" int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, void* const* arrays,
                                 void* stream) {
-  if (!arrays || nx < 1 || ny < 1 || nz < 1 || nz > TC_MAX_NZ || jb < 1 || (pattern != 0 && pattern != 1))
+  if (!arrays || nx < 1 || ny < 1 || nz < 1 || nz > TC_MAX_NZ || jb < 1 || pattern < 0 || pattern > 2)
     return TC_ERR_ARG;
   for (int q = 0; q < 12; ++q)
     if (!arrays[q]) return TC_ERR_ARG;
