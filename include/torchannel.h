@@ -66,14 +66,17 @@ int tc_grid_destroy(tc_grid* g);
 
 /* Launch-shape knobs of a grid's kernels (no counterpart in the reference: its kernels are
  * TorchScript-fused).  Results do not depend on them; torchannel_amd/autotune.py picks them by
- * timing.  TC_TUNE_K1_JB: j-columns per explicit/CN pencil, 1..ny+2 (0 restores the default,
- * about ny/24 in steps of 4 within [8, 16]).
+ * timing.  TC_TUNE_K1_JB: j-columns per explicit/CN pencil, 1..ny+2 (0 restores the default:
+ * for the per-component pencils about ny/24 in steps of 4 within [8, 16]; for the
+ * three-component pencils 8 in float64, in float32 4 below ny = 256 and 6 from there).
  * TC_TUNE_THOMAS_SB: (kx, ky) systems per block of the wall-normal Poisson solve (16 lanes each,
  * 32 above 128 rows): 4, 8 or 16, at most 256 threads per block (0 restores the default, 16).
  * TC_TUNE_K1_TILE: form of the explicit + CN kernel: 0 one component per block, each wavefront
  * fetching its neighbour columns; 1 the three components of a band in one block with the
  * neighbour columns staged through LDS (float64 up to 128 rows, float32 up to 256; larger
- * columns, or a device with less LDS per block, run the first form).
+ * columns, or a device with less LDS per block, run the first form); 2 one wavefront per i
+ * marching u, v and w together (float64 up to 128 rows, float32 up to 256, else the first form).
+ * Default (-1, or never set): 2 where it is built and a lane holds two or more rows, else 0.
  * TC_TUNE_THOMAS_LDS: form of the wall-normal Poisson solve: 0 each lane loads its own rows; 1 (the
  * default) a block's systems are copied into LDS and back with consecutive 16-byte accesses
  * (where the rows are 16-byte vectors; otherwise the first form runs). */

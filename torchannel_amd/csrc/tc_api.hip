@@ -167,7 +167,7 @@ int tc_grid_set_tuning(tc_grid* h, int key, int value) {
       g.thomas_sb = value;
       return TC_OK;
     case TC_TUNE_K1_TILE:
-      if (value < 0 || value > 2) return TC_ERR_ARG;
+      if (value < -1 || value > 2) return TC_ERR_ARG;
       g.k1_tile = value;
       return TC_OK;
     case TC_TUNE_THOMAS_LDS:

@@ -180,6 +180,22 @@ __device__ __forceinline__ void rows_dn(T* buf, const T (&x)[R], T (&dn)[R], T l
   if (lane == 0) dn[0] = lo;
 }
 
+// rows_pm without the top override and rows_up without any: for columns whose row n is held by
+// the register rows themselves (the caller's row n is the buffer's next row)
+template <int R, typename T>
+__device__ __forceinline__ void rows_pm_lo(T* buf, const T (&x)[R], T (&dn)[R], T (&up)[R], T lo, int lane) {
+  rows_put<R, T>(buf, x, lane);
+  rows_get<-1, R, T>(buf, dn, lane);
+  rows_get<1, R, T>(buf, up, lane);
+  if (lane == 0) dn[0] = lo;
+}
+
+template <int R, typename T>
+__device__ __forceinline__ void rows_up_in(T* buf, const T (&x)[R], T (&up)[R], int lane) {
+  rows_put<R, T>(buf, x, lane);
+  rows_get<1, R, T>(buf, up, lane);
+}
+
 template <int R, typename T>
 __device__ __forceinline__ void rows_up(T* buf, const T (&x)[R], T (&up)[R], T hi, int n, int lane) {
   rows_put<R, T>(buf, x, lane);
@@ -718,7 +734,7 @@ struct K3Next {
   T uE[R], vW[R], vE[R], wW[R], wE[R];   // plane j+1: columns only step j+1 reads
   T uN[R], umN[R], vN[R], wN[R];         // plane j+2: u(i), u(i-1), v(i), w(i)
   T pu[R], pv[R], pw[R];                 // history at j+1
-  T ub0, ubn, ubz, umbz, vb0, vbn, vbz, wbz, wEb0;       // ghost / top rows at j+1
+  T ub0, ubn, vb0, vbn, wEb0;                          // ghost rows at j+1
   T wNb0;                                // w(i, j+2)[0]
 };
 
@@ -769,12 +785,8 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
     col_load<R, T>(wi + cw1 + d.syw, L.wN, n, lane);
     L.ub0 = ui[cu1];
     L.ubn = ui[cu1 + n + 1];
-    L.ubz = ui[cu1 + n];
-    L.umbz = ui[cu1 + n - d.sxu];
     L.vb0 = vi[cv1];
     L.vbn = vi[cv1 + n + 1];
-    L.vbz = vi[cv1 + n];
-    L.wbz = wi[cw1 + n];
     L.wEb0 = wi[cw1 + d.sxw];
     L.wNb0 = wi[cw1 + d.syw];
   };
@@ -784,7 +796,7 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
   col_load<R, T>(vi + (long long)(j0 - 1) * d.syv, vS, n, lane);
   col_load<R, T>(vi + d.sxv + (long long)(j0 - 1) * d.syv, v1m, n, lane);
   col_load<R, T>(wi + (long long)(j0 - 1) * d.syw, wS, n, lane);
-  rows_up<R, T>(cu.rows, vS, vm1, vi[(long long)(j0 - 1) * d.syv + n], nw, lane);
+  rows_up_in<R, T>(cu.rows, vS, vm1, lane);
   // plane j0: u(i-1), u(i), v(i), w(i) (the rest comes with fetch)
   T uC[R], uW[R], vC[R], wC[R];
   col_load<R, T>(ui + (long long)j0 * d.syu, uC, n, lane);
@@ -839,12 +851,13 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
       }
       cn_v = k1_rhs_ab2<T, R>(cv, rhs, vC, A.pv, rv + ov + cv1, nullptr, n, n + 2, true, xv);
       // w (k1_pencil_w, rows k = 1..nz-1): u00 = u(i,j), um0 = u(i-1,j), v00 = v(i,j), vm0 = v(i,j-1),
-      // each with its row above
+      // each with its row above; the pencil's top overrides (row nz of w, u and v) are the rows
+      // these columns hold in registers, so the plain exchanges give the same values
       T wb[R], wt[R], u01[R], um1[R], v01[R];
-      rows_pm<R, T>(cu.rows, wC, wb, wt, wC0, A.wbz, nw, lane);
-      rows_up<R, T>(cu.rows, uC, u01, A.ubz, nw, lane);
-      rows_up<R, T>(cu.rows, uW, um1, A.umbz, nw, lane);
-      rows_up<R, T>(cu.rows, vC, v01, A.vbz, nw, lane);
+      rows_pm_lo<R, T>(cu.rows, wC, wb, wt, wC0, lane);
+      rows_up_in<R, T>(cu.rows, uC, u01, lane);
+      rows_up_in<R, T>(cu.rows, uW, um1, lane);
+      rows_up_in<R, T>(cu.rows, vC, v01, lane);
       T rf2[R];
       k3_met<5, T, R>(met, rf2, lane);
 #pragma unroll
