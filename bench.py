@@ -322,6 +322,7 @@ def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
     replaying each channel's own 10-step graph."""
     import torch
     import torchannel_amd as tc
+    from torchannel_amd import _lib as L
 
     ens = tc.make_ensemble(tc.ensemble_configs(n))
     ens.step(0.005)
@@ -338,6 +339,12 @@ def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
     ms = e0.elapsed_time(e1)
     cells = ens.n_cells
     done = (steps // chunk) * chunk
+    # results after the timed steps: every channel finite (the device non-finite flag its
+    # projection kernel sets) and its bulk velocity held at the target by the forcing controller
+    scal = torch.stack([ch._scal for ch in ens.channels]).cpu()
+    check = {"all_finite": bool((scal[:, L.S_NONFINITE] == 0).all()),
+             "max_abs_u_bulk_error": float((scal[:, L.S_UBULK] - scal[:, L.S_UTARGET]).abs().max()),
+             "steps_per_channel": int(scal[:, L.S_STEP].min())}
     # looping the single-channel path
     chans = ens.channels
     loop_steps = 20
@@ -366,7 +373,8 @@ def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
             "loop_eager_ms_per_step": round(ms_loop, 4),
             "loop_graph_ms_per_step": round(ms_loop_graph, 4),
             "speedup_vs_loop_eager": round(ms_loop / ms_step, 2),
-            "speedup_vs_loop_graph": round(ms_loop_graph / ms_step, 2)}
+            "speedup_vs_loop_graph": round(ms_loop_graph / ms_step, 2),
+            "check": check}
 
 
 def _cpu_model():
