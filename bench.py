@@ -30,6 +30,7 @@ KERNEL_OF = {"explicit_ab2_cn": "k_explicit_cn_uvw", "div_fft_x": "k_fwd_x", "ff
              "poisson_thomas": "k_poisson_thomas", "fft_y_inv": "k_fft_y", "ifft_x_p": "k_inv_x",
              "project_bc_bulk": "k_project_bc", "cfl_max": "k_cfl"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "imex_step_192x192x128_hbm_traffic.json")
+LAYOUT = "dense"      # the solver's array layout (compute.layout), --layout
 
 
 def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
@@ -40,7 +41,7 @@ def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
         "time": {"dt": 0.005, "n_steps": steps, "CFL_target": 0.25, "dt_update_interval": 10,
                  "dt_max": 0.01, "dt_min": 1e-4},
         "initialization": {"type": "parabolic", "perturbation_intensity": 0.1, "seed": 0},
-        "compute": {"device": "cuda", "precision": precision},
+        "compute": {"device": "cuda", "precision": precision, "layout": LAYOUT},
         "output": {"save_init": False},
         "statistics": {"n_stats": 200, "t_stats": 0.0, "z_plus_target": 15.0},
     }
@@ -177,6 +178,7 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
         per[name] = {"avg_us": avg_ms * 1e3, "alg_bytes": byts,
                      "gbs": byts / (avg_ms * 1e-3) / 1e9 if byts else 0.0}
     out["kernels"] = per
+    out["pitch"] = ch.pitch
     out["k1_form"] = ch._grid.get_tuning(L.TUNE_K1_TILE)
     out["k1_jb"] = ch._grid.get_tuning(L.TUNE_K1_JB)
     step_kernels = [k for k in PHASES if k not in ("cfl_max",)]
@@ -190,29 +192,32 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
     return out
 
 
-def column_ceiling(precision, nx, ny, nz, jb, reps=20):
+def column_ceiling(precision, nx, ny, nz, jb, pitch, reps=20):
     """Bandwidth of the explicit + CN kernel's column access patterns without its arithmetic
     (tc_calib_columns): the three-component pencils' pattern (the form the kernel runs at these
     sizes), the per-component pencils' pattern with their neighbour-column loads, and the
     compulsory streams alone; GB/s of the kernel's 12 algorithmic words per cell, from HIP events
-    on the launch stream.  jb: the pencil length of the three-component form."""
+    on the launch stream.  jb: the pencil length of the three-component form; pitch: the column
+    stride of the solver's arrays (0 dense), which the twelve arrays here share."""
     import ctypes
     import torch
     from torchannel_amd import _lib as L
 
     dt = torch.float64 if precision == "fp64" else torch.float32
-    arrs = [torch.rand(nx + 2, ny + 2, nz + 2, dtype=dt, device="cuda") for _ in range(12)]
+    arrs = [L.empty_field((nx + 2, ny + 2, nz + 2), pitch, dt, "cuda") for _ in range(12)]
+    for a in arrs:
+        a.uniform_()
     ptrs = (ctypes.c_void_p * 12)(*[a.data_ptr() for a in arrs])
     st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    out = {}
+    out = {"pitch": pitch}
     for pat, name, pjb in ((2, "k1_pattern", jb), (0, "pencil_pattern", 8), (1, "compulsory", 8)):
         for _ in range(3):
-            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, pjb, ptrs, st)
+            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, pjb, pitch, ptrs, st)
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
         for _ in range(reps):
-            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, pjb, ptrs, st)
+            L.call("tc_calib_columns", L.dtype_code(dt), pat, nx, ny, nz, pjb, pitch, ptrs, st)
         e1.record()
         torch.cuda.synchronize()
         us = e0.elapsed_time(e1) * 1e3 / reps
@@ -304,14 +309,14 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
     for name, avg in phase_times(ch, phase_reps).items():
         byts = words[name] * 4 * n_cells
         per[name] = {"avg_us": round(avg * 1e3, 2), "gbs": round(byts / (avg * 1e-3) / 1e9, 1) if byts else 0.0}
-    jb = ch._grid.get_tuning(L.TUNE_K1_JB)
+    jb, pitch = ch._grid.get_tuning(L.TUNE_K1_JB), ch.pitch
     del ch
     torch.cuda.empty_cache()
     return {"workload": "384x384x256, Lx=2pi, Ly=pi, Re_bulk=10000, float32, adaptive dt CFL 0.25 (first dt "
                         f"from the CFL rule), {(steps // chunk) * chunk} steps", "finite": finite,
             "value": round(n_cells / (ms * 1e-3) / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(ms, 4),
             "dt_final": dt_final, "u_bulk": u_bulk, "kernels": per,
-            "column_pattern_ceiling": column_ceiling("fp32", nx, ny, nz, jb)}
+            "column_pattern_ceiling": column_ceiling("fp32", nx, ny, nz, jb, pitch)}
 
 
 def run_ensemble(n=48, steps=300, warmup=10, chunk=10):
@@ -436,6 +441,8 @@ def main():
     ap.add_argument("--c3-only", action="store_true",
                     help="run only the 384x384x256 leg (for profiling it on its own)")
     ap.add_argument("--c3-steps", type=int, default=500)
+    ap.add_argument("--layout", default="dense", choices=["dense", "padded"],
+                    help="the solver's array layout (compute.layout); dense is the default")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
                          "kernel for scaling HBM byte counters")
@@ -449,6 +456,9 @@ def main():
         print("bench.py: --steps must be >= 1 and --warmup >= 0", flush=True)
         sys.exit(2)
     steps, warmup = args.steps, args.warmup
+    global LAYOUT
+    LAYOUT = args.layout
+    print(f"[bench] layout {LAYOUT}", file=sys.stderr, flush=True)
     print(f"[bench] {args.precision} {NX}x{NY}x{NZ} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
     if args.copy_calib_mb > 0:
         import torch
@@ -470,7 +480,7 @@ def main():
     if not args.no_fp32 and args.precision == "fp64":
         other = run_gpu("fp32", steps, warmup)
         print(f"[bench] fp32: {other['mcell_steps_per_s']:.1f} Mcell-steps/s", file=sys.stderr, flush=True)
-    ceiling = column_ceiling(args.precision, NX, NY, NZ, main_run["k1_jb"])
+    ceiling = column_ceiling(args.precision, NX, NY, NZ, main_run["k1_jb"], main_run["pitch"])
     print(f"[bench] column pattern: {ceiling['k1_pattern']['gbs']} GB/s, compulsory streams "
           f"{ceiling['compulsory']['gbs']} GB/s", file=sys.stderr, flush=True)
     c3 = None
@@ -514,7 +524,8 @@ def main():
         "config": {"workload": "Re_tau~180 production step: 192x192x128, Lx=4pi, Ly=4pi/3, gamma=2.6, "
                                "Re_bulk=2870, adaptive dt CFL 0.25 every 10 steps, constant flow rate, "
                                "turbstats every 200 steps",
-                   "precision": args.precision, "graph_chunk_steps": 10},
+                   "precision": args.precision, "graph_chunk_steps": 10, "layout": LAYOUT,
+                   "column_pitch": main_run["pitch"]},
         "roofline": {"bound": "hbm", "kernel": dom, "k1_form": main_run["k1_form"], "k1_jb": main_run["k1_jb"],
                      "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -87,6 +87,18 @@ int tc_grid_destroy(tc_grid* g);
 int tc_grid_set_tuning(tc_grid* g, int key, int value);
 int tc_grid_get_tuning(const tc_grid* g, int key, int* value);
 
+/* Column stride of the arrays a grid's calls take (no counterpart in the reference, whose arrays
+ * are always dense torch.zeros tensors, initflow.py:6-9).  pitch 0 (the default): each array is
+ * dense, its columns nz+2 (u, v, p) or nz+1 (w) elements apart.  pitch >= nz+2: the columns of
+ * u, v, w and p all sit `pitch` elements apart, with the same (i, j, k) indexing; element
+ * [i][j][k] is at i*(columns per plane)*pitch + j*pitch + k.  With pitch a multiple of 128 bytes
+ * and row k = 1 of the first column on a 128-byte line, every column's interior starts on a
+ * line, which the column kernels stream faster (ChannelFlow's compute.layout 'padded').  Buffers the
+ * library lays out itself (a solver's post-CN velocities, the adjoint's arrays) follow the
+ * pitch.  Set it before a solver is created on the grid (TC_ERR_STATE after). */
+int tc_grid_set_pitch(tc_grid* g, int pitch);
+int tc_grid_get_pitch(const tc_grid* g, int* pitch);
+
 /* apply_bc_all (solver.py:23-79): periodic x/y images, then wall reflections, in place. */
 int tc_apply_bc(const tc_grid* g, void* u, void* v, void* w, void* stream);
 
@@ -235,14 +247,16 @@ int tc_solver_set_newest(tc_solver* s, int newest);
 
 /* ---- roofline calibration ------------------------------------------------------------------
  * No counterpart in the reference.  Sweeps twelve arrays shaped like p ((nx+2) x (ny+2) x
- * (nz+2), arrays[0..5] read, arrays[6..11] written) in the column order of the explicit + CN
+ * (nz+2), columns `pitch` elements apart, 0 meaning nz+2; arrays[0..5] read, arrays[6..11]
+ * written) in the column order of the explicit + CN
  * kernel without its arithmetic: pattern 0 moves the per-component pencils' eight input and two
  * output columns per column step (neighbour columns included), pattern 1 only the compulsory
  * streams (one input column and the AB2 history in, two columns out), pattern 2 the
  * three-component pencils' nine input, three history and six output columns per (i, j) step.
  * Timed beside the kernel, it gives the bandwidth each access pattern reaches on the device.
  * nz <= 256. */
-int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, void* const* arrays, void* stream);
+int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, int pitch, void* const* arrays,
+                     void* stream);
 
 /* ---- discrete adjoint --------------------------------------------------------------------
  * No counterpart in the reference.  Gradient of J = sum_n weight_n * tau_w(u_n) with respect to

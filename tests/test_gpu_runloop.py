@@ -93,7 +93,7 @@ def test_statistics_share_the_solver_plan(tc, tmp_path):
     c["statistics"] = {"n_stats": 10, "t_stats": 0.0, "z_plus_target": 15.0}
     ch = tc.ChannelFlow(config=c)
     assert L.device_key("cuda") == L.device_key(ch.u.device)
-    assert ch.turbulence_stats._plan(ch.u.dtype, ch.u.device).value == ch._grid.poisson().value
+    assert ch.turbulence_stats._plan(ch.u.dtype, ch.u.device, ch.pitch).value == ch._grid.poisson().value
     n = len(L._grid_cache)
     ch.step_imex(0.005)
     ch.turbulence_stats.accumulate_statistics(ch.u, ch.v, ch.w)

@@ -35,6 +35,8 @@ SIGNATURES = {
     "tc_grid_create": [_i, _i, _i, _d, _d, _d, _i, _i, _pd, _pd, ctypes.POINTER(_vp)],
     "tc_grid_destroy": [_vp],
     "tc_grid_set_tuning": [_vp, _i, _i],
+    "tc_grid_set_pitch": [_vp, _i],
+    "tc_grid_get_pitch": [_vp, ctypes.POINTER(_i)],
     "tc_solver_set_newest": [_vp, _i],
     "tc_adjoint_create": [_vp, _d, ctypes.POINTER(_vp)],
     "tc_adjoint_destroy": [_vp],
@@ -75,7 +77,7 @@ SIGNATURES = {
     "tc_solver_capture": [_vp, _i, _i, _d, _d, _d, _vp],
     "tc_solver_replay": [_vp, _vp],
     "tc_solver_phase": [_vp, _i, _i, _vp],
-    "tc_calib_columns": [_i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp), _vp],
+    "tc_calib_columns": [_i, _i, _i, _i, _i, _i, _i, ctypes.POINTER(_vp), _vp],
 }
 
 _lib = None
@@ -121,15 +123,16 @@ def stream_of(t: torch.Tensor):
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-def require_device(*tensors):
-    """The product path runs only on HIP devices, on contiguous float32/float64 tensors."""
+def require_device(*tensors, dense=True):
+    """The product path runs only on HIP devices, on float32/float64 tensors, contiguous unless
+    `dense` is False (field arrays, whose layout require_fields checks)."""
     dt = None
     for t in tensors:
         if t is None:
             continue
         if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
             raise RuntimeError("torchannel_amd: tensors must live on a HIP device (no CPU path)")
-        if not t.is_contiguous():
+        if dense and not t.is_contiguous():
             raise RuntimeError("torchannel_amd: tensors must be contiguous (reference C-order layout)")
         if t.dtype not in (torch.float64, torch.float32):
             raise RuntimeError(f"torchannel_amd: unsupported dtype {t.dtype}")
@@ -139,6 +142,83 @@ def require_device(*tensors):
             raise RuntimeError("torchannel_amd: mixed dtypes")
     load()
     return dt
+
+
+LINE_BYTES = 128      # cache-line size the padded layout aligns every column's row 1 to
+
+
+def padded_pitch(nz, dtype) -> int:
+    """Column stride (elements) of the padded layout: nz + 2 rounded up to whole 128-byte lines."""
+    a = LINE_BYTES // torch.empty((), dtype=dtype).element_size()
+    return -(-(nz + 2) // a) * a
+
+
+def empty_field(shape, pitch, dtype, device, zero=True):
+    """A ghosted array of `shape` (planes, columns, rows).  pitch 0: a dense tensor.  pitch > 0:
+    a strided view whose columns sit `pitch` elements apart, with row 1 of every column on a
+    128-byte line; it indexes like the dense array, and the storage behind the view extends at
+    least one full pitch past every column (flat passes of the library cover
+    planes * columns * pitch elements)."""
+    n0, n1, n2 = (int(x) for x in shape)
+    if not pitch:
+        return (torch.zeros if zero else torch.empty)((n0, n1, n2), dtype=dtype, device=device)
+    if pitch < n2:
+        raise ValueError(f"pitch {pitch} is shorter than the column length {n2}")
+    esz = torch.empty((), dtype=dtype).element_size()
+    a = LINE_BYTES // esz
+    if (pitch * esz) % LINE_BYTES:
+        raise ValueError(f"pitch {pitch} is not a whole number of {LINE_BYTES}-byte lines")
+    buf = (torch.zeros if zero else torch.empty)(n0 * n1 * pitch + 2 * a, dtype=dtype, device=device)
+    off = (-(buf.data_ptr() // esz + 1)) % a             # element off + 1 on a line
+    return buf.as_strided((n0, n1, n2), (n1 * pitch, pitch, 1), buf.storage_offset() + off)
+
+
+def field_pitch(t) -> int:
+    """0 for a contiguous (dense) array, else its column stride; raises for other layouts."""
+    if t.is_contiguous():
+        return 0
+    if t.dim() == 3:
+        P = t.stride(1)
+        if t.stride() == (t.shape[1] * P, P, 1) and P >= t.shape[2]:
+            return P
+    raise RuntimeError("torchannel_amd: field arrays must be dense or have columns at one stride "
+                       "(the padded layout)")
+
+
+def field_like(t, zero=False):
+    """A new array with t's shape, dtype, device and layout (dense or padded)."""
+    P = field_pitch(t)
+    if P == 0:
+        return torch.zeros_like(t) if zero else torch.empty_like(t)
+    return empty_field(t.shape, P, t.dtype, t.device, zero=zero)
+
+
+def clone_field(t):
+    out = field_like(t)
+    out.copy_(t)
+    return out
+
+
+def clone_field_as(src, like):
+    """A copy of src in the layout of `like` (same shape)."""
+    out = field_like(like)
+    out.copy_(src)
+    return out
+
+
+def require_fields(*tensors):
+    """(dtype, pitch) of ghosted field arrays passed to one call: all dense (pitch 0), or all with
+    one column stride P (a contiguous array whose column length is P counts as that layout)."""
+    ts = [t for t in tensors if t is not None]
+    dt = require_device(*ts, dense=False)
+    pitches = {field_pitch(t) for t in ts}
+    if pitches <= {0}:
+        return dt, 0
+    P = max(pitches)
+    for t in ts:
+        if field_pitch(t) not in (0, P) or (field_pitch(t) == 0 and t.shape[2] != P):
+            raise RuntimeError("torchannel_amd: the field arrays of one call must share one layout")
+    return dt, P
 
 
 def dtype_code(dt):
@@ -154,11 +234,12 @@ def top_code(top_wall_bc_type: str) -> int:
 class Grid:
     """Owns a tc_grid (extents, spacings, metric tables on the device)."""
 
-    def __init__(self, nx, ny, nz, dx, dy, nu, top, dtype, dz_f_host, dz_c_host, device):
+    def __init__(self, nx, ny, nz, dx, dy, nu, top, dtype, dz_f_host, dz_c_host, device, pitch=0):
         lib = load()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.dtype = dtype
         self.device = device
+        self.pitch = int(pitch)
         f = (ctypes.c_double * nz)(*dz_f_host)
         c = (ctypes.c_double * (nz + 1))(*dz_c_host)
         h = ctypes.c_void_p()
@@ -166,6 +247,8 @@ class Grid:
             check(lib.tc_grid_create(nx, ny, nz, float(dx), float(dy), float(nu), top, dtype_code(dtype),
                                      f, c, ctypes.byref(h)))
         self.handle = h
+        if self.pitch:
+            check(lib.tc_grid_set_pitch(h, self.pitch))
         self._poisson = None
 
     def set_tuning(self, key, value):
@@ -224,18 +307,19 @@ def device_key(device) -> str:
     return str(d)
 
 
-def get_grid(nx, ny, nz, dx, dy, nu, top, dtype, device, dz_f=None, dz_c=None) -> Grid:
-    """Cached Grid for these arguments.  The metric tensors are identified by object (weak
-    reference) and version counter, so an in-place change or a new tensor rebuilds the grid."""
+def get_grid(nx, ny, nz, dx, dy, nu, top, dtype, device, dz_f=None, dz_c=None, pitch=0) -> Grid:
+    """Cached Grid for these arguments (pitch: the column stride of the arrays its calls take, 0
+    for dense arrays).  The metric tensors are identified by object (weak reference) and version
+    counter, so an in-place change or a new tensor rebuilds the grid."""
     key = (int(nx), int(ny), int(nz), float(dx), float(dy), float(nu), int(top), dtype, device_key(device),
-           id(dz_f), getattr(dz_f, "_version", -1), id(dz_c), getattr(dz_c, "_version", -1))
+           id(dz_f), getattr(dz_f, "_version", -1), id(dz_c), getattr(dz_c, "_version", -1), int(pitch))
     ent = _grid_cache.get(key)
     if ent is not None:
         g, rf, rc = ent
         if (dz_f is None or rf() is dz_f) and (dz_c is None or rc() is dz_c):
             return g
     g = Grid(int(nx), int(ny), int(nz), dx, dy, nu, top, dtype, _metric_host(dz_f, nz),
-             _metric_host(dz_c, nz + 1), device)
+             _metric_host(dz_c, nz + 1), device, pitch=pitch)
     rf = weakref.ref(dz_f) if dz_f is not None else (lambda: None)
     rc = weakref.ref(dz_c) if dz_c is not None else (lambda: None)
     if len(_grid_cache) > 64:

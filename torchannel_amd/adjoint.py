@@ -76,15 +76,22 @@ class WallShearAdjoint:
         ch._newest = newest.value
 
     def _check_f(self, f):
+        """f in the solver's layout (a copy when the solver's arrays are padded)."""
         ch = self.ch
-        if f.shape != ch.u.shape or f.dtype != torch.float64 or f.device != ch.u.device or not f.is_contiguous():
-            raise ValueError(f"body force must be a contiguous float64 tensor of shape {tuple(ch.u.shape)} "
+        if f.shape != ch.u.shape or f.dtype != torch.float64 or f.device != ch.u.device:
+            raise ValueError(f"body force must be a float64 tensor of shape {tuple(ch.u.shape)} "
                              f"on {ch.u.device}")
+        if L.field_pitch(ch.u) == 0 and f.is_contiguous():
+            return f
+        key = (f.data_ptr(), f._version)
+        if getattr(self, "_f_key", None) != key:
+            self._f_key, self._f_lay = key, L.clone_field_as(f, ch.u)
+        return self._f_lay
 
     # -- objective and gradient --------------------------------------------------------------
     def objective(self, f):
         """J(f) as a 0-d float64 device tensor; keeps the checkpoints for gradient(f)."""
-        self._check_f(f)
+        fl = self._check_f(f)
         ch = self.ch
         self._restore(self._init)
         J = torch.zeros(1, dtype=torch.float64, device=ch.u.device)
@@ -93,20 +100,20 @@ class WallShearAdjoint:
         for n in range(self.n_steps):
             if n % self.k == 0:
                 ckpts.append(self._snapshot())
-            self._step(f)
+            self._step(fl)
             L.call("tc_adjoint_objective", self._h, L.ptr(ch.u), w, L.ptr(J), ch._stream())
         self._ckpts = ckpts
         self._ckpt_f = (f.data_ptr(), f._version)
         return J[0]
 
     def gradient(self, f):
-        """dJ/df (shaped like u; zero on ghost entries) by the reverse sweep."""
-        self._check_f(f)
+        """dJ/df (shaped like u, dense; zero on ghost entries) by the reverse sweep."""
+        fl = self._check_f(f)
         if self._ckpts is None or self._ckpt_f != (f.data_ptr(), f._version):
             self.objective(f)
         ch = self.ch
         st = ch._stream()
-        grad = torch.zeros_like(ch.u)
+        grad = L.field_like(ch.u, zero=True)
         w = 1.0 / self.n_steps
         fresh = self._init["newest"] < 0
         L.call("tc_adjoint_begin", self._h, self.dt, w, L.ptr(grad), st)
@@ -116,14 +123,14 @@ class WallShearAdjoint:
             self._restore(self._ckpts[seg])
             states = []
             for _ in range(n0, n1):
-                states.append([q.clone() for q in (ch.u, ch.v, ch.w)])
-                self._step(f)
+                states.append([L.clone_field(q) for q in (ch.u, ch.v, ch.w)])
+                self._step(fl)
             for n in range(n1 - 1, n0 - 1, -1):
                 q = states[n - n0]
                 first = 1 if (n == 0 and fresh) else 0
                 L.call("tc_adjoint_step", self._h, L.ptr(q[0]), L.ptr(q[1]), L.ptr(q[2]), first,
                        w if n >= 1 else 0.0, L.ptr(grad), st)
-        return grad
+        return grad.contiguous()
 
 
 class _WallShear(torch.autograd.Function):

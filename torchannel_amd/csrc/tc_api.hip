@@ -200,6 +200,22 @@ int tc_grid_get_tuning(const tc_grid* h, int key, int* value) {
   }
 }
 
+int tc_grid_set_pitch(tc_grid* h, int pitch) {
+  if (!h) return TC_ERR_ARG;
+  Grid& g = h->g;
+  if (pitch != 0 && pitch < g.d.nz + 2) return TC_ERR_ARG;
+  if (g.has_solver) return TC_ERR_STATE;
+  g.pitch = pitch;
+  g.d = make_dims(g.d.nx, g.d.ny, g.d.nz, g.d.top > 0 ? TC_TOP_NEUMANN : TC_TOP_DIRICHLET, pitch);
+  return TC_OK;
+}
+
+int tc_grid_get_pitch(const tc_grid* h, int* pitch) {
+  if (!h || !pitch) return TC_ERR_ARG;
+  *pitch = h->g.pitch;
+  return TC_OK;
+}
+
 int tc_apply_bc(const tc_grid* h, void* u, void* v, void* w, void* stream) {
   if (!h || !u || !v || !w) return TC_ERR_ARG;
   return launch_apply_bc(&h->g, u, v, w, (hipStream_t)stream);
@@ -346,9 +362,7 @@ static size_t ws_align(size_t b) { return (b + 255) & ~(size_t)255; }
 static size_t solver_ws_parts(const Grid& g, size_t sizes[7]) {
   const Dims& d = g.d;
   const size_t esz = g.dtype == TC_F64 ? 8 : 4;
-  sizes[0] = esz * (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
-  sizes[1] = esz * (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
-  sizes[2] = esz * (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
+  for (int c = 0; c < 3; ++c) sizes[c] = esz * ((size_t)g.lead() + field_span(d, c));
   sizes[3] = sizeof(double) * 2 * TC_TAB_STRIDE;
   sizes[4] = sizeof(double) * (size_t)k5_blocks(&g);
   sizes[5] = sizeof(double) * (size_t)std::max(kMaxRedBlocks, kCflBlocks);
@@ -377,6 +391,7 @@ int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w
   tc_solver* s = new (std::nothrow) tc_solver();
   if (!s) return TC_ERR_NOMEM;
   const Grid& g = h->g;
+  g.has_solver = true;             // the workspace layout now depends on the grid's pitch
   s->g = &g;
   s->pl = &pl->p;
   s->u = u; s->v = v; s->w = w; s->p = p;
@@ -395,7 +410,8 @@ int tc_solver_create(const tc_grid* h, tc_poisson* pl, void* u, void* v, void* w
     parts[q] = cur;
     cur += ws_align(sizes[q]);
   }
-  s->su = parts[0]; s->sv = parts[1]; s->sw = parts[2];
+  const size_t lead_bytes = (size_t)g.lead() * (g.dtype == TC_F64 ? 8 : 4);
+  s->su = (char*)parts[0] + lead_bytes; s->sv = (char*)parts[1] + lead_bytes; s->sw = (char*)parts[2] + lead_bytes;
   s->tabs = (double*)parts[3]; s->partials = (double*)parts[4]; s->cfl_partials = (double*)parts[5];
   s->init_scal = (double*)parts[6];
   bool ok = hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking) == hipSuccess;
@@ -467,7 +483,7 @@ int tc_solver_rescale(tc_solver* s, void* stream) {
   const Grid* g = s->g;
   const Dims& d = g->d;
   hipStream_t st = (hipStream_t)stream;
-  const size_t nu_ = (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
+  const size_t nu_ = field_span(d, 0);
   // u_b of the initial field into scal[TC_S_UBULK] (cell volume dx*dy*dz_f expanded, solver.py:231)
   TC_RC(launch_bulk(g, s->u, g->wbulk, 0, 0, 1, s->inv_volume, s->scal + TC_S_UBULK, g->scratch,
                     kMaxRedBlocks, st));
@@ -483,9 +499,7 @@ int tc_solver_project_initial(tc_solver* s, int rescale, void* stream) {
   const Grid* g = s->g;
   const Dims& d = g->d;
   hipStream_t st = (hipStream_t)stream;
-  const size_t nu_ = (size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2);
-  const size_t nv_ = (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2);
-  const size_t nw_ = (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1);
+  const size_t nu_ = field_span(d, 0), nv_ = field_span(d, 1), nw_ = field_span(d, 2);
   const size_t esz = g->dtype == TC_F64 ? 8 : 4;
   if (rescale) TC_RC(tc_solver_rescale(s, stream));
   TC_HIP(hipMemcpyAsync(s->su, s->u, esz * nu_, hipMemcpyDeviceToDevice, st));
@@ -606,6 +620,7 @@ struct tc_adjoint {
   double* one_scal = nullptr;                   // dt = 1 block for the projection
   double* partials = nullptr;
   double* tpart = nullptr;
+  void* raw[18] = {};                           // the allocations behind the 18 arrays above
 };
 
 int tc_adjoint_create(tc_solver* s, double wall_dist, tc_adjoint** out) {
@@ -619,12 +634,17 @@ int tc_adjoint_create(tc_solver* s, double wall_dist, tc_adjoint** out) {
   const Dims& d = g->d;
   a->obj_top = d.top < 0 ? 1 : 0;
   a->tau_coef = (a->obj_top ? 0.5 : 1.0) * g->nu / wall_dist;
-  const size_t n3[3] = {(size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2), (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2),
-                        (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1)};
+  // each array laid out like the state (the grid's pitch), row k = 1 on a 128-byte line
+  const size_t lead = (size_t)g->lead();
   bool ok = true;
   for (int c = 0; c < 3 && ok; ++c) {
+    const size_t bytes = 8 * (lead + field_span(d, c));
     void** bufs[6] = {&a->lq[c], &a->qs[c], &a->lr[c], &a->acc[c], &a->e[c], &a->z[c]};
-    for (void** b : bufs) ok = ok && hipMalloc(b, 8 * n3[c]) == hipSuccess && hipMemset(*b, 0, 8 * n3[c]) == hipSuccess;
+    for (int q = 0; q < 6 && ok; ++q) {
+      void** r = &a->raw[6 * c + q];
+      ok = hipMalloc(r, bytes) == hipSuccess && hipMemset(*r, 0, bytes) == hipSuccess;
+      if (ok) *bufs[q] = (double*)*r + lead;
+    }
   }
   ok = ok && hipMalloc(&a->ascal, sizeof(double) * TC_A_NSCAL) == hipSuccess &&
        hipMalloc(&a->one_scal, sizeof(double) * TC_NSCAL) == hipSuccess &&
@@ -645,9 +665,7 @@ int tc_adjoint_create(tc_solver* s, double wall_dist, tc_adjoint** out) {
 
 int tc_adjoint_destroy(tc_adjoint* a) {
   if (!a) return TC_OK;
-  for (int c = 0; c < 3; ++c) {
-    hipFree(a->lq[c]); hipFree(a->qs[c]); hipFree(a->lr[c]); hipFree(a->acc[c]); hipFree(a->e[c]); hipFree(a->z[c]);
-  }
+  for (void* r : a->raw) hipFree(r);
   hipFree(a->ascal); hipFree(a->one_scal); hipFree(a->partials); hipFree(a->tpart);
   delete a;
   return TC_OK;
@@ -666,8 +684,7 @@ int tc_adjoint_begin(tc_adjoint* a, double dt, double obj_weight, void* grad, vo
   const Dims& d = g->d;
   hipStream_t st = (hipStream_t)stream;
   a->dt = dt;
-  const size_t n3[3] = {(size_t)(d.nx + 1) * (d.ny + 2) * (d.nz + 2), (size_t)(d.nx + 2) * (d.ny + 1) * (d.nz + 2),
-                        (size_t)(d.nx + 2) * (d.ny + 2) * (d.nz + 1)};
+  const size_t n3[3] = {field_span(d, 0), field_span(d, 1), field_span(d, 2)};
   for (int c = 0; c < 3; ++c) {
     TC_HIP(hipMemsetAsync(a->acc[c], 0, 8 * n3[c], st));
     TC_HIP(hipMemsetAsync(a->qs[c], 0, 8 * n3[c], st));

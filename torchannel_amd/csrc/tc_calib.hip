@@ -2,7 +2,8 @@
 // the explicit + CN kernel without its arithmetic, so that the kernel's achieved bandwidth can be
 // set against what that pattern itself reaches on the device, next to the 8 TB/s HBM peak.
 //
-// Twelve arrays shaped like p ((nx+2) x (ny+2) x (nz+2), z contiguous): A0..A2 stand for u, v, w,
+// Twelve arrays shaped like p ((nx+2) x (ny+2) x (nz+2), z contiguous, columns `pitch` elements
+// apart): A0..A2 stand for u, v, w,
 // P0..P2 for the AB2 history (read), O0..O2 for the rhs and S0..S2 for the star velocities
 // (written).  Blocks of 4 wavefronts own one component and a band of 4 consecutive i; each
 // wavefront marches its (component, i) column along j over jb columns, fetching one column ahead
@@ -42,13 +43,13 @@ struct CalIn {
 };
 
 template <typename T, int R, int PAT>
-__global__ __launch_bounds__(256) void k_calib_columns(int nx, int ny, int nz, int jb, const T* __restrict__ A0,
+__global__ __launch_bounds__(256) void k_calib_columns(int nx, int ny, int nz, int jb, int pitch, const T* __restrict__ A0,
                                                        const T* __restrict__ A1, const T* __restrict__ A2,
                                                        const T* __restrict__ P0, const T* __restrict__ P1,
                                                        const T* __restrict__ P2, T* __restrict__ O0,
                                                        T* __restrict__ O1, T* __restrict__ O2, T* __restrict__ S0,
                                                        T* __restrict__ S1, T* __restrict__ S2) {
-  const long long sy = nz + 2, sx = (long long)(ny + 2) * (nz + 2);
+  const long long sy = pitch, sx = (long long)(ny + 2) * pitch;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int comp = b % 3, band = b / 3;
   const int nbj = (ny + jb - 1) / jb;
@@ -104,13 +105,13 @@ __global__ __launch_bounds__(256) void k_calib_columns(int nx, int ny, int nz, i
 
 
 template <typename T, int R>
-__global__ __launch_bounds__(256) void k_calib_uvw(int nx, int ny, int nz, int jb, const T* __restrict__ A0,
+__global__ __launch_bounds__(256) void k_calib_uvw(int nx, int ny, int nz, int jb, int pitch, const T* __restrict__ A0,
                                                    const T* __restrict__ A1, const T* __restrict__ A2,
                                                    const T* __restrict__ P0, const T* __restrict__ P1,
                                                    const T* __restrict__ P2, T* __restrict__ O0,
                                                    T* __restrict__ O1, T* __restrict__ O2, T* __restrict__ S0,
                                                    T* __restrict__ S1, T* __restrict__ S2) {
-  const long long sy = nz + 2, sx = (long long)(ny + 2) * (nz + 2);
+  const long long sy = pitch, sx = (long long)(ny + 2) * pitch;
   const int b = xcd_remap(blockIdx.x, gridDim.x);
   const int nbj = (ny + jb - 1) / jb;
   const int bi = b / nbj, bj = b - bi * nbj;
@@ -170,17 +171,18 @@ __global__ __launch_bounds__(256) void k_calib_uvw(int nx, int ny, int nz, int j
 }
 
 template <typename T, int R>
-static void calib_launch(int pattern, int blocks, hipStream_t st, int nx, int ny, int nz, int jb, void* const* a) {
+static void calib_launch(int pattern, int blocks, hipStream_t st, int nx, int ny, int nz, int jb, int pitch,
+                         void* const* a) {
   if (pattern == 2)
-    hipLaunchKernelGGL((k_calib_uvw<T, R>), dim3(blocks / 3), dim3(256), 0, st, nx, ny, nz, jb, (const T*)a[0],
+    hipLaunchKernelGGL((k_calib_uvw<T, R>), dim3(blocks / 3), dim3(256), 0, st, nx, ny, nz, jb, pitch, (const T*)a[0],
                        (const T*)a[1], (const T*)a[2], (const T*)a[3], (const T*)a[4], (const T*)a[5], (T*)a[6],
                        (T*)a[7], (T*)a[8], (T*)a[9], (T*)a[10], (T*)a[11]);
   else if (pattern == 0)
-    hipLaunchKernelGGL((k_calib_columns<T, R, 0>), dim3(blocks), dim3(256), 0, st, nx, ny, nz, jb, (const T*)a[0],
+    hipLaunchKernelGGL((k_calib_columns<T, R, 0>), dim3(blocks), dim3(256), 0, st, nx, ny, nz, jb, pitch, (const T*)a[0],
                        (const T*)a[1], (const T*)a[2], (const T*)a[3], (const T*)a[4], (const T*)a[5], (T*)a[6],
                        (T*)a[7], (T*)a[8], (T*)a[9], (T*)a[10], (T*)a[11]);
   else
-    hipLaunchKernelGGL((k_calib_columns<T, R, 1>), dim3(blocks), dim3(256), 0, st, nx, ny, nz, jb, (const T*)a[0],
+    hipLaunchKernelGGL((k_calib_columns<T, R, 1>), dim3(blocks), dim3(256), 0, st, nx, ny, nz, jb, pitch, (const T*)a[0],
                        (const T*)a[1], (const T*)a[2], (const T*)a[3], (const T*)a[4], (const T*)a[5], (T*)a[6],
                        (T*)a[7], (T*)a[8], (T*)a[9], (T*)a[10], (T*)a[11]);
 }
@@ -189,10 +191,12 @@ static void calib_launch(int pattern, int blocks, hipStream_t st, int nx, int ny
 
 using namespace tc;
 
-extern "C" int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, void* const* arrays,
-                                void* stream) {
+extern "C" int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, int jb, int pitch,
+                                void* const* arrays, void* stream) {
   if (!arrays || nx < 1 || ny < 1 || nz < 1 || nz > TC_MAX_NZ || jb < 1 || pattern < 0 || pattern > 2)
     return TC_ERR_ARG;
+  if (pitch == 0) pitch = nz + 2;
+  if (pitch < nz + 2) return TC_ERR_ARG;
   for (int q = 0; q < 12; ++q)
     if (!arrays[q]) return TC_ERR_ARG;
   if (dtype != TC_F64 && dtype != TC_F32) return TC_ERR_DTYPE;
@@ -201,10 +205,10 @@ extern "C" int tc_calib_columns(int dtype, int pattern, int nx, int ny, int nz, 
   hipStream_t st = (hipStream_t)stream;
 #define TC_CAL(TT)                                                              \
   switch (R) {                                                                  \
-    case 1: calib_launch<TT, 1>(pattern, blocks, st, nx, ny, nz, jb, arrays); break; \
-    case 2: calib_launch<TT, 2>(pattern, blocks, st, nx, ny, nz, jb, arrays); break; \
-    case 3: calib_launch<TT, 3>(pattern, blocks, st, nx, ny, nz, jb, arrays); break; \
-    default: calib_launch<TT, 4>(pattern, blocks, st, nx, ny, nz, jb, arrays); break; \
+    case 1: calib_launch<TT, 1>(pattern, blocks, st, nx, ny, nz, jb, pitch, arrays); break; \
+    case 2: calib_launch<TT, 2>(pattern, blocks, st, nx, ny, nz, jb, pitch, arrays); break; \
+    case 3: calib_launch<TT, 3>(pattern, blocks, st, nx, ny, nz, jb, pitch, arrays); break; \
+    default: calib_launch<TT, 4>(pattern, blocks, st, nx, ny, nz, jb, pitch, arrays); break; \
   }
   if (dtype == TC_F64) { TC_CAL(double) } else { TC_CAL(float) }
 #undef TC_CAL

@@ -40,15 +40,31 @@ struct Dims {
   int syp, sxp;
 };
 
-static inline Dims make_dims(int nx, int ny, int nz, int top_bc) {
+// pitch 0: each array dense (column stride = its own column length, the torch.zeros layout);
+// pitch > 0: every array's columns `pitch` elements apart (the padded layout of
+// tc_grid_set_pitch, pitch >= nz + 2).
+static inline Dims make_dims(int nx, int ny, int nz, int top_bc, int pitch = 0) {
   Dims d;
   d.nx = nx; d.ny = ny; d.nz = nz;
   d.top = top_bc == TC_TOP_NEUMANN ? 1 : -1;
-  d.syu = nz + 2; d.sxu = (ny + 2) * (nz + 2);
-  d.syv = nz + 2; d.sxv = (ny + 1) * (nz + 2);
-  d.syw = nz + 1; d.sxw = (ny + 2) * (nz + 1);
-  d.syp = nz + 2; d.sxp = (ny + 2) * (nz + 2);
+  const int cu = pitch > 0 ? pitch : nz + 2, cw = pitch > 0 ? pitch : nz + 1;
+  d.syu = cu; d.sxu = (ny + 2) * cu;
+  d.syv = cu; d.sxv = (ny + 1) * cu;
+  d.syw = cw; d.sxw = (ny + 2) * cw;
+  d.syp = cu; d.sxp = (ny + 2) * cu;
   return d;
+}
+
+// Elements an array spans from its first entry to one past its last column's stride: u (nx+1
+// planes), v, w, p (nx+2 planes).  Flat passes over a whole array (copies, fills, scaling) cover
+// this many elements, padding included.
+static inline size_t field_span(const Dims& d, int comp) {
+  switch (comp) {
+    case 0: return (size_t)(d.nx + 1) * d.sxu;
+    case 1: return (size_t)(d.nx + 2) * d.sxv;
+    case 2: return (size_t)(d.nx + 2) * d.sxw;
+    default: return (size_t)(d.nx + 2) * d.sxp;
+  }
 }
 
 // Per-grid wall-normal metric, precomputed once (tc_grid_create) in float64 and in the

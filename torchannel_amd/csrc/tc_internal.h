@@ -52,6 +52,13 @@ struct Grid {
                                   // 2 three-component pencils, -1 the default below
   int thomas_sb = 0;              // systems per wall-normal Thomas block (0: plan default)
   int thomas_lds = 1;             // wall-normal Thomas form: 0 direct row loads, 1 staged through LDS
+  int pitch = 0;                  // column stride of every array (0: dense, tc_grid_set_pitch)
+  mutable bool has_solver = false;  // a solver's workspace was laid out for this pitch
+
+  // Elements ahead of an array's first entry in a buffer the library lays out itself (the
+  // solver's post-CN velocities, the adjoint's arrays): with a padded pitch, row k = 1 of every
+  // column then starts on a 128-byte line when the buffer starts on one.
+  int lead() const { return pitch > 0 ? 128 / (dtype == TC_F64 ? 8 : 4) - 1 : 0; }
 
   // K1 form: the tuned value, else the three-component pencils where they are built for the row
   // count (float64 up to 128 rows, float32 up to 256) and a lane holds two or more rows (measured

@@ -6,15 +6,16 @@ from . import _lib as L
 from .tridiag import pcr_solve
 
 
-def _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu, top="dirichlet"):
-    dt = L.require_device(u)
-    return L.get_grid(nx, ny, nz, dx, dy, float(nu), L.top_code(top), dt, u.device, dz_f=dz_f, dz_c=dz_c)
+def _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu, top="dirichlet", others=()):
+    """The grid of a call on field arrays u (and `others`), for their dtype and layout."""
+    dt, pitch = L.require_fields(u, *others)
+    return L.get_grid(nx, ny, nz, dx, dy, float(nu), L.top_code(top), dt, u.device, dz_f=dz_f, dz_c=dz_c,
+                      pitch=pitch)
 
 
 def _rhs(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu, branch):
-    L.require_device(u, v, w)
-    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
-    ru, rv, rw = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu, others=(v, w))
+    ru, rv, rw = L.field_like(u), L.field_like(v), L.field_like(w)
     L.call("tc_rhs_imex", g.handle, branch, L.ptr(u), L.ptr(v), L.ptr(w), L.ptr(ru), L.ptr(rv),
            L.ptr(rw), L.stream_of(u))
     return ru, rv, rw
@@ -35,18 +36,16 @@ def compute_momentum_rhs_separate(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
 def compute_momentum_rhs_fused_v2(u, v, w, nx, ny, nz, dx, dy, dz_c, dz_f, nu):
     """operators.py:817-1034: diffusion (x, y and z, all explicit) minus advection for the FE
     scheme's right-hand side, zero outside the interiors (u-face nx and v-face ny included)."""
-    L.require_device(u, v, w)
-    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu)
-    ru, rv, rw = torch.empty_like(u), torch.empty_like(v), torch.empty_like(w)
+    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, nu, others=(v, w))
+    ru, rv, rw = L.field_like(u), L.field_like(v), L.field_like(w)
     L.call("tc_rhs_fused_v2", g.handle, L.ptr(u), L.ptr(v), L.ptr(w), L.ptr(ru), L.ptr(rv), L.ptr(rw),
            L.stream_of(u))
     return ru, rv, rw
 
 
 def _cn(q, comp, dt, nx, ny, nz, dz_c, dz_f, nu, theta, top):
-    L.require_device(q)
     g = _grid_from_u(q, nx, ny, nz, 1.0, 1.0, dz_c, dz_f, nu, top)
-    out = q.clone()
+    out = L.clone_field(q)
     if isinstance(dt, torch.Tensor):
         dtd = dt.to(device=q.device, dtype=torch.float64).reshape(()).contiguous()
         L.call("tc_cn_solve", g.handle, comp, 0.0, L.ptr(dtd), float(theta), L.ptr(q), L.ptr(out),
@@ -80,8 +79,7 @@ def solve_tridiagonal_batch(a, b, c, d):
 
 def compute_cfl_fused(u, v, w, nx, ny, nz, dx, dy, dz_f, dz_c):
     """operators.py:1232-1333: largest convective rate as a Python float (one device sync)."""
-    L.require_device(u, v, w)
-    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, 0.0)
+    g = _grid_from_u(u, nx, ny, nz, dx, dy, dz_c, dz_f, 0.0, others=(v, w))
     out = torch.empty((), dtype=torch.float64, device=u.device)
     L.call("tc_cfl_rate", g.handle, L.ptr(u), L.ptr(v), L.ptr(w), L.ptr(out), L.stream_of(u))
     return float(out.item())

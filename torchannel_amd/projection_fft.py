@@ -14,10 +14,12 @@ from . import _lib as L
 from .tridiag import thomas_solve
 
 
-def initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f, top_wall_bc_type="dirichlet", dtype=None):
+def initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f, top_wall_bc_type="dirichlet", dtype=None, _grid=None):
     """projection_fft.py:3-99: same dictionary keys (kx, ky, tri_a, tri_b, tri_c, nx, ny, nz,
     workspace_p, top_wall_bc_type), plus the private '_grid' that holds the HIP plan.
-    ``dtype`` (default dz_c.dtype) selects the working precision of the solve."""
+    ``dtype`` (default dz_c.dtype) selects the working precision of the solve; ``_grid`` (a
+    solver's grid of the same sizes) makes the plan that grid's and workspace_p an array in its
+    layout."""
     device = dz_c.device
     wdt = dtype or dz_c.dtype
     kx = torch.fft.fftfreq(nx, d=dx / (2 * math.pi), device=device, dtype=dz_c.dtype)
@@ -36,11 +38,12 @@ def initialize_fft_solver(nx, ny, nz, dx, dy, dz_c, dz_f, top_wall_bc_type="diri
     tb = -((cl + cr).view(1, 1, -1) + k2.unsqueeze(-1).expand(nkx, nky, nz))
     tb[:, :, 0] += cl[0]
     tb[:, :, -1] += cr[-1]
-    workspace_p = torch.zeros(nx + 2, ny + 2, nz + 2, device=device, dtype=wdt)
-    grid = None
+    grid = _grid
+    workspace_p = L.empty_field((nx + 2, ny + 2, nz + 2), grid.pitch if grid is not None else 0, wdt, device)
     if device.type == "cuda":
-        grid = L.get_grid(nx, ny, nz, dx, dy, 0.0, L.top_code(top_wall_bc_type), wdt, device,
-                          dz_f=dz_f, dz_c=dz_c)
+        if grid is None:
+            grid = L.get_grid(nx, ny, nz, dx, dy, 0.0, L.top_code(top_wall_bc_type), wdt, device,
+                              dz_f=dz_f, dz_c=dz_c)
         grid.poisson()
     return {"kx": kx, "ky": ky, "tri_a": ta, "tri_b": tb, "tri_c": tc, "nx": nx, "ny": ny, "nz": nz,
             "workspace_p": workspace_p, "top_wall_bc_type": top_wall_bc_type, "_grid": grid}
@@ -50,10 +53,13 @@ def solve_poisson_fft(div, fft_data):
     """projection_fft.py:101-165: returns fft_data['workspace_p'] (an alias, as the reference)
     holding p with every ghost entry set (periodic x/y images, Neumann z walls)."""
     p = fft_data["workspace_p"]
-    L.require_device(div, p)
+    L.require_device(div)
+    _, pitch = L.require_fields(p)
     g = fft_data.get("_grid")
     if g is None:
         raise RuntimeError("torchannel_amd: fft_data was initialised off the HIP device")
+    if pitch != g.pitch and not (pitch == 0 and p.shape[2] == g.pitch):
+        raise RuntimeError("torchannel_amd: fft_data's workspace_p does not have its plan's layout")
     if div.dtype != p.dtype:
         div = div.to(p.dtype)
     div = div.contiguous()

@@ -34,20 +34,20 @@ class TurbulenceStats:
     def use_grid(self, grid, dtype, device):
         """Take the plane transforms of an existing grid of the same nx, ny, dx, dy (the solver's),
         so that the first sample builds no plan of its own inside the time loop."""
-        self._grids[(dtype, L.device_key(device))] = grid
+        self._grids[(dtype, L.device_key(device), grid.pitch)] = grid
 
-    def _plan(self, dt, device):
-        key = (dt, L.device_key(device))
+    def _plan(self, dt, device, pitch=0):
+        key = (dt, L.device_key(device), pitch)
         if key not in self._grids:
             g = L.get_grid(self.nx, self.ny, self.nz, self.dx, self.dy, self.nu, L.TC_TOP_DIRICHLET, dt,
-                           device, dz_f=self.dz_f, dz_c=self.dz_c)
+                           device, dz_f=self.dz_f, dz_c=self.dz_c, pitch=pitch)
             self._grids[key] = g
         return self._grids[key].poisson()
 
     def accumulate_statistics(self, u, v, w, u_tau_current=None):
         """turbstats.py:101-228: one sample into the running sums."""
-        dt = L.require_device(u, v, w)
-        L.call("tc_stats_accumulate", self._plan(dt, u.device), L.ptr(u), L.ptr(v), L.ptr(w),
+        dt, pitch = L.require_fields(u, v, w)
+        L.call("tc_stats_accumulate", self._plan(dt, u.device, pitch), L.ptr(u), L.ptr(v), L.ptr(w),
                self.k_bot, self.k_top, L.ptr(self._prof), L.ptr(self._spec), L.stream_of(u))
         self.n_samples += 1
 

@@ -63,8 +63,8 @@ def _dims_u(u):
 
 def compute_divergence(u, v, w, nx, ny, nz, dx, dy, dz_f):
     """utils.py:226-240 -> new (nx, ny, nz) tensor."""
-    dt = L.require_device(u, v, w)
-    g = L.get_grid(nx, ny, nz, dx, dy, 0.0, L.TC_TOP_DIRICHLET, dt, u.device, dz_f=dz_f)
+    dt, pitch = L.require_fields(u, v, w)
+    g = L.get_grid(nx, ny, nz, dx, dy, 0.0, L.TC_TOP_DIRICHLET, dt, u.device, dz_f=dz_f, pitch=pitch)
     div = torch.empty((nx, ny, nz), dtype=dt, device=u.device)
     L.call("tc_divergence", g.handle, L.ptr(u), L.ptr(v), L.ptr(w), L.ptr(div), L.stream_of(u))
     return div
@@ -72,12 +72,12 @@ def compute_divergence(u, v, w, nx, ny, nz, dx, dy, dz_f):
 
 def compute_bulk_velocity(u, cell_vol_ratio, total_volume):
     """utils.py:207-223 -> 0-D float64 tensor on the device."""
-    dt = L.require_device(u)
+    dt, pitch = L.require_fields(u)
     nx, ny, nz = cell_vol_ratio.shape
     wgt = cell_vol_ratio
     if wgt.dtype != torch.float64 or wgt.device != u.device:
         wgt = wgt.to(device=u.device, dtype=torch.float64)
-    g = L.get_grid(nx, ny, nz, 1.0, 1.0, 0.0, L.TC_TOP_DIRICHLET, dt, u.device)
+    g = L.get_grid(nx, ny, nz, 1.0, 1.0, 0.0, L.TC_TOP_DIRICHLET, dt, u.device, pitch=pitch)
     out = torch.empty((), dtype=torch.float64, device=u.device)
     s = wgt.stride()
     L.call("tc_bulk_velocity", g.handle, L.ptr(u), ctypes_ptr(wgt), s[0], s[1], s[2],
@@ -92,9 +92,9 @@ def ctypes_ptr(t):
 
 def compute_u_tau(u, z_c, nu, top_wall_bc_type="dirichlet"):
     """utils.py:176-204 (means over the whole k=1 and k=nz planes) -> 0-D tensor."""
-    dt = L.require_device(u)
+    dt, pitch = L.require_fields(u)
     nx, ny, nz = _dims_u(u)
-    g = L.get_grid(nx, ny, nz, 1.0, 1.0, float(nu), L.top_code(top_wall_bc_type), dt, u.device)
+    g = L.get_grid(nx, ny, nz, 1.0, 1.0, float(nu), L.top_code(top_wall_bc_type), dt, u.device, pitch=pitch)
     out = torch.empty((), dtype=torch.float64, device=u.device)
     L.call("tc_u_tau", g.handle, L.ptr(u), float(z_c[1]), L.ptr(out), L.stream_of(u))
     return out if dt == torch.float64 else out.to(dt)
