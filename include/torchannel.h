@@ -35,6 +35,8 @@ extern "C" {
 #define TC_ERR_NOMEM (-1004)        /* library workspace allocation failed */
 #define TC_ERR_STATE (-1005)        /* AB2 history parity differs from the captured graph's */
 #define TC_ERR_SCRATCH (-1006)      /* batch*n above the static scratch of tc_thomas_batched_static */
+#define TC_ERR_TIMEOUT (-1007)      /* a persistent ensemble launch's dependency wait overran its bound */
+#define TC_ERR_DISABLED (-1008)     /* a code path that is built but switched off */
 
 #define TC_TOP_DIRICHLET 0          /* no-slip top wall  (solver.py:53-54) */
 #define TC_TOP_NEUMANN 1            /* free-slip top wall (solver.py:51-52) */
@@ -295,6 +297,21 @@ int tc_ensemble_step(tc_ensemble* e, double dt, int first, void* stream);
  * set differs from the one at capture. */
 int tc_ensemble_capture(tc_ensemble* e, int steps, void* stream);
 int tc_ensemble_replay(tc_ensemble* e, void* stream);
+/* Form of tc_ensemble_step and of the captured chunks.  TC_ENS_PHASED (the default): one launch
+ * per phase over all channels.  TC_ENS_PERSISTENT: one persistent launch per step (per chunk
+ * when captured: a chunk is the CN-table build and one launch that advances every channel all of
+ * the chunk's steps); its blocks take (channel, phase, block) items from a ticket counter and
+ * each item waits only for its own channel's previous phase, so a channel moves on while others
+ * finish theirs.  Same device code per item.  The persistent form is built but switched off
+ * (TC_ERR_DISABLED): on gfx950 its first run made no progress for minutes, cause not established.
+ * Setting the form drops a captured graph. */
+#define TC_ENS_PHASED 0
+#define TC_ENS_PERSISTENT 1
+int tc_ensemble_set_form(tc_ensemble* e, int form);
+/* Waits for `stream`, then reports TC_ERR_TIMEOUT (and clears it) if a persistent launch since
+ * the last check had a dependency wait overrun its bound (its results are then not to be used);
+ * TC_OK otherwise.  Every wait is bounded, so such a launch still ends. */
+int tc_ensemble_check(tc_ensemble* e, void* stream);
 
 #ifdef __cplusplus
 }

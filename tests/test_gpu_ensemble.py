@@ -34,15 +34,20 @@ def state(ch):
     return [ch.u, ch.v, ch.w, ch.p, ch._scal] + [r for s in ch._rhs for r in s]
 
 
+PERSISTENT_OFF = ("the persistent ensemble form is switched off: its first GPU run made no progress for "
+                  "minutes and the cause is not established")
+
+
+@pytest.mark.parametrize("form", ["phased", pytest.param("persistent", marks=pytest.mark.skip(reason=PERSISTENT_OFF))])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_ensemble_matches_single_channels(tc, precision):
+def test_ensemble_matches_single_channels(tc, precision, form):
     cfgs = [cfg(16, 12, 16, seed=1, precision=precision),
             cfg(24, 18, 48, re=2100.0, gamma=2.1, seed=2, precision=precision),
             cfg(32, 24, 96, re=3900.0, gamma=2.75, seed=3, precision=precision),
             cfg(16, 12, 16, seed=1, precision=precision),                   # same grid, own plan
             cfg(20, 16, 130, re=3000.0, seed=4, precision=precision, top="neumann")]
-    print(f"[ensemble] {precision} {len(cfgs)} channels", flush=True)
-    ens = tc.make_ensemble(cfgs)
+    print(f"[ensemble] {precision} {form} {len(cfgs)} channels", flush=True)
+    ens = tc.make_ensemble(cfgs, form=form)
     singles = [tc.ChannelFlow(config=copy.deepcopy(c)) for c in cfgs]
     ens.step(0.005)
     for s in singles:
@@ -55,10 +60,56 @@ def test_ensemble_matches_single_channels(tc, precision):
     for _ in range(8):
         for s in singles:
             s.step_imex(None)
+    ens.check()
     torch.cuda.synchronize()
     for q, (a, b) in enumerate(zip(ens.channels, singles)):
         for x, y in zip(state(a), state(b)):
             assert torch.equal(x, y), q
+
+
+@pytest.mark.skip(reason=PERSISTENT_OFF)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_persistent_form_switches_and_small_nz(tc, precision):
+    """The persistent form mixed with phased steps and graph chunks, on channels whose columns
+    take the smallest persistent instance (nz <= 64) and on one with nz up to 256 (the largest),
+    bit-identical to each channel alone; switching the form drops the captured graph."""
+    from torchannel_amd import _lib as L
+    for shapes in ([(16, 12, 16), (24, 18, 48), (12, 10, 64)], [(16, 12, 200), (12, 10, 40)]):
+        cfgs = [cfg(*sh, seed=q + 5, precision=precision) for q, sh in enumerate(shapes)]
+        print(f"[ensemble] persistent {precision} {shapes}", flush=True)
+        ens = tc.make_ensemble(cfgs, form="persistent")
+        singles = [tc.ChannelFlow(config=copy.deepcopy(c)) for c in cfgs]
+        ens.step(0.004)                      # forward-Euler start inside the persistent launch
+        ens.step()
+        ens.set_form("phased")
+        ens.step()
+        ens.set_form("persistent")
+        ens.step()
+        ens.run_graph(1, 6)                  # one launch of 6 steps per chunk
+        ens.set_form("phased")
+        ens.run_graph(1, 2)
+        ens.set_form("persistent")
+        ens.run_graph(2, 2)
+        ens.check()
+        for s in singles:
+            s.step_imex(0.004)
+            for _ in range(3 + 6 + 2 + 4):
+                s.step_imex(None)
+        torch.cuda.synchronize()
+        for q, (a, b) in enumerate(zip(ens.channels, singles)):
+            for x, y in zip(state(a), state(b)):
+                assert torch.equal(x, y), (shapes, q)
+    with pytest.raises(ValueError):
+        ens.set_form("tiled")
+
+
+def test_persistent_form_is_switched_off(tc):
+    ens = tc.make_ensemble([cfg(16, 12, 16, seed=1), cfg(12, 10, 24, seed=2)])
+    with pytest.raises(RuntimeError, match="switched off"):
+        ens.set_form("persistent")
+    assert ens.form == "phased"
+    ens.step(0.005)
+    ens.check()
 
 
 def test_ensemble_rejects_repeated_channel(tc):

@@ -48,6 +48,8 @@ SIGNATURES = {
     "tc_ensemble_step": [_vp, _d, _i, _vp],
     "tc_ensemble_capture": [_vp, _i, _vp],
     "tc_ensemble_replay": [_vp, _vp],
+    "tc_ensemble_set_form": [_vp, _i],
+    "tc_ensemble_check": [_vp, _vp],
     "tc_grid_get_tuning": [_vp, _i, ctypes.POINTER(_i)],
     "tc_apply_bc": [_vp, _vp, _vp, _vp, _vp],
     "tc_rhs_imex": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp],

@@ -61,4 +61,10 @@ void poisson_geometry(const Poisson* pl, int tsz, int* blocks, size_t* lds);
 int ens_launch_poisson(int dtype, const void* dch, int n, const int* const* start, const int* blocks,
                        const size_t* lds, int th_rm, hipStream_t st);
 int thomas_rows(int nz);   // rows per lane of the wall-normal Poisson solve
+// persistent form (tc_step.hip): phases of a work item, its LDS and its launch
+#define TC_ENS_NPH 8
+size_t ens_persistent_lds(int dtype, int rmax, const size_t* lds);
+int ens_launch_persistent(int dtype, int rmax, int thrm, const void* dch, const int2* items, int per_step, int steps,
+                          const int* nbt, int* sync, int nchan, int first, int newest0, int branch, size_t lds,
+                          int* grid, hipStream_t st);
 }  // namespace tc

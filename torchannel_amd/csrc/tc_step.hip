@@ -7,6 +7,7 @@
 // wall-normal tridiagonal solve is a wavefront-level parallel cyclic reduction with lane shuffles.
 #include "tc_ensemble.h"
 #include "tc_internal.h"
+#include "tc_poisson_dev.h"
 
 namespace tc {
 
@@ -914,19 +915,36 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
 __host__ __device__ __forceinline__ int k1_bands_i(int nx) { return (nx + 2 + TC_K1_WAVES - 1) / TC_K1_WAVES; }
 __host__ __device__ __forceinline__ int k1_bands_j(int ny, int jb) { return (ny + 2 + jb - 1) / jb; }
 
+// LDS of the per-component pencils' block: the CN table, then one row buffer per wavefront.
+template <typename T, int R>
+__host__ __device__ constexpr size_t k1_block_lds() {
+  return sizeof(T) * ((size_t)k1_tab_len<R>() + (size_t)TC_K1_WAVES * 3 * TC_WAVE * R);
+}
+
 // FULL: the launcher has checked nz == 64*R, so every lane's rows lie inside the u and v columns
 // (and all but the last lane's last row inside the w column); with nz a compile-time constant the
 // bounds guards of the column loads and stores fold away.
-template <typename T, int R, bool FULL = false>
+// DYN: the table and row buffers sit at `smem` (k1_block_lds<T, R>() bytes of the caller's LDS)
+// instead of in static LDS of their own, for a kernel that runs several bodies in turn.
+template <typename T, int R, bool FULL = false, bool DYN = false>
 __device__ __forceinline__ void explicit_cn_body(
     Dims d, Metric<T> mt, const T* __restrict__ u, const T* __restrict__ v, const T* __restrict__ w,
     const T* __restrict__ pu, const T* __restrict__ pv, const T* __restrict__ pw, T* __restrict__ ru,
     T* __restrict__ rv, T* __restrict__ rw, T* __restrict__ su, T* __restrict__ sv, T* __restrict__ sw,
     const double* __restrict__ tab_cell, const double* __restrict__ tab_face,
     const double* __restrict__ scal, const T* __restrict__ body, int mode, int branch, double theta, int jb,
-    int b) {
-  __shared__ T lds_tab[k1_tab_len<R>()];
-  __shared__ T lds_rows[TC_K1_WAVES][3 * TC_WAVE * R];
+    int b, T* smem = nullptr) {
+  T* lds_tab;
+  T* lds_rows0;
+  if constexpr (DYN) {
+    lds_tab = smem;
+    lds_rows0 = smem + k1_tab_len<R>();
+  } else {
+    __shared__ T s_tab[k1_tab_len<R>()];
+    __shared__ T s_rows[TC_K1_WAVES][3 * TC_WAVE * R];
+    lds_tab = s_tab;
+    lds_rows0 = &s_rows[0][0];
+  }
   if constexpr (FULL) d.nz = TC_WAVE * R;
   const int comp = b % 3, band = b / 3;
   const int nbj = k1_bands_j(d.ny, jb);
@@ -947,7 +965,7 @@ __device__ __forceinline__ void explicit_cn_body(
   K1Ctx<T, R> c;
   c.d = d;
   c.mt = mt;
-  c.rows = lds_rows[wv];
+  c.rows = lds_rows0 + wv * (3 * TC_WAVE * R);
   c.tabl = lds_tab;
   c.lane = lane;
   c.mode = mode;
@@ -1650,6 +1668,243 @@ template <typename T>
 __global__ __launch_bounds__(1024) void k_step_finalize_ens(const EnsChan<T>* __restrict__ ch, int flags) {
   const EnsChan<T>& c = ch[blockIdx.x];
   step_finalize_body(c.partials, c.k5_blocks, c.scal, c.inv_volume, flags);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Persistent ensemble form: one launch advances every channel of an ensemble `steps` IMEX steps.
+//
+// The work is a list of items (channel, phase, block of that phase), phases K1, P1 .. P5, K5 and
+// the bulk fold, ordered step-major, then phase-major, then by channel.  Every block of the
+// launch takes the next item from a ticket counter, waits until its channel has finished the
+// phase before it (K1: the previous step's fold), runs the same device body as the per-phase
+// kernels with that block index, and counts the item done.  Results are therefore bit-identical
+// to the per-phase kernels and to stepping each channel alone; what changes is that a channel's
+// next phase starts as soon as its own previous phase is done, while others are still running.
+//
+// Progress: an item waits only for items of smaller tickets, which blocks that are already
+// running hold; the oldest unfinished item waits for nothing, so every wait ends.  Each wait is
+// also bounded (spin_limit polls, about a second): past it the block sets the error word and goes
+// on, and every later wait of the launch returns at once, so a defect shows as TC_ERR_TIMEOUT from
+// tc_ensemble_check, never as a hung device.
+// Memory ordering: each wave releases its stores at agent scope before the block's barrier (a
+// barrier alone does not wait for other waves' stores), then one thread counts the item; a
+// waiting thread acquires at agent scope before the block's barrier.
+// sync: [0] ticket, [1] blocks exited, [2] error word, [3] unused, [4 + 8c + phase] items of
+// that phase the channel has finished in this launch; the last block out zeroes [0, 1] and the
+// counts for the next launch.
+// ---------------------------------------------------------------------------------------------
+#define TC_PH_K1 0
+#define TC_PH_P1 1
+#define TC_PH_P2 2
+#define TC_PH_P3 3
+#define TC_PH_P4 4
+#define TC_PH_P5 5
+#define TC_PH_K5 6
+#define TC_PH_FIN 7
+#define TC_NPH 8
+
+// step_finalize_body's fold with 256 threads: the same 1024 partial sums (thread t of 256 forms
+// those of t, t+256, t+512, t+768) and the same tree, so the result is bit-identical.  acc: 1024
+// doubles of LDS.
+__device__ __forceinline__ void step_finalize_body_256(const double* __restrict__ partials, int nb,
+                                                       double* __restrict__ scal, double inv_volume, int flags,
+                                                       double* acc) {
+  for (int m = 0; m < 4; ++m) {
+    const int t = threadIdx.x + 256 * m;
+    double s = 0.0;
+    int q = t;
+    for (; q + 3 * 1024 < nb; q += 4 * 1024) {
+      const double a0 = partials[q], a1 = partials[q + 1024], a2 = partials[q + 2048], a3 = partials[q + 3072];
+      s += (a0 + a1) + (a2 + a3);
+    }
+    for (; q < nb; q += 1024) s += partials[q];
+    acc[t] = s;
+  }
+  __syncthreads();
+  for (int o = 512; o > 0; o >>= 1) {
+    for (int t = threadIdx.x; t < o; t += 256) acc[t] += acc[t + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double ub = acc[0] * inv_volume;
+    const double dtd = scal[TC_S_DT];
+    scal[TC_S_UBULK] = ub;
+    if (flags & TC_K5_FORCING) {
+      scal[TC_S_F] = scal[TC_S_F] + (scal[TC_S_UTARGET] - ub) / dtd * 0.1;
+      scal[TC_S_TIME] = scal[TC_S_TIME] + dtd;
+      scal[TC_S_STEP] = scal[TC_S_STEP] + 1.0;
+    }
+  }
+}
+
+// RMAX: largest rows per lane of a channel's explicit + CN and projection columns; THRM: rows per
+// lane template of the wall-normal Poisson solve (the ensemble's largest).
+template <typename T, int RMAX, int THRM>
+__global__ __launch_bounds__(256) void k_ensemble_persistent(const EnsChan<T>* __restrict__ ch,
+                                                             const int2* __restrict__ items, int per_step,
+                                                             int steps, const int* __restrict__ nbt,
+                                                             int* __restrict__ sync, int nchan, int first,
+                                                             int newest0, int branch, int spin_limit) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  __shared__ int s_item;
+  const int total = per_step * steps;
+  const int cur0 = first ? 0 : 1 - newest0;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = __hip_atomic_fetch_add(sync, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const int g = s_item;
+    if (g >= total) break;
+    const int st = g / per_step;
+    const int2 it = items[g - st * per_step];
+    const int ph = it.x >> 16, c = it.x & 0xffff, lb = it.y;
+    if (threadIdx.x == 0) {
+      const int dep = ph == TC_PH_K1 ? TC_PH_FIN : ph - 1;
+      const int target = ph == TC_PH_K1 ? st : (st + 1) * nbt[c * TC_NPH + dep];
+      if (target > 0) {
+        const int* ctr = sync + 4 + c * TC_NPH + dep;
+        int spins = 0;
+        while (__hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          __builtin_amdgcn_s_sleep(2);
+          if (++spins > spin_limit) {
+            __hip_atomic_store(sync + 2, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          // once any wait has overrun, no block waits any more: the launch drains at once
+          if ((spins & 255) == 0 && __hip_atomic_load(sync + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+    }
+    __syncthreads();
+    const EnsChan<T>& cc = ch[c];   // fields are passed to the bodies by value
+    const int R = (cc.d.nz + TC_WAVE - 1) / TC_WAVE;
+    switch (ph) {
+      case TC_PH_K1: {
+        const int cur = cur0 ^ (st & 1);
+        const int prev = (first && st == 0) ? -1 : 1 - cur;
+        const int mode = TC_K1_RHS | TC_K1_STEP | TC_K1_FORCE | ((first && st == 0) ? TC_K1_FIRST : 0);
+        T* const* rc = cc.rs[cur];
+        T* const* rp = cc.rs[prev < 0 ? cur : prev];
+#define TC_PK1(RR)                                                                                              \
+  explicit_cn_body<T, RR, false, true>(cc.d, cc.mt, cc.u, cc.v, cc.w, rp[0], rp[1], rp[2], rc[0], rc[1], rc[2], \
+                                       cc.su, cc.sv, cc.sw, cc.tabs, cc.tabs + TC_TAB_STRIDE, cc.scal, cc.body,  \
+                                       mode, branch, cc.theta, cc.jb, lb, (T*)smem_raw)
+        if (R == 1) TC_PK1(1);
+        else if constexpr (RMAX >= 2) {
+          if (R == 2) TC_PK1(2);
+          else if constexpr (RMAX >= 3) {
+            if (R == 3) TC_PK1(3);
+            else if constexpr (RMAX >= 4) TC_PK1(4);
+          }
+        }
+#undef TC_PK1
+        break;
+      }
+      case TC_PH_P1:
+        fwd_x_body<T, true>(cc.d, cc.mt, cc.su, cc.sv, cc.sw, nullptr, cc.d.nz, cc.scal, cc.fx, cc.twx, cc.spec,
+                            cc.pairs, lb);
+        break;
+      case TC_PH_P2:
+        fft_y_body<T>(cc.nkx, cc.d.ny, cc.d.nz, cc.fy, cc.twy, cc.spec, cc.kzy, -1, lb);
+        break;
+      case TC_PH_P3: {
+        const int L = thomas_lanes(cc.d.nz);
+        thomas_body<T, THRM>(cc.nkx * cc.d.ny, cc.d.nz, thomas_rows_per_lane(cc.d.nz), L, TH_THREADS / L, cc.cl,
+                             cc.cr, cc.K2, cc.th_cb, cc.spec, lb);
+        break;
+      }
+      case TC_PH_P4:
+        fft_y_body<T>(cc.nkx, cc.d.ny, cc.d.nz, cc.fy, cc.twy, cc.spec, cc.kzy, +1, lb);
+        break;
+      case TC_PH_P5:
+        inv_x_body<T>(cc.d, cc.fx, cc.twx, cc.spec, cc.p, (T)(1.0 / ((double)cc.d.nx * (double)cc.d.ny)), cc.pairs,
+                      lb);
+        break;
+      case TC_PH_K5: {
+#define TC_PK5(RR)                                                                                          \
+  project_bc_body<T, RR>(cc.d, cc.mt, cc.su, cc.sv, cc.sw, cc.p, cc.u, cc.v, cc.w, cc.scal, cc.partials,    \
+                         TC_K5_BULK | TC_K5_FORCING, lb)
+        if (R == 1) TC_PK5(1);
+        else if constexpr (RMAX >= 2) {
+          if (R == 2) TC_PK5(2);
+          else if constexpr (RMAX >= 3) {
+            if (R == 3) TC_PK5(3);
+            else if constexpr (RMAX >= 4) TC_PK5(4);
+          }
+        }
+#undef TC_PK5
+        break;
+      }
+      default:
+        step_finalize_body_256(cc.partials, cc.k5_blocks, cc.scal, cc.inv_volume, TC_K5_BULK | TC_K5_FORCING,
+                               (double*)smem_raw);
+        break;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(sync + 4 + c * TC_NPH + ph, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x == 0) {
+    const int out = __hip_atomic_fetch_add(sync + 1, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (out == (int)gridDim.x - 1) {
+      for (int q = 0; q < TC_NPH * nchan; ++q) sync[4 + q] = 0;
+      sync[0] = 0;
+      sync[1] = 0;
+    }
+  }
+}
+
+// Dynamic LDS of the persistent kernel: the largest of the Poisson passes' LDS (lds), the
+// per-component pencils' table and row buffers and the fold's 1024 partial sums.
+size_t ens_persistent_lds(int dtype, int rmax, const size_t* lds) {
+  size_t b = 1024 * sizeof(double);
+  for (int q = 0; q < 4; ++q) b = std::max(b, lds[q]);
+  const size_t k1 = dtype == TC_F64 ? (rmax <= 1 ? k1_block_lds<double, 1>() : rmax <= 2 ? k1_block_lds<double, 2>()
+                                                  : rmax <= 3 ? k1_block_lds<double, 3>() : k1_block_lds<double, 4>())
+                                    : (rmax <= 1 ? k1_block_lds<float, 1>() : rmax <= 2 ? k1_block_lds<float, 2>()
+                                                  : rmax <= 3 ? k1_block_lds<float, 3>() : k1_block_lds<float, 4>());
+  return std::max(b, k1);
+}
+
+template <typename T, int RMAX, int THRM>
+static int ens_persistent_t(const void* dch, const int2* items, int per_step, int steps, const int* nbt, int* sync,
+                            int nchan, int first, int newest0, int branch, size_t lds, int* grid, hipStream_t st) {
+  auto kern = k_ensemble_persistent<T, RMAX, THRM>;
+  if (*grid <= 0) {
+    int dev = 0, ncu = 0, per = 0;
+    TC_HIP_RC(hipGetDevice(&dev));
+    TC_HIP_RC(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TC_HIP_RC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 256, lds));
+    if (per < 1) return TC_ERR_UNSUPPORTED;
+    *grid = per * ncu;
+  }
+  const int blocks = std::max(1, std::min(*grid, per_step * steps));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(256), lds, st, (const EnsChan<T>*)dch, items, per_step, steps, nbt,
+                     sync, nchan, first, newest0, branch, 1 << 20);
+  TC_CHECK_LAUNCH();
+  return TC_OK;
+}
+
+// One persistent launch of `steps` steps.  The instance follows the ensemble's largest column:
+// rmax rows per lane (1, 2 or up to 4) and the Poisson solve's rows-per-lane template (4 up to
+// 64 rows, else 8).  *grid: resident blocks, found on the first call (0) and kept by the caller.
+int ens_launch_persistent(int dtype, int rmax, int thrm, const void* dch, const int2* items, int per_step, int steps,
+                          const int* nbt, int* sync, int nchan, int first, int newest0, int branch, size_t lds,
+                          int* grid, hipStream_t st) {
+  if (lds > 160 * 1024) return TC_ERR_ARG;
+#define TC_EP(TT, RM, TH)                                                                                     \
+  return ens_persistent_t<TT, RM, TH>(dch, items, per_step, steps, nbt, sync, nchan, first, newest0, branch, lds,   \
+                                      grid, st)
+  if (dtype == TC_F64) {
+    if (rmax <= 1 && thrm <= 4) TC_EP(double, 1, 4);
+    if (rmax <= 2) TC_EP(double, 2, 8);
+    TC_EP(double, 4, 8);
+  }
+  if (rmax <= 1 && thrm <= 4) TC_EP(float, 1, 4);
+  if (rmax <= 2) TC_EP(float, 2, 8);
+  TC_EP(float, 4, 8);
+#undef TC_EP
 }
 
 // ---------------------------------------------------------------------------------------------

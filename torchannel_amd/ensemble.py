@@ -1,10 +1,13 @@
 """Ensembles of independent channels advanced together (no counterpart in the reference, which
 runs one channel per process).
 
-Every phase of the IMEX step is one launch over all channels (tc_ensemble_*): small channels
-share the GPU instead of each launching its own under-filled kernels, and a chunk of steps is one
-HIP-graph launch.  Each channel is an ordinary ChannelFlow (with ``compute.private_plan`` so the
-Poisson workspaces are distinct); results are bit-identical to stepping it alone.
+Every phase of the IMEX step is one launch over all channels ('phased', tc_ensemble_set_form;
+the 'persistent' form, one launch per step whose blocks take (channel, phase, block) work items
+from a ticket counter, is built but switched off in the library): small channels share the GPU
+instead of each launching its own under-filled kernels, and a chunk of steps is one HIP-graph
+launch.  Each
+channel is an ordinary ChannelFlow (with ``compute.private_plan`` so the Poisson workspaces are
+distinct); results are bit-identical to stepping it alone, in both forms.
 """
 import copy
 import ctypes
@@ -16,8 +19,11 @@ from . import _lib as L
 from .solver import ChannelFlow
 
 
+FORMS = {"phased": 0, "persistent": 1}
+
+
 class ChannelEnsemble:
-    def __init__(self, channels):
+    def __init__(self, channels, form="phased"):
         if not channels:
             raise ValueError("empty ensemble")
         c0 = channels[0]
@@ -32,6 +38,23 @@ class ChannelEnsemble:
         L.call("tc_ensemble_create", arr, len(channels), ctypes.byref(h))
         self._h = h
         self._graph_steps = None
+        self.form = "phased"
+        self.set_form(form)
+
+    def set_form(self, form):
+        """'phased' (one launch per phase).  'persistent' (one launch per step or chunk) is built
+        but switched off in the library and raises."""
+        if form not in FORMS:
+            raise ValueError(f"Invalid ensemble form: {form}. Must be one of {sorted(FORMS)}")
+        L.call("tc_ensemble_set_form", self._h, FORMS[form])
+        if form != self.form:
+            self._graph_steps = None
+        self.form = form
+
+    def check(self):
+        """Wait for the ensemble's stream; raise if a persistent launch since the last check had a
+        dependency wait overrun its bound (its results are then invalid)."""
+        L.call("tc_ensemble_check", self._h, self._stream())
 
     def __del__(self):
         try:
@@ -98,6 +121,6 @@ def ensemble_configs(n=48, seed=2026, precision="fp32", nz_choices=(32, 48, 64, 
     return out
 
 
-def make_ensemble(configs):
+def make_ensemble(configs, form="phased"):
     chans = [ChannelFlow(config=copy.deepcopy(c)) for c in configs]
-    return ChannelEnsemble(chans)
+    return ChannelEnsemble(chans, form=form)
