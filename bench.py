@@ -31,6 +31,17 @@ KERNEL_OF = {"explicit_ab2_cn": "k_explicit_cn_uvw", "div_fft_x": "k_fwd_x", "ff
              "project_bc_bulk": "k_project_bc", "cfl_max": "k_cfl"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "imex_step_192x192x128_hbm_traffic.json")
 LAYOUT = "dense"      # the solver's array layout (compute.layout), --layout
+TUNE = True           # pick the launch-shape knobs with the autotuner after the first warm-up step
+KNOB_NAMES = {1: "k1_pencil_length", 2: "thomas_systems_per_block", 3: "k1_form", 4: "thomas_lds"}
+
+
+def _tune(ch):
+    """The autotuner on a channel that has taken a step (results do not depend on the knobs);
+    returns the chosen knobs by name, or None when tuning is off or no step was taken yet."""
+    if not TUNE or ch._newest < 0:
+        return None
+    from torchannel_amd.autotune import tune
+    return {KNOB_NAMES.get(k, str(k)): v for k, v in tune(ch).items()}
 
 
 def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
@@ -121,8 +132,11 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
     ch = tc.ChannelFlow(config=config(precision, steps=steps))
     interval = ch.dt_update_interval
     chunk = 10
-    # warm-up: forward-Euler start then AB2, eager, with the reference cadence
-    _eager(ch, warmup, interval)
+    # warm-up: forward-Euler start then AB2, eager, with the reference cadence; the autotuner
+    # after the first warm-up step
+    _eager(ch, min(warmup, 1), interval)
+    tuning = _tune(ch)
+    _eager(ch, warmup - min(warmup, 1), interval)
     # the timed K steps: eager up to a multiple of the chunk, chunks, eager rest
     # (with no warm-up the first chunk's worth is eager: a graph needs the AB2 history)
     lead = min(steps, (-warmup) % chunk if warmup > 0 else chunk)
@@ -179,6 +193,7 @@ def run_gpu(precision, steps, warmup, phase_reps=20):
                      "gbs": byts / (avg_ms * 1e-3) / 1e9 if byts else 0.0}
     out["kernels"] = per
     out["pitch"] = ch.pitch
+    out["tuning"] = tuning
     out["k1_form"] = ch._grid.get_tuning(L.TUNE_K1_TILE)
     out["k1_jb"] = ch._grid.get_tuning(L.TUNE_K1_JB)
     step_kernels = [k for k in PHASES if k not in ("cfl_max",)]
@@ -287,10 +302,13 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
     # the first dt from the CFL rule (a white-noise start at this resolution is far above CFL 0.25
     # at the nominal dt = 0.005), then the regular cadence
     ch.update_dt_device()
+    tuning = None
     for s_ in range(warmup):
         if (s_ + 1) % 10 == 0:
             ch.update_dt_device()
         ch.step_imex(None)
+        if s_ == 0:
+            tuning = _tune(ch)
     ch.run_graph(0, chunk, dt_update_interval=10)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
@@ -315,7 +333,7 @@ def run_c3(steps=500, warmup=20, phase_reps=10):
     return {"workload": "384x384x256, Lx=2pi, Ly=pi, Re_bulk=10000, float32, adaptive dt CFL 0.25 (first dt "
                         f"from the CFL rule), {(steps // chunk) * chunk} steps", "finite": finite,
             "value": round(n_cells / (ms * 1e-3) / 1e6, 2), "unit": "Mcell-steps/s", "ms_per_step": round(ms, 4),
-            "dt_final": dt_final, "u_bulk": u_bulk, "kernels": per,
+            "dt_final": dt_final, "u_bulk": u_bulk, "kernels": per, "tuning": tuning,
             "column_pattern_ceiling": column_ceiling("fp32", nx, ny, nz, jb, pitch)}
 
 
@@ -443,6 +461,8 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=500)
     ap.add_argument("--layout", default="dense", choices=["dense", "padded"],
                     help="the solver's array layout (compute.layout); dense is the default")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="run the default launch shapes instead of the autotuner's choice")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
                          "kernel for scaling HBM byte counters")
@@ -456,8 +476,9 @@ def main():
         print("bench.py: --steps must be >= 1 and --warmup >= 0", flush=True)
         sys.exit(2)
     steps, warmup = args.steps, args.warmup
-    global LAYOUT
+    global LAYOUT, TUNE
     LAYOUT = args.layout
+    TUNE = not args.no_tune
     print(f"[bench] layout {LAYOUT}", file=sys.stderr, flush=True)
     print(f"[bench] {args.precision} {NX}x{NY}x{NZ} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
     if args.copy_calib_mb > 0:
@@ -525,7 +546,7 @@ def main():
                                "Re_bulk=2870, adaptive dt CFL 0.25 every 10 steps, constant flow rate, "
                                "turbstats every 200 steps",
                    "precision": args.precision, "graph_chunk_steps": 10, "layout": LAYOUT,
-                   "column_pitch": main_run["pitch"]},
+                   "column_pitch": main_run["pitch"], "tuning": main_run["tuning"]},
         "roofline": {"bound": "hbm", "kernel": dom, "k1_form": main_run["k1_form"], "k1_jb": main_run["k1_jb"],
                      "achieved": round(k["gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(k["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic,
