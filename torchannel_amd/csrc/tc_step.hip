@@ -760,56 +760,69 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
   const T h = T(0.5);
   const bool step = (cu.mode & TC_K1_STEP) != 0, ab2 = step && !(cu.mode & TC_K1_FIRST);
   const long long ou = (long long)i * d.sxu, ov = (long long)i * d.sxv, ow = (long long)i * d.sxw;
+  const long long sxu = d.sxu, sxv = d.sxv, sxw = d.sxw, syu = d.syu, syv = d.syv, syw = d.syw;
   const T* ui = u + ou;
   const T* vi = v + ov;
   const T* wi = w + ow;
-  const T* bi = cu.body != nullptr ? cu.body + ou : nullptr;
-  // loads of step j's inputs that are not yet in registers
-  auto fetch_now = [&](int j, K3Next<T, R>& L) {
-    const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
-    col_load<R, T>(ui + d.sxu + cu1, L.uE, n, lane);
-    col_load<R, T>(vi - d.sxv + cv1, L.vW, n, lane);
-    col_load<R, T>(vi + d.sxv + cv1, L.vE, n, lane);
-    col_load<R, T>(wi - d.sxw + cw1, L.wW, n, lane);
-    col_load<R, T>(wi + d.sxw + cw1, L.wE, n, lane);
+  // Column pointers carried along the march (each advanced by its array's column stride per
+  // step, not recomputed from j): uj, vj, wj and the history columns at the (i, j) whose inputs
+  // are being fetched; the rhs and star outputs at the (i, j) being finished.
+  const T* uj = ui + (long long)j0 * syu;
+  const T* vj = vi + (long long)j0 * syv;
+  const T* wj = wi + (long long)j0 * syw;
+  const T* puj = pu + ou + (long long)j0 * syu;
+  const T* pvj = pv + ov + (long long)j0 * syv;
+  const T* pwj = pw + ow + (long long)j0 * syw;
+  T* ruj = ru + ou + (long long)j0 * syu;
+  T* rvj = rv + ov + (long long)j0 * syv;
+  T* rwj = rw + ow + (long long)j0 * syw;
+  T* suj = su + ou + (long long)j0 * syu;
+  T* svj = sv + ov + (long long)j0 * syv;
+  T* swj = sw + ow + (long long)j0 * syw;
+  const T* bj = cu.body != nullptr ? cu.body + ou + (long long)j0 * syu : nullptr;
+  // loads of the fetched step's inputs that are not yet in registers
+  auto fetch_now = [&](K3Next<T, R>& L) {
+    col_load<R, T>(uj + sxu, L.uE, n, lane);
+    col_load<R, T>(vj - sxv, L.vW, n, lane);
+    col_load<R, T>(vj + sxv, L.vE, n, lane);
+    col_load<R, T>(wj - sxw, L.wW, n, lane);
+    col_load<R, T>(wj + sxw, L.wE, n, lane);
     if (ab2) {
-      col_load_hist<R, T>(pu + ou + cu1, L.pu, n, lane);
-      col_load_hist<R, T>(pv + ov + cv1, L.pv, n, lane);
-      col_load_hist<R, T>(pw + ow + cw1, L.pw, n, lane);
+      col_load_hist<R, T>(puj, L.pu, n, lane);
+      col_load_hist<R, T>(pvj, L.pv, n, lane);
+      col_load_hist<R, T>(pwj, L.pw, n, lane);
     }
   };
-  auto fetch_ahead = [&](int j, K3Next<T, R>& L) {
-    const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
-    col_load<R, T>(ui + cu1 + d.syu, L.uN, n, lane);
-    col_load<R, T>(ui - d.sxu + cu1 + d.syu, L.umN, n, lane);
-    col_load<R, T>(vi + cv1 + d.syv, L.vN, n, lane);
-    col_load<R, T>(wi + cw1 + d.syw, L.wN, n, lane);
-    L.ub0 = ui[cu1];
-    L.ubn = ui[cu1 + n + 1];
-    L.vb0 = vi[cv1];
-    L.vbn = vi[cv1 + n + 1];
-    L.wEb0 = wi[cw1 + d.sxw];
-    L.wNb0 = wi[cw1 + d.syw];
+  auto fetch_ahead = [&](K3Next<T, R>& L) {
+    col_load<R, T>(uj + syu, L.uN, n, lane);
+    col_load<R, T>(uj - sxu + syu, L.umN, n, lane);
+    col_load<R, T>(vj + syv, L.vN, n, lane);
+    col_load<R, T>(wj + syw, L.wN, n, lane);
+    L.ub0 = uj[0];
+    L.ubn = uj[n + 1];
+    L.vb0 = vj[0];
+    L.vbn = vj[n + 1];
+    L.wEb0 = wj[sxw];
+    L.wNb0 = wj[syw];
   };
   // plane j0-1: the columns step j0 reads from it, and v(i, j0-1) one row up
   T uS[R], vS[R], v1m[R], wS[R], vm1[R];
-  col_load<R, T>(ui + (long long)(j0 - 1) * d.syu, uS, n, lane);
-  col_load<R, T>(vi + (long long)(j0 - 1) * d.syv, vS, n, lane);
-  col_load<R, T>(vi + d.sxv + (long long)(j0 - 1) * d.syv, v1m, n, lane);
-  col_load<R, T>(wi + (long long)(j0 - 1) * d.syw, wS, n, lane);
+  col_load<R, T>(uj - syu, uS, n, lane);
+  col_load<R, T>(vj - syv, vS, n, lane);
+  col_load<R, T>(vj + sxv - syv, v1m, n, lane);
+  col_load<R, T>(wj - syw, wS, n, lane);
   rows_up_in<R, T>(cu.rows, vS, vm1, lane);
   // plane j0: u(i-1), u(i), v(i), w(i) (the rest comes with fetch)
   T uC[R], uW[R], vC[R], wC[R];
-  col_load<R, T>(ui + (long long)j0 * d.syu, uC, n, lane);
-  col_load<R, T>(ui - d.sxu + (long long)j0 * d.syu, uW, n, lane);
-  col_load<R, T>(vi + (long long)j0 * d.syv, vC, n, lane);
-  col_load<R, T>(wi + (long long)j0 * d.syw, wC, n, lane);
-  T wC0 = wi[(long long)j0 * d.syw];
+  col_load<R, T>(uj, uC, n, lane);
+  col_load<R, T>(uj - sxu, uW, n, lane);
+  col_load<R, T>(vj, vC, n, lane);
+  col_load<R, T>(wj, wC, n, lane);
+  T wC0 = wj[0];
   K3Next<T, R> A;
-  fetch_ahead(j0, A);
-  fetch_now(j0, A);
+  fetch_ahead(A);
+  fetch_now(A);
   for (int j = j0; j < j1; ++j) {
-    const long long cu1 = (long long)j * d.syu, cv1 = (long long)j * d.syv, cw1 = (long long)j * d.syw;
     T xu[R], xv[R], xw[R];
     bool cn_u, cn_v, cn_w;
     {
@@ -832,8 +845,7 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
         const T fz = (h * (wC[r] + A.wE[r]) * (h * (uC[r] + ut[r])) - h * (w0m[r] + w1m[r]) * (h * (ub[r] + uC[r]))) * rc2[r];
         rhs[r] = rhs[r] - (fx + fy + fz);
       }
-      cn_u = k1_rhs_ab2<T, R>(cu, rhs, uC, A.pu, ru + ou + cu1, bi != nullptr ? bi + cu1 : nullptr, n, n + 2, true,
-                              xu);
+      cn_u = k1_rhs_ab2<T, R>(cu, rhs, uC, A.pu, ruj, bj, n, n + 2, true, xu);
       // v (k1_pencil_v): u00 = u(i,j), u01 = u(i,j+1), um0 = u(i-1,j), um1 = u(i-1,j+1), w00 = w(i,j),
       // w01 = w(i,j+1), w0m = w(i,j) one row down (as above)
       T vb[R], vt[R], w1mv[R];
@@ -850,7 +862,7 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
         const T fz = (h * (wC[r] + A.wN[r]) * (h * (vC[r] + vt[r])) - h * (w0m[r] + w1mv[r]) * (h * (vb[r] + vC[r]))) * rc2[r];
         rhs[r] = rhs[r] - (fx + fy + fz);
       }
-      cn_v = k1_rhs_ab2<T, R>(cv, rhs, vC, A.pv, rv + ov + cv1, nullptr, n, n + 2, true, xv);
+      cn_v = k1_rhs_ab2<T, R>(cv, rhs, vC, A.pv, rvj, nullptr, n, n + 2, true, xv);
       // w (k1_pencil_w, rows k = 1..nz-1): u00 = u(i,j), um0 = u(i-1,j), v00 = v(i,j), vm0 = v(i,j-1),
       // each with its row above; the pencil's top overrides (row nz of w, u and v) are the rows
       // these columns hold in registers, so the plain exchanges give the same values
@@ -870,7 +882,7 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
         const T fz = (wr * wr - wl * wl) * rf2[r];
         rhs[r] = lap - (fx + fy + fz);
       }
-      cn_w = k1_rhs_ab2<T, R>(cw, rhs, wC, A.pw, rw + ow + cw1, nullptr, nw, n + 1, true, xw);
+      cn_w = k1_rhs_ab2<T, R>(cw, rhs, wC, A.pw, rwj, nullptr, nw, n + 1, true, xw);
       // carry plane j as plane j-1 and plane j+1 as plane j
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -888,24 +900,37 @@ __device__ __forceinline__ void k1_pencil_uvw(const K1Ctx<T, R>& cu, const K1Ctx
     }
     // the next step's loads are in flight during the three CN solves
     if (j + 1 < j1) {
-      fetch_ahead(j + 1, A);
-      fetch_now(j + 1, A);
+      uj += syu;
+      vj += syv;
+      wj += syw;
+      puj += syu;
+      pvj += syv;
+      pwj += syw;
+      fetch_ahead(A);
+      fetch_now(A);
     }
     if (cn_u || cn_v) {
       T a0[R], a1[R], a2[R];
       k3_met<0, T, R>(met, a0, lane);
       k3_met<1, T, R>(met, a1, lane);
       k3_met<2, T, R>(met, a2, lane);
-      if (cn_u) k1_cn_store<T, R, false>(cu, xu, su + ou + cu1, n, a0, a1, a2, T(0));
-      if (cn_v) k1_cn_store<T, R, false>(cv, xv, sv + ov + cv1, n, a0, a1, a2, T(0));
+      if (cn_u) k1_cn_store<T, R, false>(cu, xu, suj, n, a0, a1, a2, T(0));
+      if (cn_v) k1_cn_store<T, R, false>(cv, xv, svj, n, a0, a1, a2, T(0));
     }
     if (cn_w) {
       T a0[R], a1[R], a2[R];
       k3_met<3, T, R>(met, a0, lane);
       k3_met<4, T, R>(met, a1, lane);
       k3_met<5, T, R>(met, a2, lane);
-      k1_cn_store<T, R, true>(cw, xw, sw + ow + cw1, nw, a0, a1, a2, T(0));
+      k1_cn_store<T, R, true>(cw, xw, swj, nw, a0, a1, a2, T(0));
     }
+    ruj += syu;
+    rvj += syv;
+    rwj += syw;
+    suj += syu;
+    svj += syv;
+    swj += syw;
+    if (bj != nullptr) bj += syu;
   }
 }
 
