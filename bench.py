@@ -32,6 +32,7 @@ KERNEL_OF = {"explicit_ab2_cn": "k_explicit_cn_uvw", "div_fft_x": "k_fwd_x", "ff
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "imex_step_192x192x128_hbm_traffic.json")
 LAYOUT = "dense"      # the solver's array layout (compute.layout), --layout
 TUNE = True           # pick the launch-shape knobs with the autotuner after the first warm-up step
+TUNE_CACHE = None     # --tune-cache: a JSON file of earlier choices (read if it has this workload)
 KNOB_NAMES = {1: "k1_pencil_length", 2: "thomas_systems_per_block", 3: "k1_form", 4: "thomas_lds"}
 
 
@@ -41,7 +42,7 @@ def _tune(ch):
     if not TUNE or ch._newest < 0:
         return None
     from torchannel_amd.autotune import tune
-    return {KNOB_NAMES.get(k, str(k)): v for k, v in tune(ch).items()}
+    return {KNOB_NAMES.get(k, str(k)): v for k, v in tune(ch, path=TUNE_CACHE).items()}
 
 
 def config(precision, nx=NX, ny=NY, nz=NZ, steps=1000):
@@ -463,6 +464,9 @@ def main():
                     help="the solver's array layout (compute.layout); dense is the default")
     ap.add_argument("--no-tune", action="store_true",
                     help="run the default launch shapes instead of the autotuner's choice")
+    ap.add_argument("--tune-cache", default=None,
+                    help="JSON file of the autotuner's choices: read when it holds this workload (no "
+                         "timing launches), else written after tuning")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run a plain device copy of this many MiB (5 times), the reference "
                          "kernel for scaling HBM byte counters")
@@ -476,9 +480,10 @@ def main():
         print("bench.py: --steps must be >= 1 and --warmup >= 0", flush=True)
         sys.exit(2)
     steps, warmup = args.steps, args.warmup
-    global LAYOUT, TUNE
+    global LAYOUT, TUNE, TUNE_CACHE
     LAYOUT = args.layout
     TUNE = not args.no_tune
+    TUNE_CACHE = args.tune_cache
     print(f"[bench] layout {LAYOUT}", file=sys.stderr, flush=True)
     print(f"[bench] {args.precision} {NX}x{NY}x{NZ} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
     if args.copy_calib_mb > 0:

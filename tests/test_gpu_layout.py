@@ -175,3 +175,29 @@ def test_adjoint_gradient_padded_equal_dense(tc):
         assert g.is_contiguous() and g.shape == ch.u.shape
         out.append((J.clone(), g))
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ensemble_of_padded_channels(tc, precision):
+    """Channels in the padded layout stepped as an ensemble (per-channel strides in the channel
+    descriptors) equal the same channels in the dense layout, bit for bit."""
+    import copy
+    shapes = [(16, 12, 16), (24, 18, 48), (20, 16, 130)]
+    print(f"[layout] ensemble {precision} {shapes}", flush=True)
+    out = []
+    for layout in ("dense", "padded"):
+        cfgs = []
+        for q, sh in enumerate(shapes):
+            c = cfg(*sh, precision, "gpu", "dirichlet", layout)
+            c["initialization"]["seed"] = q + 1
+            c["compute"]["private_plan"] = True
+            cfgs.append(c)
+        ens = tc.make_ensemble([copy.deepcopy(c) for c in cfgs])
+        ens.step(0.005)
+        ens.step()
+        ens.run_graph(1, 4)
+        torch.cuda.synchronize()
+        out.append(ens.channels)
+    for a, b in zip(*out):
+        assert b.pitch > 0 and a.pitch == 0
+        _equal(a, b, "ensemble")

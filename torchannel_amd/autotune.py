@@ -13,8 +13,9 @@ solver's stream and keeps the fastest setting on the grid.  Knobs:
 * ``TUNE_THOMAS_LDS`` that solve's rows copied through LDS with whole-block 16-byte accesses, or
   loaded by each lane directly (searched jointly with the previous knob)
 
-Chosen settings are cached per (nx, ny, nz, dtype, top wall, device name) in memory and, when
-``path`` (or the ``TORCHANNEL_TUNE_CACHE`` environment variable) names a JSON file, on disk.
+Chosen settings are cached per (nx, ny, nz, dtype, top wall, device architecture and CU count) in
+memory and, when ``path`` (or the ``TORCHANNEL_TUNE_CACHE`` environment variable) names a JSON
+file, on disk.
 """
 import json
 import os
@@ -31,8 +32,12 @@ _memory = {}
 
 
 def _key(ch):
-    name = torch.cuda.get_device_name(ch.u.device)
-    return f"{ch.nx}x{ch.ny}x{ch.nz}:{str(ch.wdtype).split('.')[-1]}:{ch.top_wall_bc_type}:{name}"
+    """Cache key: grid, dtype, top wall and the device's architecture and CU count (the marketing
+    name is not used: it reads empty under some tools, e.g. a profiler)."""
+    props = torch.cuda.get_device_properties(ch.u.device)
+    arch = (getattr(props, "gcnArchName", "") or props.name or "device").split(":")[0]
+    return (f"{ch.nx}x{ch.ny}x{ch.nz}:{str(ch.wdtype).split('.')[-1]}:{ch.top_wall_bc_type}:"
+            f"{arch}:{props.multi_processor_count}cu")
 
 
 def _candidates(ch):
