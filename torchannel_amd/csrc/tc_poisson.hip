@@ -430,10 +430,10 @@ static int poisson_phase_t(const Poisson* pl, int ph, bool from_vel, const void*
     case 3: return launch_fft_y_t<T>(pl, pl->spec, d.nz, pl->kzy, +1, st);
     case 4: {
       const int KZ = 2 * pl->pairs;
-      const int blocks5 = d.ny * ((d.nz + KZ - 1) / KZ);
+      const int tasks = d.ny * ((d.nz + KZ - 1) / KZ);
       const size_t lds = fft_lds<T>(pl->pairs, d.nx);
       const T scale = (T)(1.0 / ((double)d.nx * (double)d.ny));
-      hipLaunchKernelGGL((k_inv_x<T>), dim3(blocks5), dim3(256), lds, st, d, pl->fx, pl->twx,
+      hipLaunchKernelGGL((k_inv_x<T>), dim3(tasks), dim3(256), lds, st, d, pl->fx, pl->twx,
                          (const cplx<T>*)pl->spec, (T*)p, scale, pl->pairs);
       TC_CHECK_LAUNCH();
       return TC_OK;

@@ -166,30 +166,17 @@ __global__ __launch_bounds__(256) void k_fft_y(int nkx, int ny, int nz_in, FftDe
   fft_y_body<T>(nkx, ny, nz_in, fy, twy, S, KZ, sign, xcd_remap(blockIdx.x, gridDim.x));
 }
 
-// P5: half spectra -> real rows (irfft semantics: the imaginary parts of the kx = 0 and Nyquist
-// bins are dropped), scaled by 1/(nx*ny), written to p with the ghost images of
-// projection_fft.py:150-163 (x copies, y copies, Neumann z walls).  Two real z-rows ka, ka + 1
-// travel as one complex sequence (pair b): input x of the pair is A + iB at bin m = x for x <= nx/2
-// and its conjugate-symmetric image at bin m = nx - x above, with A, B the rows' bins; output x
-// holds row ka in its real and row ka + 1 in its imaginary part.
+// Input slots t0 + 256 q (q < FFT_U) of the transform of block task (j, k0): bins ka = k0 + 2 qq
+// and ka + 1 of spectrum column (m, j), slot t = m * pairs + qq.  Slots past the end and rows
+// past nz are clamped into range (the staging zeroes them), so no load sits behind a branch.
 template <typename T>
-__device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const double2* __restrict__ twx,
-                                               const cplx<T>* __restrict__ S, T* __restrict__ p,
-                                               T scale, int pairs,
-    int blk) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int KZ = 2 * pairs;
-  const int nch = (nz + KZ - 1) / KZ;
-  const int j = blk / nch, k0 = (blk - j * nch) * KZ;
-  const int ldb = fft_ld<T>(nx);
-  const int h = nx / 2;
-  const int lpr = 31 - __clz(pairs);   // pairs is a power of two (checked by the launchers)
-  cplx<T>* twl = Z + pairs * ldb;
-  fft_stage_twiddles<T>(twl, twx, nx);
-  // bins ka and ka + 1 of column (m, j), clamped into the column (the caller zeroes rows >= nz)
-  auto bins = [&](int m, int ka, cplx<T>& A, cplx<T>& B) {
+__device__ __forceinline__ void inv_x_fetch(const cplx<T>* __restrict__ S, int ny, int nz, int j, int k0,
+                                            int lpr, int pairs, int tot, int t0, cplx<T> (&va)[FFT_U],
+                                            cplx<T> (&vb)[FFT_U]) {
+#pragma unroll
+  for (int q = 0; q < FFT_U; ++q) {
+    const int t = min(t0 + 256 * q, tot - 1);
+    const int m = t >> lpr, ka = k0 + 2 * (t & (pairs - 1));
     const cplx<T>* col = S + ((size_t)m * ny + j) * nz;
     if constexpr (sizeof(T) == 8) {
       // the spectrum's last read of the step: non-temporal in float64 (measured: step -1 %;
@@ -197,44 +184,48 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
       typedef T v2 __attribute__((ext_vector_type(2)));
       const v2 a2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka, nz - 1)));
       const v2 b2 = __builtin_nontemporal_load(reinterpret_cast<const v2*>(col + min(ka + 1, nz - 1)));
-      A = {a2.x, a2.y};
-      B = {b2.x, b2.y};
+      va[q] = {a2.x, a2.y};
+      vb[q] = {b2.x, b2.y};
     } else {
-      A = col[min(ka, nz - 1)];
-      B = col[min(ka + 1, nz - 1)];
+      va[q] = col[min(ka, nz - 1)];
+      vb[q] = col[min(ka + 1, nz - 1)];
     }
-  };
+  }
+}
+
+// Slots of inv_x_fetch into the block's LDS sequences: pair qq, input x = A + iB at bin m and its
+// conjugate-symmetric image at nx - m (rows ka >= nz are zero, the imaginary parts of bins 0 and
+// nx/2 are dropped); slots past the end go to the spare entry.
+template <typename T>
+__device__ __forceinline__ void inv_x_stage(cplx<T>* Z, int ldb, int nx, int nz, int k0, int lpr, int pairs,
+                                            int tot, int spare, int t0, const cplx<T> (&va)[FFT_U],
+                                            const cplx<T> (&vb)[FFT_U]) {
+  const int h = nx / 2;
+#pragma unroll
+  for (int q = 0; q < FFT_U; ++q) {
+    const int t = t0 + 256 * q;
+    const int qq = t & (pairs - 1), m = t >> lpr;
+    const int ka = k0 + 2 * qq, kb = ka + 1;
+    cplx<T> A = ka < nz ? va[q] : cplx<T>{T(0), T(0)};
+    cplx<T> B = kb < nz ? vb[q] : cplx<T>{T(0), T(0)};
+    if (m == 0 || m == h) { A.im = T(0); B.im = T(0); }
+    const bool in = t < tot;
+    Z[in ? qq * ldb + fpad<T>(m) : spare] = {A.re - B.im, A.im + B.re};
+    if (m > 0 && m < h) Z[in ? qq * ldb + fpad<T>(nx - m) : spare] = {A.re + B.im, B.re - A.im};
+  }
+}
+
+// Rows of the transformed sequences to p: 16 consecutive lanes write 16 consecutive z-rows of one
+// x-row.  Each axis has the cell's index and at most one image (x: face nx -> 0, face 1 -> nx + 1;
+// y likewise; z: Neumann copies 1 -> 0, nz -> nz + 1), so a value goes to at most 8 places.
+template <typename T>
+__device__ __forceinline__ void inv_x_store(const Dims& d, const cplx<T>* Z, int ldb, T* __restrict__ p,
+                                            T scale, int j, int k0, int lpr) {
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int KZ = 2 << lpr;
   const int J = j + 1;
   const int Y1 = J == ny ? 0 : (J == 1 ? ny + 1 : -1);
   const int sxp = d.sxp, syp = d.syp;
-  // loads ahead of stores, as in fwd_x_body
-  const int tot = (h + 1) * pairs;
-  const int spare = pairs * ldb + nx;
-  for (int t0 = threadIdx.x; t0 < tot; t0 += 256 * FFT_U) {
-    cplx<T> va[FFT_U], vb[FFT_U];
-#pragma unroll
-    for (int q = 0; q < FFT_U; ++q) {
-      const int t = min(t0 + 256 * q, tot - 1);
-      bins(t >> lpr, k0 + 2 * (t & (pairs - 1)), va[q], vb[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < FFT_U; ++q) {
-      const int t = t0 + 256 * q;
-      const int qq = t & (pairs - 1), m = t >> lpr;
-      const int ka = k0 + 2 * qq, kb = ka + 1;
-      cplx<T> A = ka < nz ? va[q] : cplx<T>{T(0), T(0)};
-      cplx<T> B = kb < nz ? vb[q] : cplx<T>{T(0), T(0)};
-      if (m == 0 || m == h) { A.im = T(0); B.im = T(0); }
-      const bool in = t < tot;
-      Z[in ? qq * ldb + fpad<T>(m) : spare] = {A.re - B.im, A.im + B.re};
-      if (m > 0 && m < h) Z[in ? qq * ldb + fpad<T>(nx - m) : spare] = {A.re + B.im, B.re - A.im};
-    }
-  }
-  __syncthreads();
-  lds_fft<T>(Z, ldb, pairs, fx, twl, +1);
-  // 16 consecutive lanes write 16 consecutive z-rows of one x-row.  Each axis has the cell's index
-  // and at most one image (x: face nx -> 0, face 1 -> nx + 1; y likewise; z: Neumann copies
-  // 1 -> 0, nz -> nz + 1), so a value goes to at most 8 places.
   for (int t = threadIdx.x; t < nx * KZ; t += 256) {
     const int kk = t & (KZ - 1), i = t >> (lpr + 1);
     const int k = k0 + kk;
@@ -259,6 +250,43 @@ __device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const doub
       }
     }
   }
+}
+
+// P5: half spectra -> real rows along x (the x half of projection_fft.py:139's irfft2: the
+// imaginary parts of the kx = 0 and Nyquist bins are dropped), scaled by 1/(nx*ny), written to p
+// with the ghost images of projection_fft.py:146-163 (x copies, y copies, Neumann z walls).  Block
+// task = (y row j, KZ = 2*pairs z-rows).  Two real z-rows ka, ka + 1 travel as one complex
+// sequence (pair qq): input x of the pair is A + iB at bin m = x for x <= nx/2 and its
+// conjugate-symmetric image at bin m = nx - x above, with A, B the rows' bins; output x holds row
+// ka in its real and row ka + 1 in its imaginary part.
+template <typename T>
+__device__ __forceinline__ void inv_x_body(Dims d, const FftDesc& fx, const double2* __restrict__ twx,
+                                               const cplx<T>* __restrict__ S, T* __restrict__ p,
+                                               T scale, int pairs,
+    int blk) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cplx<T>* Z = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  const int KZ = 2 * pairs;
+  const int nch = (nz + KZ - 1) / KZ;
+  const int j = blk / nch, k0 = (blk - j * nch) * KZ;
+  const int ldb = fft_ld<T>(nx);
+  const int lpr = 31 - __clz(pairs);   // pairs is a power of two (checked by the launchers)
+  cplx<T>* twl = Z + pairs * ldb;
+  // loads ahead of stores, as in fwd_x_body; the twiddles are staged while the first slots' bins
+  // are in flight
+  const int tot = (nx / 2 + 1) * pairs;
+  const int spare = pairs * ldb + nx;
+  cplx<T> va[FFT_U], vb[FFT_U];
+  inv_x_fetch<T>(S, ny, nz, j, k0, lpr, pairs, tot, threadIdx.x, va, vb);
+  fft_stage_twiddles<T>(twl, twx, nx);
+  for (int t0 = threadIdx.x; t0 < tot; t0 += 256 * FFT_U) {
+    if (t0 != (int)threadIdx.x) inv_x_fetch<T>(S, ny, nz, j, k0, lpr, pairs, tot, t0, va, vb);
+    inv_x_stage<T>(Z, ldb, nx, nz, k0, lpr, pairs, tot, spare, t0, va, vb);
+  }
+  __syncthreads();
+  lds_fft<T>(Z, ldb, pairs, fx, twl, +1);
+  inv_x_store<T>(d, Z, ldb, p, scale, j, k0, lpr);
 }
 
 template <typename T>
