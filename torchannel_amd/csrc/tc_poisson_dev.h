@@ -215,38 +215,63 @@ __device__ __forceinline__ void inv_x_stage(cplx<T>* Z, int ldb, int nx, int nz,
   }
 }
 
-// Rows of the transformed sequences to p: 16 consecutive lanes write 16 consecutive z-rows of one
-// x-row.  Each axis has the cell's index and at most one image (x: face nx -> 0, face 1 -> nx + 1;
-// y likewise; z: Neumann copies 1 -> 0, nz -> nz + 1), so a value goes to at most 8 places.
+// Rows of the transformed sequences to p, two z-rows per thread: pair qq of x-row i holds row
+// ka = k0 + 2 qq in its real part and row ka + 1 in its imaginary part, written together as one
+// two-element vector (element-aligned: row ka + 1 sits at an odd offset in the column), so the
+// pairs of consecutive lanes cover one x-row's KZ rows.  Each axis has the cell's index and at
+// most one image (x: face nx -> 0, face 1 -> nx + 1; y likewise; z: Neumann copies 1 -> 0,
+// nz -> nz + 1), so a value goes to at most 8 places.
 template <typename T>
 __device__ __forceinline__ void inv_x_store(const Dims& d, const cplx<T>* Z, int ldb, T* __restrict__ p,
                                             T scale, int j, int k0, int lpr) {
+  typedef T v2 __attribute__((ext_vector_type(2)));
+  typedef v2 v2e __attribute__((aligned(sizeof(T))));
   const int nx = d.nx, ny = d.ny, nz = d.nz;
-  const int KZ = 2 << lpr;
+  const int pairs = 1 << lpr;
   const int J = j + 1;
   const int Y1 = J == ny ? 0 : (J == 1 ? ny + 1 : -1);
   const int sxp = d.sxp, syp = d.syp;
-  for (int t = threadIdx.x; t < nx * KZ; t += 256) {
-    const int kk = t & (KZ - 1), i = t >> (lpr + 1);
-    const int k = k0 + kk;
-    if (k >= nz) continue;
-    const cplx<T> z = Z[(kk >> 1) * ldb + fpad<T>(i)];
-    const T val = ((kk & 1) ? z.im : z.re) * scale;
-    const int I = i + 1, K = k + 1;
+  for (int t = threadIdx.x; t < nx * pairs; t += 256) {
+    const int qq = t & (pairs - 1), i = t >> lpr;
+    const int ka = k0 + 2 * qq;
+    if (ka >= nz) continue;
+    const cplx<T> z = Z[qq * ldb + fpad<T>(i)];
+    const T a = z.re * scale, b = z.im * scale;
+    const bool two = ka + 1 < nz;
+    const int I = i + 1, K = ka + 1;
     const int X1 = I == nx ? 0 : (I == 1 ? nx + 1 : -1);
-    const int Z1 = K == 1 ? 0 : (K == nz ? nz + 1 : -1);
-    p[I * sxp + J * syp + K] = val;
-    if (X1 >= 0) p[X1 * sxp + J * syp + K] = val;
+    // the column (xi, yj): rows K and K + 1, or row K alone at an odd nz's last row
+    auto put = [&](int xi, int yj) {
+      T* q = p + xi * sxp + yj * syp + K;
+      if (two) *reinterpret_cast<v2e*>(q) = v2{a, b};
+      else q[0] = a;
+    };
+    put(I, J);
+    if (X1 >= 0) put(X1, J);
     if (Y1 >= 0) {
-      p[I * sxp + Y1 * syp + K] = val;
-      if (X1 >= 0) p[X1 * sxp + Y1 * syp + K] = val;
+      put(I, Y1);
+      if (X1 >= 0) put(X1, Y1);
     }
-    if (Z1 >= 0) {
-      p[I * sxp + J * syp + Z1] = val;
-      if (X1 >= 0) p[X1 * sxp + J * syp + Z1] = val;
+    // Neumann wall images: row 1 -> 0 (from the pair's first row), row nz -> nz + 1 (from its
+    // second row, or its first where the pair is the odd last row)
+    const bool bot = K == 1;
+    const bool top = two ? K + 1 == nz : K == nz;
+    if (bot || top) {
+      const int zk = bot ? 0 : nz + 1;
+      const T zv = bot ? a : (two ? b : a);
+      p[I * sxp + J * syp + zk] = zv;
+      if (X1 >= 0) p[X1 * sxp + J * syp + zk] = zv;
       if (Y1 >= 0) {
-        p[I * sxp + Y1 * syp + Z1] = val;
-        if (X1 >= 0) p[X1 * sxp + Y1 * syp + Z1] = val;
+        p[I * sxp + Y1 * syp + zk] = zv;
+        if (X1 >= 0) p[X1 * sxp + Y1 * syp + zk] = zv;
+      }
+      if (bot && two && K + 1 == nz) {   // nz = 2: the pair's second row is also the top row
+        p[I * sxp + J * syp + nz + 1] = b;
+        if (X1 >= 0) p[X1 * sxp + J * syp + nz + 1] = b;
+        if (Y1 >= 0) {
+          p[I * sxp + Y1 * syp + nz + 1] = b;
+          if (X1 >= 0) p[X1 * sxp + Y1 * syp + nz + 1] = b;
+        }
       }
     }
   }
