@@ -1024,7 +1024,8 @@ __global__ __launch_bounds__(64 * TC_K1_WAVES) void k_explicit_cn(
 // W = 8 where a wavefront fits 128 VGPRs (one row per lane, or float32 with two), so that two
 // blocks give a CU 4 wavefronts per SIMD; else W = 4 (more VGPRs, three blocks per CU).  Capping
 // the float64 two-row instance at 128 VGPRs spills inside the march (measured: 146 -> 248 us at
-// 192x192x128), so it keeps its 152 VGPRs and 3 wavefronts per SIMD.
+// 192x192x128), so it keeps its 152 VGPRs and 3 wavefronts per SIMD; likewise capping the float32
+// two-row instance at 64 VGPRs for 8 wavefronts per SIMD spills (82 -> 157 us), so it keeps 79.
 template <typename T, int R>
 __host__ __device__ constexpr int k3_waves() { return (R == 1 || (sizeof(T) == 4 && R == 2)) ? 8 : 4; }
 template <typename T, int R>
