@@ -256,7 +256,7 @@ def test_poisson(tc, shape):
 
 
 @pytest.mark.parametrize("shape", [(16, 12, 40), (10, 8, 129), (16, 12, 200), (8, 6, 256), (320, 6, 18),
-                                   (16, 12, 128), (12, 10, 64), (48, 96, 32), (4, 1536, 6)])
+                                   (16, 12, 128), (12, 10, 64), (48, 96, 32), (4, 1536, 6), (384, 4, 40)])
 def test_poisson_fp32(tc, shape):
     """The float32 solve (float64 gauge mode) against the float64 oracle: float32 round-off
     amplified by the low-wavenumber systems stays well below 1e-4."""

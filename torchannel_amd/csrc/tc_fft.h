@@ -11,6 +11,8 @@
 // budget of the forward x transform of long rows (nx > 256): twice the z rows per block, so each
 // x row is read in segments twice as long; its passes hold twice the registers
 #define TC_FFT_ELEMS_WIDE 4096
+// float32 rows of more than 256: twice again, so each x row is read in whole 128-byte lines
+#define TC_FFT_ELEMS_WIDE2 8192
 #define TC_FFT_MAXPASS 12
 
 // LDS slot of element x of a sequence.  float64 complex (16 bytes): one pad slot after every 8

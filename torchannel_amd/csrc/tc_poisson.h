@@ -21,7 +21,8 @@ struct Poisson {
   void* spec = nullptr;        // S[nkx][ny][nz] complex, working precision
   int pairs = 1;               // z-row pairs per x-transform block
   int pairs_fx = 1;            // ... of the solve's forward x transform
-  bool wide_fx = false;        // that transform runs with the TC_FFT_ELEMS_WIDE budget
+  int wide_fx = 0;             // that transform's budget: 0 TC_FFT_ELEMS, 1 TC_FFT_ELEMS_WIDE,
+                               // 2 TC_FFT_ELEMS_WIDE2 (float32 only)
   int kzy = 1;                 // z-rows per y-transform block
   double* th_cb = nullptr;     // Thomas c' of the row before each lane's first row of the
                                // wall-normal solve, [system][lanes] (0 for lane 0)

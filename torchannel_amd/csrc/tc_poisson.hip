@@ -256,14 +256,25 @@ int poisson_init(Poisson* pl, const Grid* g, const double* dzf_host, const doubl
   // instead of 8 (the inverse transform and the y transforms keep the narrow budget, which needs
   // half the registers)
   pl->pairs_fx = pl->pairs;
-  pl->wide_fx = false;
+  pl->wide_fx = 0;
   {
+    const int zcap = pow2_floor(std::max(1, (d.nz + 1) / 2));
     int pw = std::min(8, pow2_floor(std::max(1, TC_FFT_ELEMS_WIDE / d.nx)));
-    pw = std::min(pw, pow2_floor(std::max(1, (d.nz + 1) / 2)));
+    pw = std::min(pw, zcap);
     const size_t lw = g->dtype == TC_F64 ? fft_lds<double>(pw, d.nx) : fft_lds<float>(pw, d.nx);
     if (pw > pl->pairs && lw <= 65536) {
       pl->pairs_fx = pw;
-      pl->wide_fx = true;
+      pl->wide_fx = 1;
+    }
+    // float32 rows of more than 256: 16 pairs, each x row read in 32-row (128-byte) runs
+    // (384x384x256: P1 202 -> 190 us; at 192x192x128 the same budget was slower, 21.5 -> 24.7 us,
+    // and these rows keep the narrow one)
+    if (g->dtype == TC_F32 && d.nx > 256) {
+      const int p2w = std::min(std::min(16, pow2_floor(std::max(1, TC_FFT_ELEMS_WIDE2 / d.nx))), zcap);
+      if (p2w > pl->pairs_fx && fft_lds<float>(p2w, d.nx) <= 65536) {
+        pl->pairs_fx = p2w;
+        pl->wide_fx = 2;
+      }
     }
   }
   // twiddles exp(-2 pi i t / N) in float64
@@ -345,17 +356,19 @@ void poisson_free(Poisson* pl) {
   hipFree(pl->th_cb);
 }
 
-// pairs * nx must fit the budget the kernel is built for (TC_FFT_ELEMS, or TC_FFT_ELEMS_WIDE
-// with `wide`)
+// pairs * nx must fit the budget the kernel is built for (wide 0: TC_FFT_ELEMS, 1:
+// TC_FFT_ELEMS_WIDE, 2: TC_FFT_ELEMS_WIDE2, float32 only)
 template <typename T>
 int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* v, const void* w,
-                   const void* rin, int nz_in, const double* scal, void* spec, int pairs, bool wide,
+                   const void* rin, int nz_in, const double* scal, void* spec, int pairs, int wide,
                    hipStream_t st) {
   const Grid* g = pl->g;
   const Dims& d = g->d;
   const int KZ = 2 * pairs;
   if (pairs < 1 || (pairs & (pairs - 1)) != 0) return TC_ERR_ARG;
-  if (pairs * d.nx > (wide ? TC_FFT_ELEMS_WIDE : TC_FFT_ELEMS)) return TC_ERR_ARG;
+  if (wide == 2 && sizeof(T) != 4) return TC_ERR_ARG;
+  if (pairs * d.nx > (wide == 2 ? TC_FFT_ELEMS_WIDE2 : wide == 1 ? TC_FFT_ELEMS_WIDE : TC_FFT_ELEMS))
+    return TC_ERR_ARG;
   const int blocks = d.ny * ((nz_in + KZ - 1) / KZ);
   const size_t lds = fft_lds<T>(pairs, d.nx);
   const Metric<T> mt = g->metric<T>();
@@ -364,11 +377,21 @@ int launch_fwd_x_t(const Poisson* pl, bool from_vel, const void* u, const void* 
                      (const T*)v, (const T*)w, (const T*)rin, nz_in, scal, pl->fx, pl->twx, (cplx<T>*)spec, \
                      pairs)
   if (from_vel) {
-    if (wide) TC_FWDX(true, TC_FFT_ELEMS_WIDE);
-    else TC_FWDX(true, TC_FFT_ELEMS);
+    if (wide == 2) {
+      if constexpr (sizeof(T) == 4) TC_FWDX(true, TC_FFT_ELEMS_WIDE2);
+    } else if (wide == 1) {
+      TC_FWDX(true, TC_FFT_ELEMS_WIDE);
+    } else {
+      TC_FWDX(true, TC_FFT_ELEMS);
+    }
   } else {
-    if (wide) TC_FWDX(false, TC_FFT_ELEMS_WIDE);
-    else TC_FWDX(false, TC_FFT_ELEMS);
+    if (wide == 2) {
+      if constexpr (sizeof(T) == 4) TC_FWDX(false, TC_FFT_ELEMS_WIDE2);
+    } else if (wide == 1) {
+      TC_FWDX(false, TC_FFT_ELEMS_WIDE);
+    } else {
+      TC_FWDX(false, TC_FFT_ELEMS);
+    }
   }
 #undef TC_FWDX
   TC_CHECK_LAUNCH();
